@@ -1,0 +1,185 @@
+/*
+ * rcgpu.h -- C ABI of librcgpu, the MI355X (gfx950) implementation of rust-compress's JPEG target-quality inner
+ * loop: RGB->YCbCr with chroma downsampling, 8x8 f32 AAN FDCT and quantisation, dequantisation + IDCT back to
+ * 8-bit pixels, and SSIMULACRA2 scoring, driven by the quality search.
+ *
+ * Every entry point takes plain pointers and sizes, returns an rcg_status, never aborts and never unwinds. The
+ * functions without a _dev suffix take HOST buffers (the drop-in for the Rust API, which owns its Vec<u8>/Vec<i16>);
+ * the _dev variants take DEVICE pointers that stay resident in HBM and enqueue on the context's stream.
+ *
+ * Layouts are the reference's own:
+ *   pixels        interleaved Gray8 (channels = 1) or RGB8 (channels = 3), row stride width*channels
+ *                 (rc_core::image::Image::data, core/src/image.rs:104-113);
+ *   coefficients  per component, 64 int16 per 8x8 block in zigzag order, blocks in raster order over the padded
+ *                 block grid (rc_jpeg::encode::CoefComponent::coefs, jpeg/src/encode.rs:56-69), so the Rust side
+ *                 wraps the buffers with no permutation;
+ *   qtables       64 uint8 in natural (row-major) order (CoefImage::qtables, jpeg/src/encode.rs:77-78).
+ *
+ * Thread safety: calls on one context are serialised by the context; use one context per thread for overlap.
+ */
+#ifndef RCGPU_H
+#define RCGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. They map rc_core::error::Error (core/src/error.rs:4-22): Invalid -> RCG_INVALID,
+ * Unsupported -> RCG_UNSUPPORTED; device failures and allocation failures get their own codes. */
+typedef enum {
+    RCG_OK = 0,
+    RCG_INVALID = 1,
+    RCG_UNSUPPORTED = 2,
+    RCG_DEVICE = 3,
+    RCG_OOM = 4
+} rcg_status;
+
+/* rc_jpeg::encode::Subsampling (jpeg/src/encode.rs:12-20), same order. */
+typedef enum { RCG_AUTO = 0, RCG_S444 = 1, RCG_S422 = 2, RCG_S420 = 3 } rcg_subsampling;
+
+/* Arithmetic of the SSIMULACRA2 scorer: FP32 maps with FP64 pooling, or the reference's FP64 arithmetic. */
+typedef enum { RCG_PREC_FP32 = 0, RCG_PREC_FP64 = 1 } rcg_precision;
+
+typedef struct rcg_ctx rcg_ctx;
+
+/* Create a context on HIP device `device` (its own stream and device workspace). */
+int rcg_create(int device, rcg_ctx** out);
+void rcg_destroy(rcg_ctx* ctx);
+/* Message of the last failure on the calling thread ("" if none). */
+const char* rcg_last_error(void);
+/* Library version string. */
+const char* rcg_version(void);
+
+/* Coefficient layout build_coefs produces (jpeg/src/encode.rs:428-443): sampling factors and block grids. */
+typedef struct {
+    uint32_t width, height;
+    uint32_t ncomp;                 /* 1 (Gray) or 3 (YCbCr) */
+    uint32_t hmax, vmax;            /* luma sampling factors; chroma are 1x1 */
+    uint32_t blocks_w[3], blocks_h[3];
+} rcg_layout;
+int rcg_coef_layout(uint32_t width, uint32_t height, uint32_t channels, int quality, int subsampling,
+                    rcg_layout* out);
+
+/* Quantisation tables for `quality` (jpeg/src/quality.rs:6-43): luma then chroma, natural order. */
+int rcg_quant_tables(int quality, uint8_t out[128]);
+
+/*
+ * Replaces rc_jpeg::encode::build_coefs(&Image, &EncodeOptions) -> CoefImage (jpeg/src/encode.rs:426) with
+ * trellis off: padding by edge replication, fixed-point RGB->YCbCr, box downsampling, f32 AAN FDCT and reciprocal
+ * quantisation. coefs[c] receives layout.blocks_w[c]*blocks_h[c]*64 int16 (coefs[1], coefs[2] unused for Gray).
+ * qtables (may be NULL) receives the two tables written to DQT. Bit-exact with the reference.
+ */
+int rcg_build_coefs(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                    int quality, int subsampling, int16_t* coefs[3], uint8_t qtables[128]);
+int rcg_build_coefs_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, uint32_t height, uint32_t channels,
+                        int quality, int subsampling, int16_t* d_coefs[3]);
+
+/* A coefficient image as rc_jpeg::encode::CoefImage holds it (jpeg/src/encode.rs:56-83). */
+typedef struct {
+    uint8_t id;
+    uint8_t h, v;                   /* sampling factors */
+    uint8_t tq;                     /* quantisation table slot 0..3 */
+    uint32_t blocks_w, blocks_h;
+    const int16_t* coefs;           /* blocks_w*blocks_h*64 zigzag int16 */
+} rcg_coef_component;
+typedef struct {
+    uint32_t width, height;
+    uint32_t ncomp;                 /* 1 or 3 (4-component CMYK/YCCK is RCG_UNSUPPORTED) */
+    rcg_coef_component comps[4];
+    const uint8_t* qtables[4];      /* natural-order tables by slot, NULL if absent */
+    int adobe_transform;            /* -1 when the file has no Adobe marker */
+} rcg_coef_image;
+
+/*
+ * Replaces rc_jpeg::decode::reconstruct(&CoefImage) -> Result<Image> (jpeg/src/decode.rs:334): dequantise, f32
+ * AAN IDCT, round and clamp, "fancy" upsampling of subsampled components, fixed-point YCbCr->RGB. pixels receives
+ * width*height*ncomp bytes. RCG_INVALID for a missing table; RCG_UNSUPPORTED for a non-integral sampling ratio
+ * (decode.rs:1314,1410-1412). Bit-exact with the reference.
+ */
+int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels);
+int rcg_reconstruct_dev(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pixels);
+
+/*
+ * Replaces rc_core::ssimulacra2::ssimulacra2(&Image, &Image) -> Result<f64> (core/src/ssimulacra2.rs:471).
+ * RCG_INVALID when a side is below 8 pixels or the score is not finite (ssimulacra2.rs:472-477,511).
+ * per_scale (may be NULL) receives [6][3][6] doubles: per scale, per channel X/Y/B, the SSIM mean and 4-norm,
+ * artifact mean and 4-norm, detail mean and 4-norm; nscales (may be NULL) the number of scales used.
+ */
+int rcg_ssimulacra2(rcg_ctx* ctx, const uint8_t* reference, const uint8_t* distorted, uint32_t width,
+                    uint32_t height, uint32_t channels, int precision, double* score, double* per_scale,
+                    int* nscales);
+int rcg_ssimulacra2_dev(rcg_ctx* ctx, const uint8_t* d_reference, const uint8_t* d_distorted, uint32_t width,
+                        uint32_t height, uint32_t channels, int precision, double* score, double* per_scale,
+                        int* nscales);
+
+/* Result of a quality search: rc_jpeg::target::Found (jpeg/src/target.rs:11-16) without the coefficients, which
+ * the caller gets through the coefs argument. */
+typedef struct {
+    int quality;
+    double score;                   /* score of the chosen quality (FP64 when it was within 0.05 of the target) */
+    int steps;                      /* candidates evaluated */
+} rcg_found;
+
+/*
+ * Replaces rc_jpeg::target::find_quality_with(reference, target, max_quality, make, metric) as the CLI wires it
+ * for --target-ssimulacra2 (cli/src/pipeline.rs:207-247): make = build_coefs(trellis off), metric = ssimulacra2,
+ * serial bisection over 1..=max_quality starting at max_quality (target.rs:37-55). Scores use FP32 maps; a score
+ * within 0.05 of the target is re-scored in FP64 before the comparison, so the choice equals the reference's.
+ * coefs (may be NULL) receive the chosen quality's coefficients, qtables (may be NULL) its tables.
+ * RCG_UNSUPPORTED below 8x8 pixels (pipeline.rs:215-217).
+ */
+int rcg_find_quality(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                     double target, int max_quality, int subsampling, rcg_found* out, int16_t* coefs[3],
+                     uint8_t qtables[128]);
+int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, uint32_t height,
+                         uint32_t channels, double target, int max_quality, int subsampling, rcg_found* out);
+
+/*
+ * Batched candidate scoring: many images of uneven size, each with up to 16 candidate (quality, subsampling)
+ * pairs, scored in one sequence of launches (one encode launch for every image and candidate, one launch per
+ * SSIMULACRA2 scale for every pair, one pooling launch). Candidates of one image that share a subsampling share
+ * its colour conversion and FDCT; the reference image's pyramid is built once per image. The selection follows
+ * find_quality_with (jpeg/src/target.rs:37-55) over the candidate list sorted by quality: the highest candidate
+ * first, then bisection over list indices; chosen_* report that choice and sweep_* the exhaustive one (the lowest
+ * passing candidate). Candidates whose FP32 score lies within 0.05 of the target are re-scored in FP64 first.
+ * Pixels are DEVICE pointers for rcg_score_batch_dev and HOST pointers for rcg_score_batch.
+ * This entry point has no single counterpart in the reference: it replaces the serial find_quality_with loop of
+ * cli/src/pipeline.rs:207-247 run once per file by cli/src/main.rs:57-75.
+ */
+#define RCG_MAX_CAND 16
+typedef struct {
+    const uint8_t* pixels;
+    uint32_t width, height, channels;
+    uint32_t ncand;
+    int quality[RCG_MAX_CAND];
+    int subsampling[RCG_MAX_CAND];
+    double target;
+} rcg_batch_job;
+typedef struct {
+    double scores[RCG_MAX_CAND];    /* per candidate, FP64-settled near the target */
+    int settled[RCG_MAX_CAND];      /* 1 if that score is the FP64 one */
+    int chosen_index, chosen_quality;
+    int sweep_index, sweep_quality;
+    int status;                     /* rcg_status of this job */
+} rcg_batch_result;
+int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
+int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
+
+/* Per-kernel timing with HIP events recorded on the context's stream around every launch. enable = 1 starts
+ * recording (and clears the totals), 0 stops. rcg_profile_get returns the summed milliseconds and the launch count
+ * of one kernel ("k_encode", "k_idct", "k_upsample_color", "k_ssim_scale", "k_ssim_finalize"). */
+int rcg_profile(rcg_ctx* ctx, int enable);
+int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_ms, int* launches);
+
+/* Fill `out` (width*height*3 bytes, host) with the seeded synthetic RGB8 test image: per channel 4 random 2-D
+ * sinusoids (periods 16-256 px, amplitudes 10-60) around 128, 12 hard-edged rectangles, Gaussian noise sigma 4,
+ * rounded and clamped. Deterministic for (seed, width, height) on any thread count. */
+int rcg_synth_rgb8(uint64_t seed, uint32_t width, uint32_t height, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RCGPU_H */

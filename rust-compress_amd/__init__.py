@@ -1,0 +1,378 @@
+"""rust-compress_amd: the JPEG target-quality inner loop of rust-compress on MI355X (gfx950), through librcgpu.so.
+
+This module is the Python view of the C ABI in include/rcgpu.h. It mirrors the reference's Rust API for the path:
+
+  build_coefs(img, quality, subsampling)   rc_jpeg::encode::build_coefs      (jpeg/src/encode.rs:426), trellis off
+  reconstruct(ci)                          rc_jpeg::decode::reconstruct      (jpeg/src/decode.rs:334)
+  ssimulacra2(reference, distorted)        rc_core::ssimulacra2::ssimulacra2 (core/src/ssimulacra2.rs:471)
+  find_quality(img, target, ...)           rc_jpeg::target::find_quality_with as the CLI wires it for
+                                           --target-ssimulacra2 (cli/src/pipeline.rs:207-247)
+
+Images are numpy uint8 arrays, (h, w, 3) for RGB8 or (h, w) for Gray8. Errors raise RcgError with the status code
+and message of the library (Invalid/Unsupported map rc_core::error::Error). There is no CPU fallback: without the
+built library every call raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librcgpu.so")
+
+OK, INVALID, UNSUPPORTED, DEVICE, OOM = 0, 1, 2, 3, 4
+AUTO, S444, S422, S420 = 0, 1, 2, 3
+FP32, FP64 = 0, 1
+
+_STATUS = {INVALID: "invalid", UNSUPPORTED: "unsupported", DEVICE: "device", OOM: "out of memory"}
+
+
+class RcgError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("hmax", ctypes.c_uint32), ("vmax", ctypes.c_uint32), ("blocks_w", ctypes.c_uint32 * 3),
+                ("blocks_h", ctypes.c_uint32 * 3)]
+
+
+class _CoefComponent(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint8), ("h", ctypes.c_uint8), ("v", ctypes.c_uint8), ("tq", ctypes.c_uint8),
+                ("blocks_w", ctypes.c_uint32), ("blocks_h", ctypes.c_uint32), ("coefs", ctypes.c_void_p)]
+
+
+class _CoefImage(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("comps", _CoefComponent * 4), ("qtables", ctypes.c_void_p * 4), ("adobe_transform", ctypes.c_int)]
+
+
+class _Found(ctypes.Structure):
+    _fields_ = [("quality", ctypes.c_int), ("score", ctypes.c_double), ("steps", ctypes.c_int)]
+
+
+MAX_CAND = 16
+
+
+class BatchJob(ctypes.Structure):
+    """rcg_batch_job: one image and its candidate (quality, subsampling) list."""
+    _fields_ = [("pixels", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("channels", ctypes.c_uint32), ("ncand", ctypes.c_uint32), ("quality", ctypes.c_int * MAX_CAND),
+                ("subsampling", ctypes.c_int * MAX_CAND), ("target", ctypes.c_double)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("scores", ctypes.c_double * MAX_CAND), ("settled", ctypes.c_int * MAX_CAND),
+                ("chosen_index", ctypes.c_int), ("chosen_quality", ctypes.c_int), ("sweep_index", ctypes.c_int),
+                ("sweep_quality", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load librcgpu.so (built by __graft_entry__.build() or rust-compress_amd/build.py)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RcgError(DEVICE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, U, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_double
+        L.rcg_create.argtypes = [I, ctypes.POINTER(P)]
+        L.rcg_destroy.argtypes = [P]
+        L.rcg_destroy.restype = None
+        L.rcg_last_error.restype = ctypes.c_char_p
+        L.rcg_version.restype = ctypes.c_char_p
+        L.rcg_coef_layout.argtypes = [U, U, U, I, I, ctypes.POINTER(Layout)]
+        L.rcg_quant_tables.argtypes = [I, P]
+        L.rcg_build_coefs.argtypes = [P, P, U, U, U, I, I, P, P]
+        L.rcg_build_coefs_dev.argtypes = [P, P, U, U, U, I, I, P]
+        L.rcg_reconstruct.argtypes = [P, ctypes.POINTER(_CoefImage), P]
+        L.rcg_reconstruct_dev.argtypes = [P, ctypes.POINTER(_CoefImage), P]
+        L.rcg_ssimulacra2.argtypes = [P, P, P, U, U, U, I, ctypes.POINTER(D), P, ctypes.POINTER(I)]
+        L.rcg_ssimulacra2_dev.argtypes = [P, P, P, U, U, U, I, ctypes.POINTER(D), P, ctypes.POINTER(I)]
+        L.rcg_find_quality.argtypes = [P, P, U, U, U, D, I, I, ctypes.POINTER(_Found), P, P]
+        L.rcg_find_quality_dev.argtypes = [P, P, U, U, U, D, I, I, ctypes.POINTER(_Found)]
+        L.rcg_synth_rgb8.argtypes = [ctypes.c_uint64, U, U, P]
+        L.rcg_score_batch.argtypes = [P, ctypes.POINTER(BatchJob), I, ctypes.POINTER(BatchResult)]
+        L.rcg_score_batch_dev.argtypes = [P, ctypes.POINTER(BatchJob), I, ctypes.POINTER(BatchResult)]
+        L.rcg_profile.argtypes = [P, I]
+        L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != OK:
+        raise RcgError(rc, lib().rcg_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _channels(img):
+    return 1 if img.ndim == 2 else img.shape[2]
+
+
+def synth_rgb8(seed, width, height):
+    """Seeded synthetic RGB8 image (sinusoids + rectangles + Gaussian noise), shape (height, width, 3)."""
+    out = np.empty((height, width, 3), np.uint8)
+    _check(lib().rcg_synth_rgb8(seed, width, height, _ptr(out)))
+    return out
+
+
+def quant_tables(quality):
+    out = np.zeros(128, np.uint8)
+    _check(lib().rcg_quant_tables(quality, _ptr(out)))
+    return out[:64].copy(), out[64:].copy()
+
+
+def coef_layout(width, height, channels, quality, subsampling):
+    L = Layout()
+    _check(lib().rcg_coef_layout(width, height, channels, quality, subsampling, ctypes.byref(L)))
+    return L
+
+
+@dataclass
+class CoefComponent:
+    """rc_jpeg::encode::CoefComponent (jpeg/src/encode.rs:56-69)."""
+    id: int
+    h: int
+    v: int
+    tq: int
+    blocks_w: int
+    coefs: np.ndarray  # int16, 64 zigzag coefficients per block, raster block order
+
+
+@dataclass
+class CoefImage:
+    """rc_jpeg::encode::CoefImage (jpeg/src/encode.rs:71-83) without metadata."""
+    width: int
+    height: int
+    comps: list
+    qtables: list  # natural-order uint8[64] per slot, or None
+    adobe_transform: int = -1
+
+
+@dataclass
+class Found:
+    """rc_jpeg::target::Found (jpeg/src/target.rs:11-16)."""
+    quality: int
+    score: float
+    coefs: CoefImage = None
+    steps: int = 0
+
+
+class Context:
+    """One device context: a HIP stream and a device workspace on `device`."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().rcg_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().rcg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------------ host-buffer drop-in API
+
+    def build_coefs(self, img, quality, subsampling=S420):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape[:2]
+        ch = _channels(img)
+        L = coef_layout(w, h, ch, quality, subsampling)
+        bufs = [np.empty(L.blocks_w[c] * L.blocks_h[c] * 64, np.int16) for c in range(L.ncomp)]
+        ptrs = (ctypes.c_void_p * 3)(*([b.ctypes.data for b in bufs] + [None] * (3 - len(bufs))))
+        qt = np.zeros(128, np.uint8)
+        _check(lib().rcg_build_coefs(self._h, _ptr(img), w, h, ch, quality, subsampling,
+                                     ctypes.cast(ptrs, ctypes.c_void_p), _ptr(qt)))
+        if ch == 1:
+            return CoefImage(w, h, [CoefComponent(1, 1, 1, 0, L.blocks_w[0], bufs[0])],
+                             [qt[:64].copy(), None, None, None])
+        comps = [CoefComponent(1, L.hmax, L.vmax, 0, L.blocks_w[0], bufs[0]),
+                 CoefComponent(2, 1, 1, 1, L.blocks_w[1], bufs[1]),
+                 CoefComponent(3, 1, 1, 1, L.blocks_w[2], bufs[2])]
+        return CoefImage(w, h, comps, [qt[:64].copy(), qt[64:].copy(), None, None])
+
+    def reconstruct(self, ci):
+        keep = []
+        c_ci = _coef_image_struct(ci, keep, device=False)
+        n = len(ci.comps)
+        out = np.empty((ci.height, ci.width) if n == 1 else (ci.height, ci.width, n), np.uint8)
+        _check(lib().rcg_reconstruct(self._h, ctypes.byref(c_ci), _ptr(out)))
+        return out
+
+    def ssimulacra2(self, reference, distorted, precision=FP32, with_stats=False):
+        a = np.ascontiguousarray(reference, dtype=np.uint8)
+        b = np.ascontiguousarray(distorted, dtype=np.uint8)
+        if a.shape != b.shape:
+            raise RcgError(INVALID, "images differ in size")
+        h, w = a.shape[:2]
+        score = ctypes.c_double()
+        stats = np.zeros((6, 3, 6), np.float64)
+        ns = ctypes.c_int()
+        _check(lib().rcg_ssimulacra2(self._h, _ptr(a), _ptr(b), w, h, _channels(a), precision,
+                                     ctypes.byref(score), _ptr(stats), ctypes.byref(ns)))
+        if with_stats:
+            return score.value, stats[: ns.value].copy()
+        return score.value
+
+    def find_quality(self, img, target, max_quality=100, subsampling=S420):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape[:2]
+        ch = _channels(img)
+        f = _Found()
+        # The chosen quality's layout is only known afterwards; size for the larger 4:4:4 one when Auto.
+        L = coef_layout(w, h, ch, 100, S444 if subsampling == AUTO else subsampling)
+        bufs = [np.zeros(L.blocks_w[c] * L.blocks_h[c] * 64, np.int16) for c in range(L.ncomp)]
+        ptrs = (ctypes.c_void_p * 3)(*([b.ctypes.data for b in bufs] + [None] * (3 - len(bufs))))
+        qt = np.zeros(128, np.uint8)
+        _check(lib().rcg_find_quality(self._h, _ptr(img), w, h, ch, float(target), max_quality, subsampling,
+                                      ctypes.byref(f), ctypes.cast(ptrs, ctypes.c_void_p), _ptr(qt)))
+        Lq = coef_layout(w, h, ch, f.quality, subsampling)
+        if ch == 1:
+            comps = [CoefComponent(1, 1, 1, 0, Lq.blocks_w[0], bufs[0][: Lq.blocks_w[0] * Lq.blocks_h[0] * 64])]
+            qts = [qt[:64].copy(), None, None, None]
+        else:
+            comps = [CoefComponent(1 + c, Lq.hmax if c == 0 else 1, Lq.vmax if c == 0 else 1, min(c, 1),
+                                   Lq.blocks_w[c], bufs[c][: Lq.blocks_w[c] * Lq.blocks_h[c] * 64]) for c in range(3)]
+            qts = [qt[:64].copy(), qt[64:].copy(), None, None]
+        return Found(f.quality, f.score, CoefImage(w, h, comps, qts), f.steps)
+
+    # ------------------------------------------------------------------ batched scoring
+
+    def score_batch(self, images, candidates, targets, device_ptrs=None):
+        """Score every image against its candidate list [(quality, subsampling), ...] in one launch sequence.
+
+        images: list of numpy arrays (host) or, with device_ptrs, a list of (ptr, width, height, channels).
+        Returns a list of BatchResult (scores, settled flags, chosen_* = bisection replay, sweep_* = exhaustive)."""
+        n = len(images) if device_ptrs is None else len(device_ptrs)
+        jobs = (BatchJob * n)()
+        keep = []
+        for i in range(n):
+            cand = candidates[i] if isinstance(candidates[0], (list, tuple)) and isinstance(candidates[0][0], (list, tuple)) else candidates
+            if device_ptrs is None:
+                img = np.ascontiguousarray(images[i], dtype=np.uint8)
+                keep.append(img)
+                jobs[i].pixels = img.ctypes.data
+                jobs[i].height, jobs[i].width = img.shape[:2]
+                jobs[i].channels = _channels(img)
+            else:
+                ptr, w, h, ch = device_ptrs[i]
+                jobs[i].pixels = ptr
+                jobs[i].width, jobs[i].height, jobs[i].channels = w, h, ch
+            jobs[i].ncand = len(cand)
+            for k, (q, sub) in enumerate(cand):
+                jobs[i].quality[k] = q
+                jobs[i].subsampling[k] = sub
+            jobs[i].target = float(targets[i] if isinstance(targets, (list, tuple, np.ndarray)) else targets)
+        res = (BatchResult * n)()
+        fn = lib().rcg_score_batch if device_ptrs is None else lib().rcg_score_batch_dev
+        _check(fn(self._h, jobs, n, res))
+        return list(res)
+
+    # ------------------------------------------------------------------ profiling
+
+    def profile(self, enable):
+        _check(lib().rcg_profile(self._h, 1 if enable else 0))
+
+    def profile_get(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(lib().rcg_profile_get(self._h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    # ------------------------------------------------------------------ device-resident API (raw device pointers)
+
+    def build_coefs_dev(self, d_pixels, width, height, channels, quality, subsampling, d_coefs):
+        ptrs = (ctypes.c_void_p * 3)(*(list(d_coefs) + [None] * (3 - len(d_coefs))))
+        _check(lib().rcg_build_coefs_dev(self._h, d_pixels, width, height, channels, quality, subsampling,
+                                         ctypes.cast(ptrs, ctypes.c_void_p)))
+
+    def ssimulacra2_dev(self, d_ref, d_dist, width, height, channels, precision=FP32):
+        score = ctypes.c_double()
+        ns = ctypes.c_int()
+        _check(lib().rcg_ssimulacra2_dev(self._h, d_ref, d_dist, width, height, channels, precision,
+                                         ctypes.byref(score), None, ctypes.byref(ns)))
+        return score.value
+
+    def find_quality_dev(self, d_pixels, width, height, channels, target, max_quality=100, subsampling=S420):
+        f = _Found()
+        _check(lib().rcg_find_quality_dev(self._h, d_pixels, width, height, channels, float(target), max_quality,
+                                          subsampling, ctypes.byref(f)))
+        return Found(f.quality, f.score, None, f.steps)
+
+
+def _coef_image_struct(ci, keep, device):
+    s = _CoefImage()
+    s.width = ci.width
+    s.height = ci.height
+    s.ncomp = len(ci.comps)
+    for i, c in enumerate(ci.comps):
+        s.comps[i].id = c.id
+        s.comps[i].h = c.h
+        s.comps[i].v = c.v
+        s.comps[i].tq = c.tq
+        s.comps[i].blocks_w = c.blocks_w
+        if device:
+            s.comps[i].blocks_h = c.blocks_h
+            s.comps[i].coefs = c.coefs
+        else:
+            co = np.ascontiguousarray(c.coefs, dtype=np.int16)
+            keep.append(co)
+            s.comps[i].blocks_h = len(co) // 64 // c.blocks_w
+            s.comps[i].coefs = co.ctypes.data
+    for slot, q in enumerate(ci.qtables):
+        if q is not None:
+            qa = np.ascontiguousarray(q, dtype=np.uint8)
+            keep.append(qa)
+            s.qtables[slot] = qa.ctypes.data
+    s.adobe_transform = ci.adobe_transform
+    return s
+
+
+_DEFAULT = None
+
+
+def default_context():
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+def build_coefs(img, quality, subsampling=S420):
+    return default_context().build_coefs(img, quality, subsampling)
+
+
+def reconstruct(ci):
+    return default_context().reconstruct(ci)
+
+
+def ssimulacra2(reference, distorted, precision=FP32):
+    return default_context().ssimulacra2(reference, distorted, precision)
+
+
+def find_quality(img, target, max_quality=100, subsampling=S420):
+    return default_context().find_quality(img, target, max_quality, subsampling)

@@ -1,0 +1,836 @@
+// C ABI of librcgpu (include/rcgpu.h): context management, layouts and tables, and the drop-in entry points for
+// build_coefs, reconstruct, ssimulacra2 and the target-quality search, in host-buffer and device-resident forms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "rcg_host.h"
+
+namespace rcg {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int Arena::ensure(size_t bytes) {
+    if (bytes <= cap) return RCG_OK;
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, size_t(64) << 20);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), want);
+    if (e != hipSuccess) {
+        base = nullptr;
+        set_error(std::string("device workspace of ") + std::to_string(want) + " bytes: " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RCG_OOM : RCG_DEVICE;
+    }
+    cap = want;
+    return RCG_OK;
+}
+
+int Pinned::ensure(size_t bytes) {
+    if (bytes <= cap) return RCG_OK;
+    if (base) (void)hipHostFree(base);
+    base = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, size_t(1) << 20);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&base), want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        base = nullptr;
+        set_error(std::string("pinned staging: ") + hipGetErrorString(e));
+        return RCG_OOM;
+    }
+    cap = want;
+    return RCG_OK;
+}
+
+// jpeg/src/quality.rs:26-35
+static const uint8_t kPerceptualQ[64] = {
+    16, 16, 16, 18,  25,  37,  56,  85,  16, 17, 20,  27,  34,  40,  53,  75,  16, 20, 24,  31,  43,  62,
+    91, 135, 18, 27, 31,  40,  53,  74,  106, 156, 25, 34, 43,  53,  69,  94,  131, 189, 37, 40, 62,  74,
+    94, 124, 169, 238, 56, 53, 91,  106, 131, 169, 226, 255, 85, 75, 135, 156, 189, 238, 255, 255};
+
+// jpeg/src/quality.rs:6-14 (IJG scaling); luma and chroma scale the same base (quality.rs:37-43).
+void quant_tables(int quality, uint8_t luma[64], uint8_t chroma[64]) {
+    const unsigned q = unsigned(std::min(100, std::max(1, quality)));
+    const unsigned scale = q < 50 ? 5000 / q : 200 - 2 * q;
+    for (int i = 0; i < 64; i++) {
+        unsigned t = (unsigned(kPerceptualQ[i]) * scale + 50) / 100;
+        t = std::min(255u, std::max(1u, t));
+        luma[i] = chroma[i] = uint8_t(t);
+    }
+}
+
+static const float kAan[8] = {1.0f, 1.3870398f, 1.306563f, 1.1758756f, 1.0f, 0.78569495f, 0.5411961f, 0.27589938f};
+
+// Divisors (jpeg/src/encode.rs:175-183) and dequantisation multipliers (jpeg/src/decode.rs:1317-1322), f32 in the
+// reference's order; this file is compiled with -ffp-contract=off.
+static void tables_for(const uint8_t q[64], float div[64], float mult[64]) {
+    for (int v = 0; v < 8; v++)
+        for (int u = 0; u < 8; u++) {
+            div[u * 8 + v] = 1.0f / (float(q[v * 8 + u]) * kAan[u] * kAan[v] * 8.0f);
+            mult[v * 8 + u] = float(q[v * 8 + u]) * kAan[u] * kAan[v];
+        }
+}
+
+void fill_cand_tables(const uint8_t qy[64], const uint8_t qc[64], EncCand& c) {
+    tables_for(qy, c.div[0], c.mult[0]);
+    tables_for(qc, c.div[1], c.mult[1]);
+}
+
+// jpeg/src/encode.rs:428-443
+Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int subsampling) {
+    Layout L{};
+    L.w = w;
+    L.h = h;
+    L.channels = channels;
+    if (channels == 1) {
+        L.hmax = L.vmax = 1;
+    } else {
+        switch (subsampling) {
+            case RCG_S420: L.hmax = 2; L.vmax = 2; break;
+            case RCG_S422: L.hmax = 2; L.vmax = 1; break;
+            case RCG_S444: L.hmax = 1; L.vmax = 1; break;
+            default:
+                L.hmax = L.vmax = quality < 90 ? 2 : 1;
+        }
+    }
+    const uint32_t mw = 8 * L.hmax, mh = 8 * L.vmax;
+    L.pw = (w + mw - 1) / mw * mw;
+    L.ph = (h + mh - 1) / mh * mh;
+    L.ncomp = channels == 1 ? 1 : 3;
+    L.bw[0] = L.pw / 8;
+    L.bh[0] = L.ph / 8;
+    for (int c = 1; c < 3; c++) {
+        L.bw[c] = channels == 1 ? 0 : L.pw / L.hmax / 8;
+        L.bh[c] = channels == 1 ? 0 : L.ph / L.vmax / 8;
+    }
+    return L;
+}
+
+PairPlan plan_pair(uint32_t w, uint32_t h, int nt, int th) {
+    PairPlan P{};
+    P.w = w;
+    P.h = h;
+    uint32_t sw = w, sh = h;
+    for (int s = 0; s < 6; s++) {
+        if (sw < 8 || sh < 8) break;
+        P.sw[s] = sw;
+        P.sh[s] = sh;
+        P.tiles_x[s] = ssim_tiles_x(sw, nt);
+        P.tiles_y[s] = ssim_tiles_y(sh, th);
+        P.tiles[s] = P.tiles_x[s] * P.tiles_y[s];
+        P.nscales++;
+        sw = (sw + 1) / 2;
+        sh = (sh + 1) / 2;
+    }
+    return P;
+}
+
+}  // namespace rcg
+
+using namespace rcg;
+
+namespace {
+
+bool valid_channels(uint32_t c) { return c == 1 || c == 3; }
+
+// sRGB -> linear table of core/src/ssimulacra2.rs:173-176,191: f64 pow, stored as f32.
+void make_lut(float lut[256]) {
+    for (int v = 0; v < 256; v++) {
+        const double x = double(v) / 255.0;
+        const double l = x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4);
+        lut[v] = float(l);
+    }
+}
+
+// core/src/ssimulacra2.rs:261-271
+void make_taps(double out[6]) {
+    double k[11], sum = 0.0;
+    for (int i = 0; i < 11; i++) {
+        const double x = double(i) - 5.0;
+        k[i] = std::exp(-0.5 / (1.5 * 1.5) * x * x);
+        sum += k[i];
+    }
+    for (int i = 0; i < 11; i++) k[i] /= sum;
+    for (int t = 0; t <= 5; t++) out[t] = k[5 + t];
+}
+
+// Bytes one SSIMULACRA2 evaluation of a w x h pair needs in the arena (pyramid of both images, partials).
+size_t ssim_bytes(const PairPlan& P, bool cache_a) {
+    size_t b = 0;
+    for (uint32_t s = 1; s < P.nscales; s++) {
+        const size_t n = size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float);
+        b += Arena::align(n) * 2;  // both images (the reference pyramid is kept even when cached)
+        (void)cache_a;
+    }
+    for (uint32_t s = 0; s < P.nscales; s++) b += Arena::align(size_t(P.tiles[s]) * 18 * sizeof(double));
+    b += 4096;
+    return b;
+}
+
+struct SsimBuffers {
+    float* a_planes[6] = {};
+    float* b_planes[6] = {};
+    double* partials[6] = {};
+    double* d_score = nullptr;
+    double* d_stats = nullptr;
+};
+
+SsimBuffers take_ssim(Arena& A, const PairPlan& P) {
+    SsimBuffers B;
+    for (uint32_t s = 1; s < P.nscales; s++) {
+        const size_t n = size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float);
+        B.a_planes[s] = static_cast<float*>(A.take(n));
+        B.b_planes[s] = static_cast<float*>(A.take(n));
+    }
+    for (uint32_t s = 0; s < P.nscales; s++)
+        B.partials[s] = static_cast<double*>(A.take(size_t(P.tiles[s]) * 18 * sizeof(double)));
+    B.d_score = static_cast<double*>(A.take(sizeof(double)));
+    B.d_stats = static_cast<double*>(A.take(6 * 18 * sizeof(double)));
+    return B;
+}
+
+// Stage the descriptors for every scale of one pair and its final pooling; launch_staged_ssim() enqueues them after
+// Desc::upload(). a0/b0 are the scale-0 sources. With a_cached, scales >= 1 of `a` read B.a_planes (built by an
+// earlier evaluation of the same reference) instead of writing them.
+struct StagedSsim {
+    const SsimJob* jobs[6];
+    const uint32_t* begins;
+    const FinalDesc* fin;
+    PairPlan P;
+    int precision;
+};
+
+StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimSrc& b0, const SsimBuffers& B,
+                      bool a_cached, int precision) {
+    StagedSsim st;
+    st.P = P;
+    st.precision = precision;
+    const uint32_t zero = 0;
+    st.begins = D.put(&zero, 1);
+    for (uint32_t s = 0; s < P.nscales; s++) {
+        SsimJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.w = P.sw[s];
+        j.h = P.sh[s];
+        j.tiles_x = P.tiles_x[s];
+        j.tiles_y = P.tiles_y[s];
+        j.tile_begin = 0;
+        if (s == 0) {
+            j.a = a0;
+            j.b = b0;
+        } else {
+            j.a.kind = kSrcPlanes;
+            j.a.planes = B.a_planes[s];
+            j.b.kind = kSrcPlanes;
+            j.b.planes = B.b_planes[s];
+        }
+        const bool next = s + 1 < P.nscales;
+        j.a_next = (next && !a_cached) ? B.a_planes[s + 1] : nullptr;
+        j.b_next = next ? B.b_planes[s + 1] : nullptr;
+        j.partials = B.partials[s];
+        st.jobs[s] = D.put(&j, 1);
+    }
+    FinalDesc f;
+    std::memset(&f, 0, sizeof(f));
+    f.nscales = P.nscales;
+    for (uint32_t s = 0; s < P.nscales; s++) {
+        f.ntiles[s] = P.tiles[s];
+        f.partials[s] = B.partials[s];
+        f.count[s] = double(size_t(P.sw[s]) * P.sh[s]);
+    }
+    f.stats_out = B.d_stats;
+    f.score_out = B.d_score;
+    st.fin = D.put(&f, 1);
+    return st;
+}
+
+int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
+    const SsimTune& T = st.precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
+    for (uint32_t s = 0; s < st.P.nscales; s++)
+        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], st.begins, 1, st.P.tiles[s], T.nt, T.th));
+    return launch_finalize(ctx, st.fin, 1);
+}
+
+constexpr size_t kDescBytes = 64 << 10;
+
+// Begin a call: lock the context, size the arena and the descriptor staging.
+struct Call {
+    rcg_ctx* ctx;
+    std::lock_guard<std::mutex> lock;
+    Desc D;
+    explicit Call(rcg_ctx* c) : ctx(c), lock(c->mu) {}
+    int begin(size_t arena_bytes) {
+        RCG_HIP(hipSetDevice(ctx->device));
+        ctx->arena.reset();
+        RCG_TRY(ctx->arena.ensure(arena_bytes + kDescBytes + 4096));
+        RCG_TRY(ctx->pinned.ensure(kDescBytes));
+        D.host = ctx->pinned.base;
+        D.dev = static_cast<char*>(ctx->arena.take(kDescBytes));
+        D.cap = kDescBytes;
+        D.off = 0;
+        return RCG_OK;
+    }
+};
+
+// Encode (and optionally reconstruct) one image at one quality on the device.
+EncJob make_enc_job(const Layout& L, const uint8_t* d_src) {
+    EncJob j;
+    std::memset(&j, 0, sizeof(j));
+    j.src = d_src;
+    j.src_bytes = uint64_t(L.w) * L.h * L.channels;
+    j.w = L.w;
+    j.h = L.h;
+    j.channels = L.channels;
+    j.hmax = L.hmax;
+    j.vmax = L.vmax;
+    j.pw = L.pw;
+    j.ph = L.ph;
+    j.tiles_x = (L.pw + kEncTileW - 1) / kEncTileW;
+    j.tiles_y = L.ph / (8 * L.vmax);
+    j.tile_begin = 0;
+    return j;
+}
+
+int stage_and_launch_encode(rcg_ctx* ctx, Desc& D, const Layout& L, const uint8_t* d_src, int quality,
+                            int16_t* const d_coefs[3], uint8_t* const d_rec[3]) {
+    EncCand c;
+    std::memset(&c, 0, sizeof(c));
+    uint8_t qy[64], qc[64];
+    quant_tables(quality, qy, qc);
+    fill_cand_tables(qy, qc, c);
+    for (int k = 0; k < 3; k++) {
+        c.coef[k] = d_coefs ? d_coefs[k] : nullptr;
+        c.rec[k] = d_rec ? d_rec[k] : nullptr;
+    }
+    EncJob j = make_enc_job(L, d_src);
+    j.ncand = 1;
+    j.cand = D.put(&c, 1);
+    const uint32_t zero = 0;
+    const uint32_t* db = D.put(&zero, 1);
+    const EncJob* dj = D.put(&j, 1);
+    RCG_TRY(D.upload(ctx->stream));
+    return launch_encode(ctx, dj, db, 1, j.tiles_x * j.tiles_y);
+}
+
+size_t coef_bytes(const Layout& L) {
+    size_t b = 0;
+    for (uint32_t c = 0; c < L.ncomp; c++) b += Arena::align(L.coef_count(c) * sizeof(int16_t));
+    return b;
+}
+size_t plane_bytes(const Layout& L) {
+    size_t b = 0;
+    for (uint32_t c = 0; c < L.ncomp; c++) b += Arena::align(L.plane_bytes(c));
+    return b;
+}
+
+}  // namespace
+
+int rcg::check_image(uint32_t w, uint32_t h, uint32_t channels) {
+    if (!valid_channels(channels)) {
+        set_error("pixels must be Gray8 (1 channel) or RGB8 (3 channels)");
+        return RCG_UNSUPPORTED;
+    }
+    if (w == 0 || h == 0) {
+        set_error("empty image");
+        return RCG_INVALID;
+    }
+    if (w > 65535 || h > 65535) {
+        set_error("JPEG dimensions are limited to 65535 pixels");
+        return RCG_UNSUPPORTED;
+    }
+    return RCG_OK;
+}
+
+// Recon planes of the encode stage seen as reconstruct's sample planes (jpeg/src/decode.rs:1404-1415).
+SsimSrc rcg::recon_src(const Layout& L, uint8_t* const rec[3]) {
+    SsimSrc s;
+    std::memset(&s, 0, sizeof(s));
+    s.kind = kSrcRecon;
+    s.ncomp = L.ncomp;
+    s.rgb_direct = 0;
+    for (uint32_t c = 0; c < L.ncomp; c++) {
+        const uint32_t hc = c == 0 ? L.hmax : 1, vc = c == 0 ? L.vmax : 1;
+        s.comp[c].p = rec[c];
+        s.comp[c].stride = L.bw[c] * 8;
+        s.comp[c].cw = (L.w * hc + L.hmax - 1) / L.hmax;
+        s.comp[c].ch = (L.h * vc + L.vmax - 1) / L.vmax;
+        s.comp[c].fx = L.hmax / hc;
+        s.comp[c].fy = L.vmax / vc;
+    }
+    return s;
+}
+
+SsimSrc rcg::bytes_src(const uint8_t* d, uint32_t channels) {
+    SsimSrc s;
+    std::memset(&s, 0, sizeof(s));
+    s.kind = kSrcBytes;
+    s.channels = channels;
+    s.bytes = d;
+    return s;
+}
+
+SsimSrc rcg::planes_src(const float* planes) {
+    SsimSrc s;
+    std::memset(&s, 0, sizeof(s));
+    s.kind = kSrcPlanes;
+    s.planes = planes;
+    return s;
+}
+
+namespace {
+
+int ssim_pair_dev(rcg_ctx* ctx, Call& call, const uint8_t* d_a, const uint8_t* d_b, uint32_t w, uint32_t h,
+                  uint32_t channels, int precision, double* score, double* per_scale, int* nscales) {
+    const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
+    const PairPlan P = plan_pair(w, h, T.nt, T.th);
+    SsimBuffers B = take_ssim(ctx->arena, P);
+    StagedSsim st = stage_ssim(call.D, P, bytes_src(d_a, channels), bytes_src(d_b, channels), B, false, precision);
+    RCG_TRY(call.D.upload(ctx->stream));
+    RCG_TRY(launch_staged_ssim(ctx, st));
+    double stats[6 * 18];
+    double sc = 0.0;
+    RCG_HIP(hipMemcpyAsync(&sc, B.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipMemcpyAsync(stats, B.d_stats, sizeof(stats), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    if (!std::isfinite(sc)) {
+        set_error("SSIMULACRA 2 produced a non-finite score");
+        return RCG_INVALID;
+    }
+    *score = sc;
+    if (per_scale) {
+        std::memset(per_scale, 0, sizeof(double) * 6 * 18);
+        std::memcpy(per_scale, stats, sizeof(double) * 18 * P.nscales);
+    }
+    if (nscales) *nscales = int(P.nscales);
+    return RCG_OK;
+}
+
+}  // namespace
+
+// ================================================================================================================
+// extern "C"
+
+extern "C" {
+
+const char* rcg_last_error(void) { return g_last_error.c_str(); }
+
+const char* rcg_version(void) { return "rcgpu 0.1 gfx950"; }
+
+int rcg_create(int device, rcg_ctx** out) {
+    if (!out) return RCG_INVALID;
+    *out = nullptr;
+    int n = 0;
+    RCG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        set_error("no such HIP device");
+        return RCG_INVALID;
+    }
+    RCG_HIP(hipSetDevice(device));
+    rcg_ctx* c = new (std::nothrow) rcg_ctx();
+    if (!c) return RCG_OOM;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        delete c;
+        return RCG_DEVICE;
+    }
+    float lut[256];
+    make_lut(lut);
+    e = hipMalloc(reinterpret_cast<void**>(&c->d_lut), sizeof(lut));
+    if (e == hipSuccess) e = hipMemcpy(c->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("lookup table: ") + hipGetErrorString(e));
+        rcg_destroy(c);
+        return RCG_DEVICE;
+    }
+    make_taps(c->taps64);
+    for (int i = 0; i < 6; i++) c->taps32[i] = float(c->taps64[i]);
+    c->tune32.nt = 256;
+    c->tune32.th = 64;
+    c->tune64.nt = 128;
+    c->tune64.th = 64;
+    int rc = upload_weights();
+    if (rc != RCG_OK) {
+        rcg_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return RCG_OK;
+}
+
+void rcg_destroy(rcg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->arena.base) (void)hipFree(ctx->arena.base);
+    if (ctx->pinned.base) (void)hipHostFree(ctx->pinned.base);
+    if (ctx->d_lut) (void)hipFree(ctx->d_lut);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int rcg_coef_layout(uint32_t width, uint32_t height, uint32_t channels, int quality, int subsampling,
+                    rcg_layout* out) {
+    if (!out) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    const Layout L = make_layout(width, height, channels, quality, subsampling);
+    out->width = width;
+    out->height = height;
+    out->ncomp = L.ncomp;
+    out->hmax = L.hmax;
+    out->vmax = L.vmax;
+    for (int c = 0; c < 3; c++) {
+        out->blocks_w[c] = L.bw[c];
+        out->blocks_h[c] = L.bh[c];
+    }
+    return RCG_OK;
+}
+
+int rcg_quant_tables(int quality, uint8_t out[128]) {
+    if (!out) return RCG_INVALID;
+    quant_tables(quality, out, out + 64);
+    return RCG_OK;
+}
+
+int rcg_build_coefs_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, uint32_t height, uint32_t channels,
+                        int quality, int subsampling, int16_t* d_coefs[3]) {
+    if (!ctx || !d_pixels || !d_coefs) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    const Layout L = make_layout(width, height, channels, quality, subsampling);
+    for (uint32_t c = 0; c < L.ncomp; c++)
+        if (!d_coefs[c] || (reinterpret_cast<uintptr_t>(d_coefs[c]) & 15)) {
+            set_error("coefficient buffers must be 16-byte aligned device pointers");
+            return RCG_INVALID;
+        }
+    Call call(ctx);
+    RCG_TRY(call.begin(0));
+    int16_t* co[3] = {d_coefs[0], L.ncomp == 3 ? d_coefs[1] : nullptr, L.ncomp == 3 ? d_coefs[2] : nullptr};
+    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_pixels, quality, co, nullptr));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}
+
+int rcg_build_coefs(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                    int quality, int subsampling, int16_t* coefs[3], uint8_t qtables[128]) {
+    if (!ctx || !pixels || !coefs) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    const Layout L = make_layout(width, height, channels, quality, subsampling);
+    for (uint32_t c = 0; c < L.ncomp; c++)
+        if (!coefs[c]) return RCG_INVALID;
+    const size_t src_bytes = size_t(width) * height * channels;
+    Call call(ctx);
+    RCG_TRY(call.begin(Arena::align(src_bytes) + coef_bytes(L)));
+    uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+    int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < L.ncomp; c++)
+        d_co[c] = static_cast<int16_t*>(ctx->arena.take(L.coef_count(c) * sizeof(int16_t)));
+    RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, d_co, nullptr));
+    for (uint32_t c = 0; c < L.ncomp; c++)
+        RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    if (qtables) quant_tables(quality, qtables, qtables + 64);
+    return RCG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Validate a coefficient image and derive its sample planes (jpeg/src/decode.rs:1314,1404-1415).
+int check_coef_image(const rcg_coef_image* ci, uint32_t& hmax, uint32_t& vmax, int& rgb_direct) {
+    if (!ci || ci->width == 0 || ci->height == 0) return RCG_INVALID;
+    if (ci->ncomp != 1 && ci->ncomp != 3) {
+        set_error("only 1- and 3-component images are reconstructed");
+        return RCG_UNSUPPORTED;
+    }
+    hmax = 1;
+    vmax = 1;
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const rcg_coef_component& k = ci->comps[c];
+        if (k.h == 0 || k.v == 0 || k.tq > 3 || !k.coefs) return RCG_INVALID;
+        hmax = std::max<uint32_t>(hmax, k.h);
+        vmax = std::max<uint32_t>(vmax, k.v);
+    }
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const rcg_coef_component& k = ci->comps[c];
+        if (!ci->qtables[k.tq]) {
+            set_error("missing quantization table");
+            return RCG_INVALID;
+        }
+        if (hmax % k.h || vmax % k.v) {
+            set_error("non-integral chroma subsampling ratio");
+            return RCG_UNSUPPORTED;
+        }
+        const uint32_t cw = (ci->width * k.h + hmax - 1) / hmax, ch = (ci->height * k.v + vmax - 1) / vmax;
+        if (uint64_t(k.blocks_w) * 8 < cw || uint64_t(k.blocks_h) * 8 < ch) {
+            set_error("coefficient grid smaller than the image");
+            return RCG_INVALID;
+        }
+    }
+    rgb_direct = ci->adobe_transform == 0 ||
+                 (ci->adobe_transform < 0 && ci->ncomp == 3 && ci->comps[0].id == 'R' && ci->comps[1].id == 'G' &&
+                  ci->comps[2].id == 'B');
+    return RCG_OK;
+}
+
+void mult_table(const uint8_t q[64], float mult[64]) {
+    for (int v = 0; v < 8; v++)
+        for (int u = 0; u < 8; u++) mult[v * 8 + u] = float(q[v * 8 + u]) * kAan[u] * kAan[v];
+}
+
+// IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device.
+int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[3], uint8_t* d_out,
+                          uint32_t hmax, uint32_t vmax, int rgb_direct) {
+    IdctPlane planes[3];
+    PlaneView views[3];
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const rcg_coef_component& k = ci->comps[c];
+        planes[c].coefs = d_coefs[c];
+        planes[c].blocks_w = k.blocks_w;
+        planes[c].blocks_h = k.blocks_h;
+        planes[c].plane = static_cast<uint8_t*>(ctx->arena.take(size_t(k.blocks_w) * 8 * k.blocks_h * 8));
+        mult_table(ci->qtables[k.tq], planes[c].mult);
+        views[c].p = planes[c].plane;
+        views[c].stride = k.blocks_w * 8;
+        views[c].cw = (ci->width * k.h + hmax - 1) / hmax;
+        views[c].ch = (ci->height * k.v + vmax - 1) / vmax;
+        views[c].fx = hmax / k.h;
+        views[c].fy = vmax / k.v;
+    }
+    RCG_TRY(launch_idct(ctx, planes, int(ci->ncomp)));
+    return launch_upsample_color(ctx, views, int(ci->ncomp), rgb_direct, ci->width, ci->height, d_out);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels) {
+    if (!ctx || !pixels) return RCG_INVALID;
+    uint32_t hmax, vmax;
+    int rgb_direct;
+    RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
+    size_t need = Arena::align(size_t(ci->width) * ci->height * ci->ncomp);
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+        need += Arena::align(nb * 64 * sizeof(int16_t)) + Arena::align(nb * 64);
+    }
+    Call call(ctx);
+    RCG_TRY(call.begin(need));
+    const int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const size_t n = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64 * sizeof(int16_t);
+        int16_t* d = static_cast<int16_t*>(ctx->arena.take(n));
+        RCG_HIP(hipMemcpyAsync(d, ci->comps[c].coefs, n, hipMemcpyHostToDevice, ctx->stream));
+        d_co[c] = d;
+    }
+    const size_t out_bytes = size_t(ci->width) * ci->height * ci->ncomp;
+    uint8_t* d_out = static_cast<uint8_t*>(ctx->arena.take(out_bytes));
+    RCG_TRY(reconstruct_on_device(ctx, ci, d_co, d_out, hmax, vmax, rgb_direct));
+    RCG_HIP(hipMemcpyAsync(pixels, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}
+
+int rcg_reconstruct_dev(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pixels) {
+    if (!ctx || !d_pixels) return RCG_INVALID;
+    uint32_t hmax, vmax;
+    int rgb_direct;
+    RCG_TRY(check_coef_image(d_ci, hmax, vmax, rgb_direct));
+    size_t need = 0;
+    const int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < d_ci->ncomp; c++) {
+        if (reinterpret_cast<uintptr_t>(d_ci->comps[c].coefs) & 15) {
+            set_error("coefficient buffers must be 16-byte aligned device pointers");
+            return RCG_INVALID;
+        }
+        need += Arena::align(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64);
+        d_co[c] = d_ci->comps[c].coefs;
+    }
+    Call call(ctx);
+    RCG_TRY(call.begin(need));
+    RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, d_pixels, hmax, vmax, rgb_direct));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}
+
+int rcg_ssimulacra2_dev(rcg_ctx* ctx, const uint8_t* d_reference, const uint8_t* d_distorted, uint32_t width,
+                        uint32_t height, uint32_t channels, int precision, double* score, double* per_scale,
+                        int* nscales) {
+    if (!ctx || !d_reference || !d_distorted || !score) return RCG_INVALID;
+    if (!valid_channels(channels)) return RCG_UNSUPPORTED;
+    if (width < 8 || height < 8) {
+        set_error("SSIMULACRA 2 needs at least 8x8 pixels");
+        return RCG_INVALID;
+    }
+    const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
+    const PairPlan P = plan_pair(width, height, T.nt, T.th);
+    Call call(ctx);
+    RCG_TRY(call.begin(ssim_bytes(P, false)));
+    return ssim_pair_dev(ctx, call, d_reference, d_distorted, width, height, channels, precision, score, per_scale,
+                         nscales);
+}
+
+int rcg_ssimulacra2(rcg_ctx* ctx, const uint8_t* reference, const uint8_t* distorted, uint32_t width,
+                    uint32_t height, uint32_t channels, int precision, double* score, double* per_scale,
+                    int* nscales) {
+    if (!ctx || !reference || !distorted || !score) return RCG_INVALID;
+    if (!valid_channels(channels)) return RCG_UNSUPPORTED;
+    if (width < 8 || height < 8) {
+        set_error("SSIMULACRA 2 needs at least 8x8 pixels");
+        return RCG_INVALID;
+    }
+    const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
+    const PairPlan P = plan_pair(width, height, T.nt, T.th);
+    const size_t n = size_t(width) * height * channels;
+    Call call(ctx);
+    RCG_TRY(call.begin(ssim_bytes(P, false) + 2 * Arena::align(n)));
+    uint8_t* d_a = static_cast<uint8_t*>(ctx->arena.take(n));
+    uint8_t* d_b = static_cast<uint8_t*>(ctx->arena.take(n));
+    RCG_HIP(hipMemcpyAsync(d_a, reference, n, hipMemcpyHostToDevice, ctx->stream));
+    RCG_HIP(hipMemcpyAsync(d_b, distorted, n, hipMemcpyHostToDevice, ctx->stream));
+    return ssim_pair_dev(ctx, call, d_a, d_b, width, height, channels, precision, score, per_scale, nscales);
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr double kGuard = 0.05;  // |f32 score - target| below this is settled with the f64 scorer
+
+// Score one candidate quality on the device: encode + reconstruct planes, SSIMULACRA2 against the source.
+// The reference pyramid is built by the first evaluation and reused (a_cached) by the later ones.
+int score_candidate(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_src, int quality,
+                    uint8_t* const rec[3], const SsimBuffers& B32, const SsimBuffers& B64, bool& a32_ready,
+                    bool& a64_ready, double target, double& score, bool& settled) {
+    call.D.off = 0;
+    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, nullptr, rec));
+    const PairPlan P = plan_pair(L.w, L.h, ctx->tune32.nt, ctx->tune32.th);
+    StagedSsim st = stage_ssim(call.D, P, bytes_src(d_src, L.channels), recon_src(L, rec), B32, a32_ready,
+                               RCG_PREC_FP32);
+    RCG_TRY(call.D.upload(ctx->stream));
+    RCG_TRY(launch_staged_ssim(ctx, st));
+    double s = 0.0;
+    RCG_HIP(hipMemcpyAsync(&s, B32.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    a32_ready = true;
+    settled = false;
+    if (std::isfinite(s) && std::fabs(s - target) < kGuard) {
+        call.D.off = 0;
+        const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64.nt, ctx->tune64.th);
+        StagedSsim s64 = stage_ssim(call.D, P64, bytes_src(d_src, L.channels), recon_src(L, rec), B64, a64_ready,
+                                    RCG_PREC_FP64);
+        RCG_TRY(call.D.upload(ctx->stream));
+        RCG_TRY(launch_staged_ssim(ctx, s64));
+        RCG_HIP(hipMemcpyAsync(&s, B64.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        a64_ready = true;
+        settled = true;
+    }
+    // target.rs:36 / pipeline.rs:218: a candidate that cannot be scored counts as f64::MIN.
+    score = std::isfinite(s) ? s : -DBL_MAX;
+    return RCG_OK;
+}
+
+int find_quality_impl(rcg_ctx* ctx, const uint8_t* pixels, bool host_src, uint32_t width, uint32_t height,
+                      uint32_t channels, double target, int max_quality, int subsampling, rcg_found* out,
+                      int16_t* coefs[3], uint8_t qtables[128]) {
+    if (!ctx || !pixels || !out) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    if (width < 8 || height < 8) {
+        set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
+        return RCG_UNSUPPORTED;
+    }
+    // Auto subsampling depends on the quality; size for the larger (4:4:4) layout.
+    const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    const PairPlan P32 = plan_pair(width, height, ctx->tune32.nt, ctx->tune32.th);
+    const PairPlan P64 = plan_pair(width, height, ctx->tune64.nt, ctx->tune64.th);
+    const size_t src_bytes = size_t(width) * height * channels;
+    const size_t need = Arena::align(src_bytes) + plane_bytes(Lmax) + coef_bytes(Lmax) + ssim_bytes(P32, false) +
+                        ssim_bytes(P64, false);
+    Call call(ctx);
+    RCG_TRY(call.begin(need));
+    const uint8_t* d_src = pixels;
+    if (host_src) {
+        uint8_t* d = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+        RCG_HIP(hipMemcpyAsync(d, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+        d_src = d;
+    }
+    uint8_t* rec[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < Lmax.ncomp; c++) rec[c] = static_cast<uint8_t*>(ctx->arena.take(Lmax.plane_bytes(c)));
+    int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < Lmax.ncomp; c++)
+        d_co[c] = static_cast<int16_t*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(int16_t)));
+    const SsimBuffers B32 = take_ssim(ctx->arena, P32);
+    const SsimBuffers B64 = take_ssim(ctx->arena, P64);
+    bool a32 = false, a64 = false;
+
+    // jpeg/src/target.rs:37-55
+    int lo = 1, hi = std::max(1, max_quality);
+    int steps = 0;
+    double s = 0.0;
+    bool settled = false;
+    Layout L = make_layout(width, height, channels, hi, subsampling);
+    RCG_TRY(score_candidate(ctx, call, L, d_src, hi, rec, B32, B64, a32, a64, target, s, settled));
+    steps++;
+    int best_q = hi;
+    double best_s = s;
+    if (s >= target) {
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            L = make_layout(width, height, channels, mid, subsampling);
+            RCG_TRY(score_candidate(ctx, call, L, d_src, mid, rec, B32, B64, a32, a64, target, s, settled));
+            steps++;
+            if (s >= target) {
+                hi = mid;
+                best_q = mid;
+                best_s = s;
+            } else {
+                lo = mid + 1;
+            }
+        }
+    }
+    out->quality = best_q;
+    out->score = best_s;
+    out->steps = steps;
+    if (coefs) {
+        L = make_layout(width, height, channels, best_q, subsampling);
+        call.D.off = 0;
+        int16_t* co[3] = {d_co[0], L.ncomp == 3 ? d_co[1] : nullptr, L.ncomp == 3 ? d_co[2] : nullptr};
+        RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, best_q, co, nullptr));
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            if (coefs[c])
+                RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (qtables) quant_tables(best_q, qtables, qtables + 64);
+    return RCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rcg_find_quality(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                     double target, int max_quality, int subsampling, rcg_found* out, int16_t* coefs[3],
+                     uint8_t qtables[128]) {
+    return find_quality_impl(ctx, pixels, true, width, height, channels, target, max_quality, subsampling, out,
+                             coefs, qtables);
+}
+
+int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, uint32_t height,
+                         uint32_t channels, double target, int max_quality, int subsampling, rcg_found* out) {
+    return find_quality_impl(ctx, d_pixels, false, width, height, channels, target, max_quality, subsampling, out,
+                             nullptr, nullptr);
+}
+
+}  // extern "C"

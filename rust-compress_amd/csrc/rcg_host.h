@@ -1,0 +1,193 @@
+// Host-side internals of librcgpu: the context (device, stream, workspace arena, descriptor staging) and the
+// kernel launchers. Not part of the public C ABI (include/rcgpu.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rcgpu.h"
+#include "rcg_jobs.h"
+
+namespace rcg {
+
+void set_error(const std::string& msg);
+
+#define RCG_HIP(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            ::rcg::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+            return e_ == hipErrorOutOfMemory ? RCG_OOM : RCG_DEVICE;                                   \
+        }                                                                                              \
+    } while (0)
+
+// Device workspace: one growing allocation carved by a bump pointer. A call first adds up what it needs
+// (reserve), then takes pointers (take); nothing is freed until the context is destroyed or grows.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    static size_t align(size_t n) { return (n + 255) & ~size_t(255); }
+    int ensure(size_t bytes);
+    void reset() { used = 0; }
+    void* take(size_t bytes) {
+        void* p = base + used;
+        used += align(bytes);
+        return p;
+    }
+};
+
+// Pinned host staging for job descriptors and small results.
+struct Pinned {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    int ensure(size_t bytes);
+    void reset() { used = 0; }
+    void* take(size_t bytes) {
+        void* p = base + used;
+        used += (bytes + 255) & ~size_t(255);
+        return p;
+    }
+};
+
+// Per-kernel HIP-event timing on the context's stream.
+struct Prof {
+    bool on = false;
+    struct Pending {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, std::pair<double, int>> totals;
+    hipEvent_t get();
+    void collect();  // stream must be synchronised
+};
+
+// Launch shapes of the SSIMULACRA2 scale kernel: threads per workgroup (columns + 12 halo) and rows per tile.
+struct SsimTune {
+    int nt = 256;
+    int th = 64;
+};
+
+}  // namespace rcg
+
+struct rcg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    rcg::Arena arena;
+    rcg::Pinned pinned;
+    float* d_lut = nullptr;  // sRGB -> linear table, 256 f32
+    rcg::SsimTune tune32, tune64;
+    rcg::Prof prof;
+    double taps64[6];
+    float taps32[6];
+};
+
+namespace rcg {
+
+// Sampling layout of build_coefs (jpeg/src/encode.rs:428-443).
+struct Layout {
+    uint32_t w, h, channels;
+    uint32_t hmax, vmax, pw, ph;
+    uint32_t ncomp;
+    uint32_t bw[3], bh[3];
+    size_t coef_count(int c) const { return size_t(bw[c]) * bh[c] * 64; }
+    size_t plane_bytes(int c) const { return size_t(bw[c]) * 8 * bh[c] * 8; }
+};
+Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int subsampling);
+void quant_tables(int quality, uint8_t luma[64], uint8_t chroma[64]);
+void fill_cand_tables(const uint8_t qy[64], const uint8_t qc[64], EncCand& c);
+
+// Brackets one kernel launch with profiling events when profiling is on.
+struct ProfScope {
+    rcg_ctx* ctx;
+    const char* name;
+    hipEvent_t b = nullptr;
+    ProfScope(rcg_ctx* c, const char* n);
+    ~ProfScope();
+};
+
+// Launchers (defined next to their kernels).
+int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t total_tiles);
+struct IdctPlane {
+    const int16_t* coefs;
+    uint8_t* plane;
+    uint32_t blocks_w, blocks_h;
+    float mult[64];
+};
+int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
+int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
+                          uint8_t* out);
+int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
+                      uint32_t total_tiles, int nt, int th);
+struct FinalDesc {
+    uint32_t nscales;
+    uint32_t ntiles[6];
+    const double* partials[6];
+    double count[6];
+    double* stats_out;
+    double* score_out;
+};
+int launch_finalize(rcg_ctx* ctx, const FinalDesc* d_jobs, int n);
+int upload_weights();
+
+// Tiles of the SSIMULACRA2 scale kernel for a w x h scale.
+inline uint32_t ssim_tiles_x(uint32_t w, int nt) { return (w + (nt - 12) - 1) / (nt - 12); }
+inline uint32_t ssim_tiles_y(uint32_t h, int th) { return (h + th - 1) / th; }
+
+// One SSIMULACRA2 evaluation of an image pair through all of its scales, described for the batched launcher.
+struct PairPlan {
+    uint32_t w, h, nscales;
+    uint32_t sw[6], sh[6];
+    uint32_t tiles[6], tiles_x[6], tiles_y[6];
+};
+PairPlan plan_pair(uint32_t w, uint32_t h, int nt, int th);
+
+// Descriptor staging: descriptors are written to pinned memory at the offsets they will have in a device region
+// and uploaded with one copy before the launches that read them.
+struct Desc {
+    char* host = nullptr;
+    char* dev = nullptr;
+    size_t off = 0, cap = 0;
+    bool overflow = false;
+    template <class T>
+    T* put(const T* items, size_t n) {
+        off = (off + 63) & ~size_t(63);
+        if (off + sizeof(T) * n > cap) {
+            overflow = true;
+            return reinterpret_cast<T*>(dev);
+        }
+        T* d = reinterpret_cast<T*>(dev + off);
+        std::memcpy(host + off, items, sizeof(T) * n);
+        off += sizeof(T) * n;
+        return d;
+    }
+    int upload(hipStream_t s) {
+        if (overflow) {
+            set_error("internal: descriptor staging overflow");
+            return RCG_DEVICE;
+        }
+        if (off) RCG_HIP(hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, s));
+        return RCG_OK;
+    }
+};
+
+// Sources of the SSIMULACRA2 loader.
+SsimSrc recon_src(const Layout& L, uint8_t* const rec[3]);
+SsimSrc bytes_src(const uint8_t* d, uint32_t channels);
+SsimSrc planes_src(const float* planes);
+int check_image(uint32_t w, uint32_t h, uint32_t channels);
+
+#define RCG_TRY(expr)                    \
+    do {                                 \
+        int rc_ = (expr);                \
+        if (rc_ != RCG_OK) return rc_;   \
+    } while (0)
+
+}  // namespace rcg

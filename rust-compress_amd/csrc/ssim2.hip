@@ -1,0 +1,444 @@
+// SSIMULACRA2 kernels for gfx950 (core/src/ssimulacra2.rs:471-512).
+//
+// k_ssim_scale<T, NT> computes one scale of one or many image pairs. A workgroup of NT threads owns a tile of
+// OW = NT-12 output columns and TH output rows. Thread t owns column x0-6+t and walks down rows y0-5 .. y0+TH+4:
+//   1. loads the linear-light RGB of both images at its column (from RGB8 through the sRGB table, from the
+//      reconstructed JPEG sample planes with fused upsampling and colour conversion, or from the previous scale's
+//      f32 planes), rows above/below the image mirrored (ssimulacra2.rs:385-397);
+//   2. converts to XYB (ssimulacra2.rs:251-258) and keeps the last 11 rows in a register ring;
+//   3. blurs the 15 quantities (mu_a, mu_b, a^2, b^2, ab for X, Y, B) down its column from the ring, then exchanges
+//      one row of them through LDS and blurs across the row; columns outside the image hold zeros, which is the
+//      reference's horizontal border rule (ssimulacra2.rs:34-40, 278-290);
+//   4. evaluates the SSIM, artifact and detail maps and accumulates their sums and 4th-power sums;
+//   5. writes the 2x2 linear-light mean for the next scale (ssimulacra2.rs:225-247, summed in f64 in the
+//      reference's order, so the pyramid is bit-identical to the reference's f32 planes).
+// Each tile writes its 18 sums as f64; k_ssim_finalize reduces them in a fixed order and pools the scales.
+//
+// T = float is the precision form (f32 XYB, blurs and maps; f32 sums within a tile column, f64 above that);
+// T = double follows the reference's f64 arithmetic and is used to settle candidates whose f32 score lies within
+// the guard band of a search target.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "rcg_host.h"
+#include "rcg_jobs.h"
+#include "rcg_pixel.h"
+
+namespace rcg {
+
+template <typename T>
+struct Taps {
+    T k[6];  // k[0] centre, k[t] at distance t (normalised 11-tap Gaussian, sigma 1.5; ssimulacra2.rs:261-271)
+};
+
+__constant__ double c_weights[108];
+
+template <typename T>
+struct Xyb;
+
+template <>
+struct Xyb<float> {
+    static __device__ __forceinline__ float cbrt_pos(float x) {
+        // x >= 0; exp2(log2(x)/3) on the transcendental unit. log2(0) = -inf gives 0.
+        return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
+    }
+    static __device__ __forceinline__ void eval(float r, float g, float b, float& X, float& Y, float& B) {
+#pragma clang fp contract(fast)
+        const float M00 = 0.30f, M02 = 0.078f, M01 = float(1.0 - 0.078 - 0.30);
+        const float M10 = 0.23f, M12 = 0.078f, M11 = float(1.0 - 0.078 - 0.23);
+        const float M20 = float(0.2434226892454782), M21 = float(0.2047674442449682);
+        const float M22 = float(1.0 - 0.2434226892454782 - 0.2047674442449682);
+        const float BIAS = float(0.0037930732552754493);
+        const float CB = float(0.15595420054924863);  // cbrt(BIAS), rounded to f32
+        const float m0 = cbrt_pos(fmaxf(M00 * r + M01 * g + M02 * b + BIAS, 0.0f)) - CB;
+        const float m1 = cbrt_pos(fmaxf(M10 * r + M11 * g + M12 * b + BIAS, 0.0f)) - CB;
+        const float m2 = cbrt_pos(fmaxf(M20 * r + M21 * g + M22 * b + BIAS, 0.0f)) - CB;
+        const float x = 0.5f * (m0 - m1), y = 0.5f * (m0 + m1);
+        X = x * 14.0f + 0.42f;
+        Y = y + 0.01f;
+        B = (m2 - y) + 0.55f;
+    }
+};
+
+template <>
+struct Xyb<double> {
+    static __device__ __forceinline__ void eval(double r, double g, double b, double& X, double& Y, double& B) {
+        const double M00 = 0.30, M02 = 0.078, M01 = 1.0 - M02 - M00;
+        const double M10 = 0.23, M12 = 0.078, M11 = 1.0 - M12 - M10;
+        const double M20 = 0.2434226892454782, M21 = 0.2047674442449682, M22 = 1.0 - M20 - M21;
+        const double BIAS = 0.0037930732552754493;
+        const double CB = cbrt(BIAS);
+        const double m0 = cbrt(fmax(M00 * r + M01 * g + M02 * b + BIAS, 0.0)) - CB;
+        const double m1 = cbrt(fmax(M10 * r + M11 * g + M12 * b + BIAS, 0.0)) - CB;
+        const double m2 = cbrt(fmax(M20 * r + M21 * g + M22 * b + BIAS, 0.0)) - CB;
+        const double x = 0.5 * (m0 - m1), y = 0.5 * (m0 + m1);
+        X = x * 14.0 + 0.42;
+        Y = y + 0.01;
+        B = (m2 - y) + 0.55;
+    }
+};
+
+// Linear-light RGB of one image at (x, y), in range.
+__device__ __forceinline__ void load_linear(const SsimSrc& S, uint32_t w, uint32_t h, uint32_t x, uint32_t y,
+                                            const float* lut, float out[3]) {
+    if (S.kind == kSrcPlanes) {
+        const uint64_t i = uint64_t(y) * w + x, n = uint64_t(w) * h;
+        out[0] = S.planes[i];
+        out[1] = S.planes[n + i];
+        out[2] = S.planes[2 * n + i];
+    } else if (S.kind == kSrcBytes) {
+        if (S.channels == 1) {
+            const float l = lut[S.bytes[uint64_t(y) * w + x]];
+            out[0] = out[1] = out[2] = l;
+        } else {
+            const uint8_t* p = S.bytes + (uint64_t(y) * w + x) * 3;
+            out[0] = lut[p[0]];
+            out[1] = lut[p[1]];
+            out[2] = lut[p[2]];
+        }
+    } else {
+        if (S.ncomp == 1) {
+            const float l = lut[upsample_at(S.comp[0], x, y)];
+            out[0] = out[1] = out[2] = l;
+        } else {
+            const int32_t p0 = upsample_at(S.comp[0], x, y);
+            const int32_t p1 = upsample_at(S.comp[1], x, y);
+            const int32_t p2 = upsample_at(S.comp[2], x, y);
+            int32_t r, g, b;
+            if (S.rgb_direct) {
+                r = p0; g = p1; b = p2;
+            } else {
+                ycc_to_rgb(p0, p1, p2, r, g, b);
+            }
+            out[0] = lut[r];
+            out[1] = lut[g];
+            out[2] = lut[b];
+        }
+    }
+}
+
+__device__ __forceinline__ int find_job(const uint32_t* begins, int njobs, uint32_t bid) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (begins[mid] <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Per-workgroup constants and per-thread state of the scale kernel's row walk.
+template <typename T, int NT>
+struct Walk {
+    const SsimJob* J;
+    const float* lut;
+    const T* taps;
+    T (*s_q)[15][NT];
+    int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
+    bool in_img, is_out, want_a, want_b, want_next, ds_writer;
+    int yy;
+    T ra[11][3], rb[11][3];
+    T acc[18];
+    float pa_own[3], pa_par[3], pb_own[3], pb_par[3];
+};
+
+// One input row of the walk, in ring slot S (compile-time, so the ring stays in registers). Returns false once the
+// walk is past its last row.
+template <int S, typename T, int NT>
+__device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
+#pragma clang fp contract(fast)
+    const int yy = st.yy;
+    if (yy > st.ylast) return false;
+    const int h = st.h, w = st.w;
+    const int r = yy < 0 ? -yy - 1 : (yy >= h ? 2 * h - 1 - yy : yy);
+    float la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
+    if (st.in_img) {
+        load_linear(st.J->a, st.J->w, st.J->h, st.gx, r, st.lut, la);
+        load_linear(st.J->b, st.J->w, st.J->h, st.gx, r, st.lut, lb);
+        Xyb<T>::eval(T(la[0]), T(la[1]), T(la[2]), st.ra[S][0], st.ra[S][1], st.ra[S][2]);
+        Xyb<T>::eval(T(lb[0]), T(lb[1]), T(lb[2]), st.rb[S][0], st.rb[S][1], st.rb[S][2]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            st.ra[S][c] = T(0);
+            st.rb[S][c] = T(0);
+        }
+    }
+    // Next-scale 2x2 mean (uniform condition: the row is one of this tile's own rows).
+    if (st.want_next && yy >= st.y0 && yy < st.y_end) {
+        float qa[3], qb[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            qa[c] = __shfl_xor(la[c], 1);
+            qb[c] = __shfl_xor(lb[c], 1);
+        }
+        const bool first_row = ((yy - st.y0) & 1) == 0;
+        if (first_row) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                st.pa_own[c] = la[c];
+                st.pa_par[c] = qa[c];
+                st.pb_own[c] = lb[c];
+                st.pb_par[c] = qb[c];
+            }
+        }
+        const bool emit = !first_row || yy == h - 1;
+        if (emit && st.ds_writer) {
+            const bool two_cols = st.gx + 1 < w;
+            const int oy = yy >> 1, ox = st.gx >> 1;
+            const uint64_t n2 = uint64_t(st.nw) * st.nh, o = uint64_t(oy) * st.nw + ox;
+            const double inv = 1.0 / double((first_row ? 1 : 2) * (two_cols ? 2 : 1));
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                // core/src/ssimulacra2.rs:236-243: row by row, left to right, in f64.
+                double sa = double(st.pa_own[c]), sb = double(st.pb_own[c]);
+                if (two_cols) {
+                    sa += double(st.pa_par[c]);
+                    sb += double(st.pb_par[c]);
+                }
+                if (!first_row) {
+                    sa += double(la[c]);
+                    sb += double(lb[c]);
+                    if (two_cols) {
+                        sa += double(qa[c]);
+                        sb += double(qb[c]);
+                    }
+                }
+                if (st.want_a) st.J->a_next[c * n2 + o] = float(sa * inv);
+                if (st.want_b) st.J->b_next[c * n2 + o] = float(sb * inv);
+            }
+        }
+    }
+    if (yy >= st.y0 + 5) {
+        // Vertical blur for output row yy-5: centre in slot (S+6)%11, distance t in (S+6-t)%11 and (S+6+t)%11.
+        constexpr int SC = (S + 6) % 11;
+        const int buf = (yy - 5) & 1;
+        const T* k = st.taps;
+        const int tid = st.tid;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T a0 = st.ra[SC][c], b0 = st.rb[SC][c];
+            T ma = k[0] * a0, mb = k[0] * b0;
+            T saa = k[0] * (a0 * a0), sbb = k[0] * (b0 * b0), sab = k[0] * (a0 * b0);
+#pragma unroll
+            for (int t = 1; t <= 5; t++) {
+                const int su = (S + 6 - t + 11) % 11, sd = (S + 6 + t) % 11;
+                const T au = st.ra[su][c], ad = st.ra[sd][c], bu = st.rb[su][c], bd = st.rb[sd][c];
+                const T kt = k[t];
+                ma += kt * (au + ad);
+                mb += kt * (bu + bd);
+                saa += kt * (au * au + ad * ad);
+                sbb += kt * (bu * bu + bd * bd);
+                sab += kt * (au * bu + ad * bd);
+            }
+            st.s_q[buf][c * 5 + 0][tid] = ma;
+            st.s_q[buf][c * 5 + 1][tid] = mb;
+            st.s_q[buf][c * 5 + 2][tid] = saa;
+            st.s_q[buf][c * 5 + 3][tid] = sbb;
+            st.s_q[buf][c * 5 + 4][tid] = sab;
+        }
+        __syncthreads();
+        if (st.is_out) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                T v[5];
+#pragma unroll
+                for (int q = 0; q < 5; q++) {
+                    const T* row = &st.s_q[buf][c * 5 + q][tid];
+                    T sum = k[0] * row[0];
+#pragma unroll
+                    for (int t = 1; t <= 5; t++) sum += k[t] * (row[-t] + row[t]);
+                    v[q] = sum;
+                }
+                // core/src/ssimulacra2.rs:424-440
+                const T m1 = v[0], m2 = v[1], s11 = v[2], s22 = v[3], s12 = v[4];
+                const T C2 = T(0.0009);
+                const T num_m = T(1) - (m1 - m2) * (m1 - m2);
+                const T num_s = T(2) * (s12 - m1 * m2) + C2;
+                const T denom_s = (s11 - m1 * m1) + (s22 - m2 * m2) + C2;
+                T d = T(1) - num_m * num_s / denom_s;
+                d = d > T(0) ? d : T(0);
+                const T d1 = (T(1) + fabs(st.rb[SC][c] - m2)) / (T(1) + fabs(st.ra[SC][c] - m1)) - T(1);
+                const T art = d1 > T(0) ? d1 : T(0);
+                const T det = -d1 > T(0) ? -d1 : T(0);
+                const T d2 = d * d, a2 = art * art, e2 = det * det;
+                st.acc[c * 6 + 0] += d;
+                st.acc[c * 6 + 1] += d2 * d2;
+                st.acc[c * 6 + 2] += art;
+                st.acc[c * 6 + 3] += a2 * a2;
+                st.acc[c * 6 + 4] += det;
+                st.acc[c * 6 + 5] += e2 * e2;
+            }
+        }
+    }
+    st.yy = yy + 1;
+    return true;
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ tile_begins, int njobs, int TH,
+                                                    Taps<T> taps, const float* __restrict__ lut_g) {
+    constexpr int OW = NT - 12;
+    constexpr int NWAVES = NT / 64;
+    __shared__ T s_q[2][15][NT];
+    __shared__ float s_lut[256];
+    __shared__ double s_red[NWAVES][18];
+
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 256; i += NT) s_lut[i] = lut_g[i];
+
+    Walk<T, NT> st;
+    st.J = &jobs[find_job(tile_begins, njobs, blockIdx.x)];
+    const SsimJob& J = *st.J;
+    const uint32_t tile = blockIdx.x - J.tile_begin;
+    const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+    st.lut = s_lut;
+    st.taps = taps.k;
+    st.s_q = s_q;
+    st.tid = tid;
+    st.w = int(J.w);
+    st.h = int(J.h);
+    st.x0 = int(tx) * OW;
+    st.y0 = int(ty) * TH;
+    st.gx = st.x0 - 6 + tid;
+    st.in_img = st.gx >= 0 && st.gx < st.w;
+    st.is_out = tid >= 6 && tid < 6 + OW && st.gx < st.w;
+    st.y_end = min(st.y0 + TH, st.h);
+    st.ylast = st.y_end + 4;
+    st.want_a = J.a_next != nullptr;
+    st.want_b = J.b_next != nullptr;
+    st.want_next = st.want_a || st.want_b;
+    st.nw = (st.w + 1) >> 1;
+    st.nh = (st.h + 1) >> 1;
+    st.ds_writer = st.want_next && st.is_out && ((st.gx & 1) == 0);
+    st.yy = st.y0 - 5;
+#pragma unroll
+    for (int i = 0; i < 18; i++) st.acc[i] = T(0);
+#pragma unroll
+    for (int c = 0; c < 3; c++) st.pa_own[c] = st.pa_par[c] = st.pb_own[c] = st.pb_par[c] = 0.0f;
+    __syncthreads();
+
+    while (walk_row<0>(st) && walk_row<1>(st) && walk_row<2>(st) && walk_row<3>(st) && walk_row<4>(st) &&
+           walk_row<5>(st) && walk_row<6>(st) && walk_row<7>(st) && walk_row<8>(st) && walk_row<9>(st) &&
+           walk_row<10>(st)) {
+    }
+
+    // Workgroup reduction of the 18 sums in f64.
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < 18; i++) {
+        double v = double(st.acc[i]);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) s_red[wave][i] = v;
+    }
+    __syncthreads();
+    if (tid < 18) {
+        double v = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < NWAVES; wv++) v += s_red[wv][tid];
+        J.partials[uint64_t(tile) * 18 + tid] = v;
+    }
+}
+
+template __global__ void k_ssim_scale<float, 256>(const SsimJob*, const uint32_t*, int, int, Taps<float>, const float*);
+template __global__ void k_ssim_scale<float, 128>(const SsimJob*, const uint32_t*, int, int, Taps<float>, const float*);
+template __global__ void k_ssim_scale<double, 128>(const SsimJob*, const uint32_t*, int, int, Taps<double>,
+                                                   const float*);
+
+// One workgroup (64 threads) per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean
+// and 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
+__global__ __launch_bounds__(64) void k_ssim_finalize(const FinalDesc* __restrict__ fj) {
+    const FinalDesc& F = fj[blockIdx.x];
+    __shared__ double s_stats[6][18];
+    const int lane = threadIdx.x;
+    for (uint32_t sc = 0; sc < F.nscales; sc++) {
+        double part[18];
+        for (int i = 0; i < 18; i++) part[i] = 0.0;
+        for (uint32_t t = lane; t < F.ntiles[sc]; t += 64) {
+            const double* p = F.partials[sc] + uint64_t(t) * 18;
+            for (int i = 0; i < 18; i++) part[i] += p[i];
+        }
+        for (int i = 0; i < 18; i++) {
+            double v = part[i];
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) s_stats[sc][i] = v;
+        }
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    double stats[6][3][6];
+    for (uint32_t sc = 0; sc < F.nscales; sc++) {
+        const double count = F.count[sc];
+        for (int c = 0; c < 3; c++)
+            for (int m = 0; m < 3; m++) {
+                stats[sc][c][2 * m] = s_stats[sc][c * 6 + 2 * m] / count;
+                stats[sc][c][2 * m + 1] = pow(s_stats[sc][c * 6 + 2 * m + 1] / count, 0.25);
+            }
+    }
+    double ssim = 0.0;
+    int i = 0;
+    for (int c = 0; c < 3; c++)
+        for (uint32_t sc = 0; sc < F.nscales; sc++)
+            for (int n = 0; n < 2; n++) {
+                ssim += c_weights[i] * fabs(stats[sc][c][n]);
+                i++;
+                ssim += c_weights[i] * fabs(stats[sc][c][2 + n]);
+                i++;
+                ssim += c_weights[i] * fabs(stats[sc][c][4 + n]);
+                i++;
+            }
+    ssim *= 0.9562382616834844;
+    ssim = 2.326765642916932 * ssim - 0.020884521182843837 * ssim * ssim + 6.248496625763138e-5 * ssim * ssim * ssim;
+    const double score = ssim > 0.0 ? 100.0 - 10.0 * pow(ssim, 0.6276336467831387) : 100.0;
+    *F.score_out = score;
+    if (F.stats_out)
+        for (uint32_t sc = 0; sc < F.nscales; sc++)
+            for (int c = 0; c < 3; c++)
+                for (int m = 0; m < 6; m++) F.stats_out[(sc * 3 + c) * 6 + m] = stats[sc][c][m];
+}
+
+// ----------------------------------------------------------------------------------------------------------------
+// Launchers.
+
+int upload_weights() {
+    static const double w[108] = {
+#include "ssim2_weights.inc"
+    };
+    RCG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_weights), w, sizeof(w)));
+    return RCG_OK;
+}
+
+int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
+                      uint32_t total_tiles, int nt, int th) {
+    if (total_tiles == 0) return RCG_OK;
+    ProfScope ps(ctx, "k_ssim_scale");
+    if (precision == RCG_PREC_FP64) {
+        Taps<double> t;
+        for (int i = 0; i < 6; i++) t.k[i] = ctx->taps64[i];
+        hipLaunchKernelGGL((k_ssim_scale<double, 128>), dim3(total_tiles), dim3(128), 0, ctx->stream, d_jobs,
+                           d_begins, njobs, th, t, ctx->d_lut);
+    } else {
+        Taps<float> t;
+        for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
+        if (nt == 128)
+            hipLaunchKernelGGL((k_ssim_scale<float, 128>), dim3(total_tiles), dim3(128), 0, ctx->stream, d_jobs,
+                               d_begins, njobs, th, t, ctx->d_lut);
+        else
+            hipLaunchKernelGGL((k_ssim_scale<float, 256>), dim3(total_tiles), dim3(256), 0, ctx->stream, d_jobs,
+                               d_begins, njobs, th, t, ctx->d_lut);
+    }
+    RCG_HIP(hipGetLastError());
+    return RCG_OK;
+}
+
+int launch_finalize(rcg_ctx* ctx, const FinalDesc* d_jobs, int n) {
+    if (n == 0) return RCG_OK;
+    ProfScope ps(ctx, "k_ssim_finalize");
+    hipLaunchKernelGGL(k_ssim_finalize, dim3(n), dim3(64), 0, ctx->stream, d_jobs);
+    RCG_HIP(hipGetLastError());
+    return RCG_OK;
+}
+
+}  // namespace rcg

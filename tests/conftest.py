@@ -14,3 +14,17 @@ if TESTS not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
     config.addinivalue_line("markers", "slow: long CPU oracle runs")
+
+
+@pytest.fixture(scope="session")
+def rc():
+    """The product package (librcgpu.so through ctypes)."""
+    import importlib
+    return importlib.import_module("rust-compress_amd")
+
+
+@pytest.fixture(scope="session")
+def ctx(rc):
+    c = rc.Context(0)
+    yield c
+    c.close()

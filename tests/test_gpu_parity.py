@@ -1,0 +1,227 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Integer and f32-DCT work (coefficients, reconstructed pixels) must be bit-exact. SSIMULACRA2 in FP32 must be within
+0.02 of the oracle's f64 score (the precision bound this path states); the FP64 kernel within 1e-6. The search must
+choose the oracle's quality."""
+import csv
+import os
+
+import numpy as np
+import pytest
+from PIL import Image
+
+from oracle import oracle as O
+import jpeg_scan
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL32 = 0.02
+TOL64 = 1e-6
+
+
+def _assert_coefs_equal(gpu, ref):
+    assert len(gpu.comps) == len(ref.comps)
+    for a, b in zip(gpu.comps, ref.comps):
+        assert (a.h, a.v, a.blocks_w) == (b.h, b.v, b.blocks_w)
+        assert a.coefs.shape == b.coefs.shape
+        if not np.array_equal(a.coefs, b.coefs):
+            bad = np.nonzero(a.coefs != b.coefs)[0]
+            raise AssertionError(f"component {a.id}: {len(bad)} coefficients differ, first at {bad[:8]}")
+    for s in (0, 1):
+        if ref.qtables[s] is not None:
+            np.testing.assert_array_equal(gpu.qtables[s], ref.qtables[s])
+
+
+def _smooth(w, h, ch):
+    y, x = np.mgrid[0:h, 0:w]
+    out = np.stack([(x * 3 + y * 2 + c * 40) % 256 for c in range(ch)], -1).astype(np.uint8)
+    return out[:, :, 0] if ch == 1 else out
+
+
+# jpeg/tests/jpeg.rs:65-77 sizes, both colour types, all three subsamplings.
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 9), (8, 8), (9, 9), (17, 1), (1, 17), (33, 31), (250, 3), (67, 45),
+                                 (513, 129)])
+@pytest.mark.parametrize("ch", [1, 3])
+@pytest.mark.parametrize("subs", [O.S444, O.S422, O.S420])
+def test_build_coefs_and_reconstruct_odd_and_tiny(ctx, w, h, ch, subs):
+    img = _smooth(w, h, ch)
+    if ch == 3:
+        img = np.ascontiguousarray(img)
+    gpu = ctx.build_coefs(img, 90, subs)
+    ref = O.build_coefs(img, 90, subs)
+    _assert_coefs_equal(gpu, ref)
+    np.testing.assert_array_equal(ctx.reconstruct(gpu), O.reconstruct(ref))
+
+
+@pytest.mark.parametrize("q", [1, 30, 75, 95, 100])
+def test_build_coefs_c1_512(ctx, rc, q):
+    img = rc.synth_rgb8(1, 512, 512)
+    gpu = ctx.build_coefs(img, q, rc.S420)
+    ref = O.build_coefs(img, q, O.S420)
+    _assert_coefs_equal(gpu, ref)
+    np.testing.assert_array_equal(ctx.reconstruct(gpu), O.reconstruct(ref))
+
+
+def test_build_coefs_and_reconstruct_4096_all_c2_cases(ctx, rc):
+    img = rc.synth_rgb8(2, 4096, 4096)
+    for subs in (rc.S420, rc.S444):
+        for q in (50, 75, 90):
+            gpu = ctx.build_coefs(img, q, subs)
+            ref = O.build_coefs(img, q, subs)
+            _assert_coefs_equal(gpu, ref)
+            np.testing.assert_array_equal(ctx.reconstruct(gpu), O.reconstruct(ref))
+
+
+def test_build_coefs_odd_3000x2001_auto(ctx, rc):
+    img = rc.synth_rgb8(3, 3000, 2001)
+    for q in (60, 95):
+        gpu = ctx.build_coefs(img, q, rc.AUTO)
+        ref = O.build_coefs(img, q, O.AUTO)
+        _assert_coefs_equal(gpu, ref)
+        np.testing.assert_array_equal(ctx.reconstruct(gpu), O.reconstruct(ref))
+
+
+@pytest.mark.parametrize("name", ["rgb8_420.jpg", "rgb8_444.jpg", "gray8.jpg"])
+def test_reconstruct_reference_files(ctx, rc, name):
+    with open(os.path.join(GOLD, name), "rb") as f:
+        fr = jpeg_scan.first_frame(f.read())
+    comps = [rc.CoefComponent(c["id"], c["h"], c["v"], c["tq"], c["blocks_w"], c["coefs"]) for c in fr["comps"]]
+    ci = rc.CoefImage(fr["width"], fr["height"], comps, [fr["qtables"].get(i) for i in range(4)])
+    oc = O.CoefImage(fr["width"], fr["height"],
+                     [O.CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, c.coefs) for c in comps], ci.qtables)
+    np.testing.assert_array_equal(ctx.reconstruct(ci), O.reconstruct(oc))
+
+
+def test_reconstruct_errors(ctx, rc):
+    img = _smooth(32, 32, 3)
+    ci = ctx.build_coefs(img, 80, rc.S420)
+    bad = rc.CoefImage(ci.width, ci.height, ci.comps, [ci.qtables[0], None, None, None])
+    with pytest.raises(rc.RcgError) as e:
+        ctx.reconstruct(bad)
+    assert e.value.status == rc.INVALID
+    odd = rc.CoefImage(ci.width, ci.height, [ci.comps[0], rc.CoefComponent(2, 3, 1, 1, ci.comps[1].blocks_w,
+                                                                           ci.comps[1].coefs), ci.comps[2]],
+                       ci.qtables)
+    with pytest.raises(rc.RcgError) as e:
+        ctx.reconstruct(odd)
+    assert e.value.status == rc.UNSUPPORTED
+
+
+# ------------------------------------------------------------------------------------------------ SSIMULACRA2
+
+def _png(name):
+    return np.asarray(Image.open(os.path.join(GOLD, name)).convert("RGB"))
+
+
+def test_ssimulacra2_golden_pairs_fp32_and_fp64(ctx, rc):
+    rows = [r for r in csv.reader(open(os.path.join(GOLD, "ssim2", "expected.csv"))) if r]
+    worst32 = worst64 = 0.0
+    for name, a, b, want in rows:
+        A, B = _png("ssim2/" + a), _png("ssim2/" + b)
+        oracle = O.ssimulacra2(A, B)
+        s32 = ctx.ssimulacra2(A, B, rc.FP32)
+        s64 = ctx.ssimulacra2(A, B, rc.FP64)
+        worst32 = max(worst32, abs(s32 - oracle))
+        worst64 = max(worst64, abs(s64 - oracle))
+        assert abs(s32 - oracle) <= TOL32, (name, s32, oracle)
+        assert abs(s64 - oracle) <= TOL64, (name, s64, oracle)
+        assert abs(s64 - float(want)) <= 1e-4, (name, s64, want)
+    print(f"golden pairs: worst |fp32 - oracle| = {worst32:.3e}, worst |fp64 - oracle| = {worst64:.3e}")
+
+
+def test_ssimulacra2_identity_is_100(ctx, rc):
+    img = _png("ssim2/identical_96x72.ref.png")
+    assert ctx.ssimulacra2(img, img, rc.FP64) == 100.0
+    assert abs(ctx.ssimulacra2(img, img, rc.FP32) - 100.0) < 1e-9
+
+
+def test_ssimulacra2_gray_input(ctx, rc):
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 256, (40, 33), dtype=np.uint8)
+    b = np.clip(a.astype(np.int32) + rng.integers(-9, 10, a.shape), 0, 255).astype(np.uint8)
+    assert abs(ctx.ssimulacra2(a, b, rc.FP64) - O.ssimulacra2(a, b)) <= TOL64
+    assert abs(ctx.ssimulacra2(a, b, rc.FP32) - O.ssimulacra2(a, b)) <= TOL32
+
+
+def test_ssimulacra2_errors(ctx, rc):
+    with pytest.raises(rc.RcgError) as e:
+        ctx.ssimulacra2(np.zeros((7, 30, 3), np.uint8), np.zeros((7, 30, 3), np.uint8))
+    assert e.value.status == rc.INVALID
+    with pytest.raises(rc.RcgError):
+        ctx.ssimulacra2(np.zeros((8, 8, 3), np.uint8), np.zeros((8, 9, 3), np.uint8))
+
+
+@pytest.mark.parametrize("w,h", [(2048, 2048), (3000, 2001)])
+def test_ssimulacra2_precision_config(ctx, rc, w, h):
+    # Distorted = reference decode at q60 4:2:0; per-scale and final |error| against the f64 oracle.
+    img = rc.synth_rgb8(11, w, h)
+    dist = O.reconstruct(O.build_coefs(img, 60, O.S420))
+    want, wstats = O.ssimulacra2(img, dist, with_stats=True)
+    got, gstats = ctx.ssimulacra2(img, dist, rc.FP32, with_stats=True)
+    got64, g64 = ctx.ssimulacra2(img, dist, rc.FP64, with_stats=True)
+    assert gstats.shape == wstats.shape
+    per_scale = np.abs(gstats - wstats).max(axis=(1, 2))
+    print(f"{w}x{h}: score {want:.6f}, fp32 |d| {abs(got - want):.3e}, fp64 |d| {abs(got64 - want):.3e}, "
+          f"per-scale max |d stat| {per_scale}")
+    assert abs(got - want) <= TOL32
+    assert abs(got64 - want) <= TOL64
+    assert np.abs(g64 - wstats).max() <= 1e-9
+
+
+# ------------------------------------------------------------------------------------------------ search
+
+@pytest.mark.parametrize("target", [60.0, 70.0, 80.0])
+def test_find_quality_matches_oracle(ctx, rc, target):
+    for seed, (w, h) in ((21, (256, 256)), (22, (301, 199))):
+        img = rc.synth_rgb8(seed, w, h)
+        found = ctx.find_quality(img, target, 100, rc.S420)
+        q, s, steps = O.find_quality(img, target, 100, O.S420)
+        assert found.quality == q, (seed, target, found.quality, q)
+        assert abs(found.score - s) <= TOL32
+        assert found.steps == steps
+        _assert_coefs_equal(found.coefs, O.build_coefs(img, q, O.S420))
+
+
+def test_find_quality_rejects_small(ctx, rc):
+    with pytest.raises(rc.RcgError) as e:
+        ctx.find_quality(np.zeros((7, 100, 3), np.uint8), 70.0)
+    assert e.value.status == rc.UNSUPPORTED
+
+
+CANDS = [(q, 3) for q in (40, 50, 60, 70, 80, 85, 90, 95)]
+
+
+def test_score_batch_mixed_sizes_matches_oracle(ctx, rc):
+    rng = np.random.default_rng(9)
+    sizes = [(int(rng.integers(64, 700)), int(rng.integers(64, 700))) for _ in range(6)] + [(8, 8), (65, 9)]
+    imgs = [rc.synth_rgb8(100 + i, w, h) for i, (w, h) in enumerate(sizes)]
+    res = ctx.score_batch(imgs, CANDS, 70.0)
+    for img, r in zip(imgs, res):
+        assert r.status == 0
+        oscores = []
+        for k, (q, sub) in enumerate(CANDS):
+            o = O.ssimulacra2(img, O.reconstruct(O.build_coefs(img, q, sub)))
+            oscores.append(o)
+            tol = TOL64 if r.settled[k] else TOL32
+            assert abs(r.scores[k] - o) <= tol, (img.shape, q, r.scores[k], o)
+        # The reference's search semantics over the candidate list, on the oracle's scores.
+        qs = [q for q, _ in CANDS]
+        order = np.argsort(qs, kind="stable")
+        lo, hi = 0, len(qs) - 1
+        best = hi
+        if oscores[order[hi]] >= 70.0:
+            while lo < hi:
+                mid = lo + (hi - lo) // 2
+                if oscores[order[mid]] >= 70.0:
+                    hi = mid
+                    best = mid
+                else:
+                    lo = mid + 1
+        assert r.chosen_quality == qs[order[best]]
+
+
+def test_score_batch_reports_per_job_errors(ctx, rc):
+    good = rc.synth_rgb8(1, 64, 64)
+    res = ctx.score_batch([good, np.zeros((7, 64, 3), np.uint8)], CANDS, 70.0)
+    assert res[0].status == 0 and res[0].chosen_quality > 0
+    assert res[1].status == rc.UNSUPPORTED
