@@ -168,6 +168,55 @@ typedef struct {
 int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
 int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
 
+/*
+ * Serving loop with continuous batching. A worker thread owns one context. Requests (an image and a
+ * --target-ssimulacra2 goal) are admitted in arrival order while their device footprint fits the HBM budget, the
+ * GPU analogue of cli/src/budget.rs:11-50: a request that alone exceeds the budget runs, but by itself. Every tick
+ * scores the next candidate(s) of every admitted request's bisection (jpeg/src/target.rs:37-55) in one batched
+ * launch sequence; requests that arrive meanwhile join at the next tick. Sources and reference pyramids stay
+ * resident in HBM for the life of a request. Each result equals the one the request gets alone.
+ * speculate = 1 scores one bisection midpoint per tick; 3 also scores both possible next midpoints, so a request
+ * takes about half the ticks for 1.5x the work.
+ * This entry point has no single counterpart in the reference (it replaces the per-file rayon tasks of
+ * cli/src/main.rs:57-75 gated by Budget::acquire, cli/src/pipeline.rs:881).
+ */
+typedef struct rcg_server rcg_server;
+typedef struct {
+    size_t hbm_budget;              /* bytes for admitted requests; 0 = 16 GiB */
+    int max_active;                 /* requests in flight at once; 0 = no limit */
+    int speculate;                  /* 1 or 3 */
+} rcg_server_opts;
+typedef struct {
+    uint64_t id;                    /* caller's tag, returned in the completion */
+    const uint8_t* pixels;          /* host Gray8/RGB8, copied by rcg_server_submit */
+    uint32_t width, height, channels;
+    double target;
+    int max_quality;                /* 1..100 (the CLI uses 100) */
+    int subsampling;
+} rcg_request;
+typedef struct {
+    uint64_t id;
+    int status;                     /* rcg_status of this request */
+    int quality;
+    double score;
+    int steps;                      /* candidates scored */
+    int ticks;                      /* batched launches it took part in */
+    double submit_s, admit_s, done_s;   /* steady-clock seconds */
+} rcg_completion;
+typedef struct {
+    uint64_t ticks, requests_done;
+    double mean_batch;              /* mean requests per tick */
+    size_t peak_footprint;          /* largest admitted footprint, bytes */
+    double busy_s;                  /* time spent inside ticks */
+} rcg_server_stats;
+int rcg_server_create(int device, const rcg_server_opts* opts, rcg_server** out);
+/* RCG_INVALID / RCG_UNSUPPORTED for a request that can never run (bad size, under 8x8); nothing is queued then. */
+int rcg_server_submit(rcg_server* server, const rcg_request* request);
+/* Up to max completions, waiting at most timeout_ms for the first one; *n receives the count. */
+int rcg_server_poll(rcg_server* server, rcg_completion* out, int max, int timeout_ms, int* n);
+int rcg_server_get_stats(rcg_server* server, rcg_server_stats* out);
+void rcg_server_destroy(rcg_server* server);
+
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch. enable = 1 starts
  * recording (and clears the totals), 0 stops. rcg_profile_get returns the summed milliseconds and the launch count
  * of one kernel ("k_encode", "k_idct", "k_upsample_color", "k_ssim_scale", "k_ssim_finalize"). */

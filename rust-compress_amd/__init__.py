@@ -70,6 +70,27 @@ class BatchResult(ctypes.Structure):
                 ("sweep_quality", ctypes.c_int), ("status", ctypes.c_int)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("speculate", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("pixels", ctypes.c_void_p), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("channels", ctypes.c_uint32), ("target", ctypes.c_double),
+                ("max_quality", ctypes.c_int), ("subsampling", ctypes.c_int)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("quality", ctypes.c_int),
+                ("score", ctypes.c_double), ("steps", ctypes.c_int), ("ticks", ctypes.c_int),
+                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_uint64), ("requests_done", ctypes.c_uint64), ("mean_batch", ctypes.c_double),
+                ("peak_footprint", ctypes.c_size_t), ("busy_s", ctypes.c_double)]
+
+
 _LIB = None
 
 
@@ -99,6 +120,12 @@ def lib():
         L.rcg_synth_rgb8.argtypes = [ctypes.c_uint64, U, U, P]
         L.rcg_score_batch.argtypes = [P, ctypes.POINTER(BatchJob), I, ctypes.POINTER(BatchResult)]
         L.rcg_score_batch_dev.argtypes = [P, ctypes.POINTER(BatchJob), I, ctypes.POINTER(BatchResult)]
+        L.rcg_server_create.argtypes = [I, ctypes.POINTER(ServerOpts), ctypes.POINTER(P)]
+        L.rcg_server_submit.argtypes = [P, ctypes.POINTER(Request)]
+        L.rcg_server_poll.argtypes = [P, ctypes.POINTER(Completion), I, I, ctypes.POINTER(I)]
+        L.rcg_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+        L.rcg_server_destroy.argtypes = [P]
+        L.rcg_server_destroy.restype = None
         L.rcg_profile.argtypes = [P, I]
         L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
         _LIB = L
@@ -350,6 +377,43 @@ def _coef_image_struct(ci, keep, device):
             s.qtables[slot] = qa.ctypes.data
     s.adobe_transform = ci.adobe_transform
     return s
+
+
+class Server:
+    """Continuous-batching target-quality service on one GPU (rcg_server_*)."""
+
+    def __init__(self, device=0, hbm_budget=0, max_active=0, speculate=1):
+        self._h = ctypes.c_void_p()
+        o = ServerOpts(hbm_budget, max_active, speculate)
+        _check(lib().rcg_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
+
+    def submit(self, req_id, img, target, max_quality=100, subsampling=S420):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape[:2]
+        r = Request(req_id, img.ctypes.data, w, h, _channels(img), float(target), max_quality, subsampling)
+        _check(lib().rcg_server_submit(self._h, ctypes.byref(r)))
+
+    def poll(self, max_items=64, timeout_ms=100):
+        buf = (Completion * max_items)()
+        n = ctypes.c_int()
+        _check(lib().rcg_server_poll(self._h, buf, max_items, timeout_ms, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().rcg_server_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self._h:
+            lib().rcg_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _DEFAULT = None

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librcgpu.so")
-SOURCES = ["api.cpp", "batch.cpp", "jpeg.hip", "ssim2.hip", "synth.cpp"]
+SOURCES = ["api.cpp", "batch.cpp", "server.cpp", "jpeg.hip", "ssim2.hip", "synth.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RCG_OFFLOAD_ARCH", "gfx950")
 
@@ -23,14 +23,33 @@ def command(out=LIB):
             "-o", out] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lpthread"]
 
 
+ROOT = os.path.dirname(HERE)
+API_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "reference_api_test.cpp")
+API_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "reference_api_test")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # hipcc's host pass does not know the packed-fp32 feature flag; that note is expected and dropped.
+    noise = "is not a recognized feature for this target"
+    out = "\n".join(l for l in p.stdout.splitlines() if noise not in l)
+    if out.strip():
+        print(out, flush=True)
+    if p.returncode != 0:
+        raise subprocess.CalledProcessError(p.returncode, cmd)
+
+
 def build(verbose=False):
     cmd = command()
     tmp = LIB + ".tmp"
     cmd[cmd.index("-o") + 1] = tmp
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    _run(cmd, verbose)
     os.replace(tmp, LIB)
+    # C++ tests of the host API (host/rc_gpu.hpp), linked against the library.
+    _run(["g++", "-std=c++17", "-O2", "-Wall", API_TEST_SRC, "-o", API_TEST_BIN, "-L" + HERE, "-lrcgpu",
+          "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../rust-compress_amd"], verbose)
     return LIB
 
 

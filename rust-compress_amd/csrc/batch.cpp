@@ -85,44 +85,59 @@ extern "C" int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_m
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Batch
+// Batch core
+
+namespace rcg {
+
+size_t ref_pyramid_bytes(const PairPlan& P) {
+    size_t b = 0;
+    for (uint32_t s = 1; s < P.nscales; s++) b += Arena::align(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float));
+    return b;
+}
+
+static size_t partial_bytes(const PairPlan& P) {
+    size_t b = 0;
+    for (uint32_t s = 0; s < P.nscales; s++) b += Arena::align(size_t(P.tiles[s]) * 18 * sizeof(double));
+    return b;
+}
+
+void take_ref_pyramid(Arena& A, const PairPlan& P, float* planes[6]) {
+    for (int s = 0; s < 6; s++) planes[s] = nullptr;
+    for (uint32_t s = 1; s < P.nscales; s++)
+        planes[s] = static_cast<float*>(A.take(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float)));
+}
 
 namespace {
 
 constexpr double kGuard = 0.05;
 
-struct PairRef {
-    int job, cand;
-};
-
-struct JobPlan {
-    bool ok = false;
-    const uint8_t* d_src = nullptr;
+// Transient per-item device buffers of one scoring pass.
+struct ItemBufs {
     PairPlan P32, P64;
-    float* ref_planes[6] = {};
     Layout L[RCG_MAX_CAND];
-    int group[RCG_MAX_CAND];  // encode job index of each candidate
     uint8_t* rec[RCG_MAX_CAND][3] = {};
     float* dist_planes[RCG_MAX_CAND][6] = {};
     double* part32[RCG_MAX_CAND][6] = {};
     double* part64[RCG_MAX_CAND][6] = {};
 };
 
-size_t pyramid_bytes(const PairPlan& P) {
-    size_t b = 0;
-    for (uint32_t s = 1; s < P.nscales; s++) b += Arena::align(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float));
-    return b;
-}
-size_t partial_bytes(const PairPlan& P) {
-    size_t b = 0;
-    for (uint32_t s = 0; s < P.nscales; s++) b += Arena::align(size_t(P.tiles[s]) * 18 * sizeof(double));
-    return b;
+struct PairRef {
+    int item, cand;
+};
+
+size_t desc_bytes(const Item* items, int n) {
+    size_t d = 1 << 16;
+    for (int i = 0; i < n; i++) {
+        d += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * items[i].ncand + 64 * (RCG_MAX_CAND + 2);
+        d += (sizeof(SsimJob) + 64) * 6 * items[i].ncand + sizeof(FinalDesc) * items[i].ncand + 256;
+    }
+    return d;
 }
 
-// Scores every pair of `pairs` at `precision`. For FP32 the pyramids are written (reference by the image's first
-// candidate); for FP64 both pyramids are read from the FP32 pass.
-int run_pairs(rcg_ctx* ctx, Desc& D, std::vector<JobPlan>& plans, const rcg_batch_job* jobs,
-              const std::vector<PairRef>& pairs, int precision, double* d_scores) {
+// Scores every pair in `pairs` at `precision`. FP32 writes the pyramids (the reference's by each item's first
+// candidate unless already built); FP64 reads both from the FP32 pass.
+int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& bufs, const std::vector<PairRef>& pairs,
+              int precision, double* d_scores) {
     const bool f64 = precision == RCG_PREC_FP64;
     const SsimTune& T = f64 ? ctx->tune64 : ctx->tune32;
     std::vector<FinalDesc> fins(pairs.size());
@@ -139,9 +154,10 @@ int run_pairs(rcg_ctx* ctx, Desc& D, std::vector<JobPlan>& plans, const rcg_batc
         std::vector<uint32_t> begins;
         uint32_t total = 0;
         for (size_t i = 0; i < pairs.size(); i++) {
-            JobPlan& jp = plans[pairs[i].job];
+            const Item& it = items[pairs[i].item];
+            ItemBufs& b = bufs[pairs[i].item];
             const int c = pairs[i].cand;
-            const PairPlan& P = f64 ? jp.P64 : jp.P32;
+            const PairPlan& P = f64 ? b.P64 : b.P32;
             if (uint32_t(s) >= P.nscales) continue;
             SsimJob j;
             std::memset(&j, 0, sizeof(j));
@@ -150,20 +166,19 @@ int run_pairs(rcg_ctx* ctx, Desc& D, std::vector<JobPlan>& plans, const rcg_batc
             j.tiles_x = P.tiles_x[s];
             j.tiles_y = P.tiles_y[s];
             j.tile_begin = total;
-            const rcg_batch_job& bj = jobs[pairs[i].job];
             if (s == 0) {
-                j.a = bytes_src(jp.d_src, bj.channels);
-                j.b = recon_src(jp.L[c], jp.rec[c]);
+                j.a = bytes_src(it.d_src, it.ch);
+                j.b = recon_src(b.L[c], b.rec[c]);
             } else {
-                j.a = planes_src(jp.ref_planes[s]);
-                j.b = planes_src(jp.dist_planes[c][s]);
+                j.a = planes_src(it.ref_planes[s]);
+                j.b = planes_src(b.dist_planes[c][s]);
             }
             const bool next = uint32_t(s + 1) < P.nscales;
             if (!f64 && next) {
-                j.b_next = jp.dist_planes[c][s + 1];
-                if (c == 0) j.a_next = jp.ref_planes[s + 1];
+                j.b_next = b.dist_planes[c][s + 1];
+                if (c == 0 && !it.ref_ready) j.a_next = it.ref_planes[s + 1];
             }
-            j.partials = f64 ? jp.part64[c][s] : jp.part32[c][s];
+            j.partials = f64 ? b.part64[c][s] : b.part32[c][s];
             FinalDesc& f = fins[i];
             f.nscales = P.nscales;
             f.ntiles[s] = P.tiles[s];
@@ -187,134 +202,85 @@ int run_pairs(rcg_ctx* ctx, Desc& D, std::vector<JobPlan>& plans, const rcg_batc
     const FinalDesc* dfin = D.put(fins.data(), fins.size());
     RCG_TRY(D.upload(ctx->stream));
     for (int s = 0; s < 6; s++)
-        if (st[s].n) RCG_TRY(launch_ssim_scale(ctx, precision, st[s].jobs, st[s].begins, st[s].n, st[s].tiles, T.nt, T.th));
+        if (st[s].n)
+            RCG_TRY(launch_ssim_scale(ctx, precision, st[s].jobs, st[s].begins, st[s].n, st[s].tiles, T.nt, T.th));
     return launch_finalize(ctx, dfin, int(pairs.size()));
 }
 
-// Selection of jpeg/src/target.rs:37-55 over the candidate list sorted by quality.
-void select(const rcg_batch_job& j, rcg_batch_result& r) {
-    std::vector<int> order(j.ncand);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return j.quality[a] < j.quality[b]; });
-    const int n = int(j.ncand);
-    int lo = 0, hi = n - 1;
-    int best = hi;
-    if (r.scores[order[hi]] >= j.target) {
-        while (lo < hi) {
-            const int mid = lo + (hi - lo) / 2;
-            if (r.scores[order[mid]] >= j.target) {
-                hi = mid;
-                best = mid;
-            } else {
-                lo = mid + 1;
-            }
+}  // namespace
+
+size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n) {
+    size_t need = 0, npairs = 0;
+    for (int i = 0; i < n; i++) {
+        const Item& it = items[i];
+        const PairPlan P32 = plan_pair(it.w, it.h, ctx->tune32.nt, ctx->tune32.th);
+        const PairPlan P64 = plan_pair(it.w, it.h, ctx->tune64.nt, ctx->tune64.th);
+        for (uint32_t c = 0; c < it.ncand; c++) {
+            const Layout L = make_layout(it.w, it.h, it.ch, it.quality[c], it.subsampling[c]);
+            for (uint32_t k = 0; k < L.ncomp; k++) need += Arena::align(L.plane_bytes(k));
+            need += ref_pyramid_bytes(P32) + partial_bytes(P32) + partial_bytes(P64);
         }
+        npairs += it.ncand;
     }
-    r.chosen_index = order[best];
-    r.chosen_quality = j.quality[order[best]];
-    int sweep = n - 1;
-    for (int k = 0; k < n; k++)
-        if (r.scores[order[k]] >= j.target) {
-            sweep = k;
-            break;
-        }
-    r.sweep_index = order[sweep];
-    r.sweep_quality = j.quality[order[sweep]];
+    return need + Arena::align(desc_bytes(items, n)) + 2 * Arena::align(npairs * sizeof(double) + 8) + (1 << 12);
 }
 
-int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results, bool host_src) {
-    if (!ctx || (!jobs && njobs) || (!results && njobs) || njobs < 0) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    std::vector<JobPlan> plans(njobs);
-    size_t need = 0, desc = 1 << 16;
+// Score every candidate of every item: one encode launch, one launch per scale, one pooling launch; then FP64
+// settling of the scores inside the guard band of each item's target. The arena must already hold
+// score_items_bytes() beyond what the caller took.
+int score_items(rcg_ctx* ctx, Item* items, int n) {
     size_t npairs = 0;
-    for (int i = 0; i < njobs; i++) {
-        const rcg_batch_job& j = jobs[i];
-        rcg_batch_result& r = results[i];
-        std::memset(&r, 0, sizeof(r));
-        r.chosen_index = r.sweep_index = -1;
-        int st = check_image(j.width, j.height, j.channels);
-        if (st == RCG_OK && (j.width < 8 || j.height < 8)) st = RCG_UNSUPPORTED;
-        if (st == RCG_OK && (j.ncand == 0 || j.ncand > RCG_MAX_CAND || !j.pixels)) st = RCG_INVALID;
-        r.status = st;
-        if (st != RCG_OK) continue;
-        JobPlan& p = plans[i];
-        p.ok = true;
-        p.P32 = plan_pair(j.width, j.height, ctx->tune32.nt, ctx->tune32.th);
-        p.P64 = plan_pair(j.width, j.height, ctx->tune64.nt, ctx->tune64.th);
-        if (host_src) need += Arena::align(size_t(j.width) * j.height * j.channels);
-        need += pyramid_bytes(p.P32);
-        for (uint32_t c = 0; c < j.ncand; c++) {
-            p.L[c] = make_layout(j.width, j.height, j.channels, j.quality[c], j.subsampling[c]);
-            for (uint32_t k = 0; k < p.L[c].ncomp; k++) need += Arena::align(p.L[c].plane_bytes(k));
-            need += pyramid_bytes(p.P32) + partial_bytes(p.P32) + partial_bytes(p.P64);
-        }
-        npairs += j.ncand;
-        desc += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * j.ncand + 64 * (RCG_MAX_CAND + 2);
-        desc += (sizeof(SsimJob) + 64) * 6 * j.ncand + sizeof(FinalDesc) * 2 * j.ncand + 256;
-    }
-    desc += 16 * 64 + npairs * 64;
-    need += Arena::align(npairs * sizeof(double)) * 2 + Arena::align(desc) + (1 << 16);
-    ctx->arena.reset();
-    RCG_TRY(ctx->arena.ensure(need));
-    RCG_TRY(ctx->pinned.ensure(desc + Arena::align(npairs * sizeof(double)) * 2));
+    for (int i = 0; i < n; i++) npairs += items[i].ncand;
+    const size_t dbytes = desc_bytes(items, n);
+    RCG_TRY(ctx->pinned.ensure(Arena::align(dbytes) + Arena::align(npairs * sizeof(double) + 8)));
     Desc D;
     D.host = ctx->pinned.base;
-    D.dev = static_cast<char*>(ctx->arena.take(desc));
-    D.cap = desc;
+    D.dev = static_cast<char*>(ctx->arena.take(dbytes));
+    D.cap = dbytes;
     double* d_s32 = static_cast<double*>(ctx->arena.take(npairs * sizeof(double) + 8));
     double* d_s64 = static_cast<double*>(ctx->arena.take(npairs * sizeof(double) + 8));
-    double* h_scores = reinterpret_cast<double*>(ctx->pinned.base + ((desc + 255) & ~size_t(255)));
+    double* h_scores = reinterpret_cast<double*>(ctx->pinned.base + Arena::align(dbytes));
 
-    // Buffers and the encode jobs.
+    std::vector<ItemBufs> bufs(n);
     std::vector<EncJob> ej;
     std::vector<std::vector<EncCand>> ecands;
     std::vector<PairRef> pairs;
-    for (int i = 0; i < njobs; i++) {
-        JobPlan& p = plans[i];
-        if (!p.ok) continue;
-        const rcg_batch_job& j = jobs[i];
-        if (host_src) {
-            const size_t n = size_t(j.width) * j.height * j.channels;
-            uint8_t* d = static_cast<uint8_t*>(ctx->arena.take(n));
-            RCG_HIP(hipMemcpyAsync(d, j.pixels, n, hipMemcpyHostToDevice, ctx->stream));
-            p.d_src = d;
-        } else {
-            p.d_src = j.pixels;
-        }
-        for (uint32_t s = 1; s < p.P32.nscales; s++)
-            p.ref_planes[s] = static_cast<float*>(ctx->arena.take(size_t(p.P32.sw[s]) * p.P32.sh[s] * 3 * 4));
-        for (uint32_t c = 0; c < j.ncand; c++) {
-            for (uint32_t k = 0; k < p.L[c].ncomp; k++)
-                p.rec[c][k] = static_cast<uint8_t*>(ctx->arena.take(p.L[c].plane_bytes(k)));
-            for (uint32_t s = 1; s < p.P32.nscales; s++)
-                p.dist_planes[c][s] =
-                    static_cast<float*>(ctx->arena.take(size_t(p.P32.sw[s]) * p.P32.sh[s] * 3 * 4));
-            for (uint32_t s = 0; s < p.P32.nscales; s++)
-                p.part32[c][s] = static_cast<double*>(ctx->arena.take(size_t(p.P32.tiles[s]) * 18 * 8));
-            for (uint32_t s = 0; s < p.P64.nscales; s++)
-                p.part64[c][s] = static_cast<double*>(ctx->arena.take(size_t(p.P64.tiles[s]) * 18 * 8));
+    for (int i = 0; i < n; i++) {
+        Item& it = items[i];
+        ItemBufs& b = bufs[i];
+        b.P32 = plan_pair(it.w, it.h, ctx->tune32.nt, ctx->tune32.th);
+        b.P64 = plan_pair(it.w, it.h, ctx->tune64.nt, ctx->tune64.th);
+        const int first_group = int(ej.size());
+        for (uint32_t c = 0; c < it.ncand; c++) {
+            it.scores[c] = -DBL_MAX;
+            it.settled[c] = 0;
+            b.L[c] = make_layout(it.w, it.h, it.ch, it.quality[c], it.subsampling[c]);
+            for (uint32_t k = 0; k < b.L[c].ncomp; k++)
+                b.rec[c][k] = static_cast<uint8_t*>(ctx->arena.take(b.L[c].plane_bytes(k)));
+            for (uint32_t s = 1; s < b.P32.nscales; s++)
+                b.dist_planes[c][s] =
+                    static_cast<float*>(ctx->arena.take(size_t(b.P32.sw[s]) * b.P32.sh[s] * 3 * sizeof(float)));
+            for (uint32_t s = 0; s < b.P32.nscales; s++)
+                b.part32[c][s] = static_cast<double*>(ctx->arena.take(size_t(b.P32.tiles[s]) * 18 * 8));
+            for (uint32_t s = 0; s < b.P64.nscales; s++)
+                b.part64[c][s] = static_cast<double*>(ctx->arena.take(size_t(b.P64.tiles[s]) * 18 * 8));
             pairs.push_back(PairRef{i, int(c)});
-        }
-        // One encode job per distinct sampling layout of this image.
-        int first_group = int(ej.size());
-        for (uint32_t c = 0; c < j.ncand; c++) {
+            // Candidates with the same sampling layout share one encode job (colour + FDCT once).
             int g = -1;
             for (int e = first_group; e < int(ej.size()); e++)
-                if (ej[e].hmax == p.L[c].hmax && ej[e].vmax == p.L[c].vmax && ej[e].ncand < uint32_t(kMaxCand)) g = e;
+                if (ej[e].hmax == b.L[c].hmax && ej[e].vmax == b.L[c].vmax && ej[e].ncand < uint32_t(kMaxCand)) g = e;
             if (g < 0) {
                 EncJob e;
                 std::memset(&e, 0, sizeof(e));
-                e.src = p.d_src;
-                e.src_bytes = uint64_t(j.width) * j.height * j.channels;
-                e.w = j.width;
-                e.h = j.height;
-                e.channels = j.channels;
-                e.hmax = p.L[c].hmax;
-                e.vmax = p.L[c].vmax;
-                e.pw = p.L[c].pw;
-                e.ph = p.L[c].ph;
+                e.src = it.d_src;
+                e.src_bytes = uint64_t(it.w) * it.h * it.ch;
+                e.w = it.w;
+                e.h = it.h;
+                e.channels = it.ch;
+                e.hmax = b.L[c].hmax;
+                e.vmax = b.L[c].vmax;
+                e.pw = b.L[c].pw;
+                e.ph = b.L[c].ph;
                 e.tiles_x = (e.pw + kEncTileW - 1) / kEncTileW;
                 e.tiles_y = e.ph / (8 * e.vmax);
                 ej.push_back(e);
@@ -324,12 +290,11 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
             EncCand k;
             std::memset(&k, 0, sizeof(k));
             uint8_t qy[64], qc[64];
-            quant_tables(j.quality[c], qy, qc);
+            quant_tables(it.quality[c], qy, qc);
             fill_cand_tables(qy, qc, k);
-            for (uint32_t q = 0; q < p.L[c].ncomp; q++) k.rec[q] = p.rec[c][q];
+            for (uint32_t q = 0; q < b.L[c].ncomp; q++) k.rec[q] = b.rec[c][q];
             ecands[g].push_back(k);
             ej[g].ncand++;
-            p.group[c] = g;
         }
     }
     std::vector<uint32_t> ebegins(ej.size());
@@ -340,42 +305,135 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
         ebegins[e] = etiles;
         etiles += ej[e].tiles_x * ej[e].tiles_y;
     }
-    const EncJob* d_ej = ej.empty() ? nullptr : D.put(ej.data(), ej.size());
-    const uint32_t* d_eb = ej.empty() ? nullptr : D.put(ebegins.data(), ebegins.size());
     if (!ej.empty()) {
+        const EncJob* d_ej = D.put(ej.data(), ej.size());
+        const uint32_t* d_eb = D.put(ebegins.data(), ebegins.size());
         RCG_TRY(D.upload(ctx->stream));
         RCG_TRY(launch_encode(ctx, d_ej, d_eb, int(ej.size()), etiles));
     }
     if (!pairs.empty()) {
-        RCG_TRY(run_pairs(ctx, D, plans, jobs, pairs, RCG_PREC_FP32, d_s32));
+        RCG_TRY(run_pairs(ctx, D, items, bufs, pairs, RCG_PREC_FP32, d_s32));
         RCG_HIP(hipMemcpyAsync(h_scores, d_s32, pairs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     }
     RCG_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<PairRef> settle;
-    std::vector<size_t> settle_idx;
     for (size_t i = 0; i < pairs.size(); i++) {
         const double s = h_scores[i];
-        rcg_batch_result& r = results[pairs[i].job];
-        r.scores[pairs[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
-        if (std::isfinite(s) && std::fabs(s - jobs[pairs[i].job].target) < kGuard) {
-            settle.push_back(pairs[i]);
-            settle_idx.push_back(i);
-        }
+        Item& it = items[pairs[i].item];
+        it.scores[pairs[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
+        if (std::isfinite(s) && std::fabs(s - it.target) < kGuard) settle.push_back(pairs[i]);
     }
+    for (int i = 0; i < n; i++) items[i].ref_ready = true;
     if (!settle.empty()) {
         D.off = 0;
-        RCG_TRY(run_pairs(ctx, D, plans, jobs, settle, RCG_PREC_FP64, d_s64));
+        RCG_TRY(run_pairs(ctx, D, items, bufs, settle, RCG_PREC_FP64, d_s64));
         RCG_HIP(hipMemcpyAsync(h_scores, d_s64, settle.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         RCG_HIP(hipStreamSynchronize(ctx->stream));
         for (size_t i = 0; i < settle.size(); i++) {
             const double s = h_scores[i];
-            rcg_batch_result& r = results[settle[i].job];
-            r.scores[settle[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
-            r.settled[settle[i].cand] = 1;
+            Item& it = items[settle[i].item];
+            it.scores[settle[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
+            it.settled[settle[i].cand] = 1;
         }
     }
-    for (int i = 0; i < njobs; i++)
-        if (plans[i].ok) select(jobs[i], results[i]);
+    return RCG_OK;
+}
+
+// Selection of jpeg/src/target.rs:37-55 over a candidate list sorted by quality: the highest first, then bisection
+// over list indices. Returns the chosen index; *sweep receives the exhaustive choice (lowest passing candidate).
+int select_candidate(const int* quality, const double* scores, int n, double target, int* sweep) {
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return quality[a] < quality[b]; });
+    int lo = 0, hi = n - 1, best = hi;
+    if (scores[order[hi]] >= target) {
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (scores[order[mid]] >= target) {
+                hi = mid;
+                best = mid;
+            } else {
+                lo = mid + 1;
+            }
+        }
+    }
+    int sw = n - 1;
+    for (int k = 0; k < n; k++)
+        if (scores[order[k]] >= target) {
+            sw = k;
+            break;
+        }
+    if (sweep) *sweep = order[sw];
+    return order[best];
+}
+
+}  // namespace rcg
+
+// ---------------------------------------------------------------------------------------------------------------
+// rcg_score_batch
+
+namespace {
+
+int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results, bool host_src) {
+    if (!ctx || (!jobs && njobs) || (!results && njobs) || njobs < 0) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    std::vector<Item> items;
+    std::vector<int> job_of;
+    size_t own = 0;
+    for (int i = 0; i < njobs; i++) {
+        const rcg_batch_job& j = jobs[i];
+        rcg_batch_result& r = results[i];
+        std::memset(&r, 0, sizeof(r));
+        r.chosen_index = r.sweep_index = -1;
+        int st = check_image(j.width, j.height, j.channels);
+        if (st == RCG_OK && (j.width < 8 || j.height < 8)) st = RCG_UNSUPPORTED;
+        if (st == RCG_OK && (j.ncand == 0 || j.ncand > RCG_MAX_CAND || !j.pixels)) st = RCG_INVALID;
+        r.status = st;
+        if (st != RCG_OK) continue;
+        Item it;
+        std::memset(&it, 0, sizeof(it));
+        it.d_src = j.pixels;
+        it.w = j.width;
+        it.h = j.height;
+        it.ch = j.channels;
+        it.ncand = j.ncand;
+        for (uint32_t c = 0; c < j.ncand; c++) {
+            it.quality[c] = j.quality[c];
+            it.subsampling[c] = j.subsampling[c];
+        }
+        it.target = j.target;
+        items.push_back(it);
+        job_of.push_back(i);
+        if (host_src) own += Arena::align(size_t(j.width) * j.height * j.channels);
+        own += ref_pyramid_bytes(plan_pair(j.width, j.height, ctx->tune32.nt, ctx->tune32.th));
+    }
+    const size_t need = own + score_items_bytes(ctx, items.data(), int(items.size()));
+    ctx->arena.reset();
+    RCG_TRY(ctx->arena.ensure(need));
+    for (size_t k = 0; k < items.size(); k++) {
+        Item& it = items[k];
+        if (host_src) {
+            const size_t n = size_t(it.w) * it.h * it.ch;
+            uint8_t* d = static_cast<uint8_t*>(ctx->arena.take(n));
+            RCG_HIP(hipMemcpyAsync(d, jobs[job_of[k]].pixels, n, hipMemcpyHostToDevice, ctx->stream));
+            it.d_src = d;
+        }
+        take_ref_pyramid(ctx->arena, plan_pair(it.w, it.h, ctx->tune32.nt, ctx->tune32.th), it.ref_planes);
+        it.ref_ready = false;
+    }
+    RCG_TRY(score_items(ctx, items.data(), int(items.size())));
+    for (size_t k = 0; k < items.size(); k++) {
+        const Item& it = items[k];
+        rcg_batch_result& r = results[job_of[k]];
+        for (uint32_t c = 0; c < it.ncand; c++) {
+            r.scores[c] = it.scores[c];
+            r.settled[c] = it.settled[c];
+        }
+        r.chosen_index = select_candidate(it.quality, it.scores, int(it.ncand), it.target, &r.sweep_index);
+        r.chosen_quality = it.quality[r.chosen_index];
+        r.sweep_quality = it.quality[r.sweep_index];
+    }
     return RCG_OK;
 }
 

@@ -184,6 +184,26 @@ SsimSrc bytes_src(const uint8_t* d, uint32_t channels);
 SsimSrc planes_src(const float* planes);
 int check_image(uint32_t w, uint32_t h, uint32_t channels);
 
+// One image of a scoring pass: its resident source, its reference pyramid (scales 1..5, built by the pass when
+// ref_ready is false) and the candidates to score. Scores come back in scores[]; settled[c] marks FP64 ones.
+struct Item {
+    const uint8_t* d_src;
+    uint32_t w, h, ch;
+    uint32_t ncand;
+    int quality[RCG_MAX_CAND];
+    int subsampling[RCG_MAX_CAND];
+    double target;
+    float* ref_planes[6];
+    bool ref_ready;
+    double scores[RCG_MAX_CAND];
+    int settled[RCG_MAX_CAND];
+};
+size_t ref_pyramid_bytes(const PairPlan& P);
+void take_ref_pyramid(Arena& A, const PairPlan& P, float* planes[6]);
+size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n);
+int score_items(rcg_ctx* ctx, Item* items, int n);
+int select_candidate(const int* quality, const double* scores, int n, double target, int* sweep);
+
 #define RCG_TRY(expr)                    \
     do {                                 \
         int rc_ = (expr);                \

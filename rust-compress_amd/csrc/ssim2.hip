@@ -14,7 +14,8 @@
 //      reference's order, so the pyramid is bit-identical to the reference's f32 planes).
 // Each tile writes its 18 sums as f64; k_ssim_finalize reduces them in a fixed order and pools the scales.
 //
-// T = float is the precision form (f32 XYB, blurs and maps; f32 sums within a tile column, f64 above that);
+// T = float is the precision form (f32 XYB shifted by per-channel constants, blurs and maps; f32 sums within a tile
+// column, f64 above that);
 // T = double follows the reference's f64 arithmetic and is used to settle candidates whose f32 score lies within
 // the guard band of a search target.
 #include <hip/hip_runtime.h>
@@ -36,8 +37,13 @@ __constant__ double c_weights[108];
 template <typename T>
 struct Xyb;
 
+// The f32 form works on XYB shifted by per-channel constants (X - 0.42, Y - 0.51, B - 0.55). Every map depends on
+// the values only through differences and variances (mu1 - mu2, a - mu1, blur(a^2) - mu1^2, blur(ab) - mu1*mu2),
+// which a constant shift leaves unchanged provided the samples outside the image hold -offset instead of 0; the
+// shift keeps blur(a^2) and mu1^2 near 0.01 instead of 0.3, so their f32 difference keeps ~5 more bits.
 template <>
 struct Xyb<float> {
+    static __device__ __forceinline__ float off(int c) { return c == 0 ? 0.42f : (c == 1 ? 0.51f : 0.55f); }
     static __device__ __forceinline__ float cbrt_pos(float x) {
         // x >= 0; exp2(log2(x)/3) on the transcendental unit. log2(0) = -inf gives 0.
         return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
@@ -54,14 +60,15 @@ struct Xyb<float> {
         const float m1 = cbrt_pos(fmaxf(M10 * r + M11 * g + M12 * b + BIAS, 0.0f)) - CB;
         const float m2 = cbrt_pos(fmaxf(M20 * r + M21 * g + M22 * b + BIAS, 0.0f)) - CB;
         const float x = 0.5f * (m0 - m1), y = 0.5f * (m0 + m1);
-        X = x * 14.0f + 0.42f;
-        Y = y + 0.01f;
-        B = (m2 - y) + 0.55f;
+        X = x * 14.0f;      // (x*14 + 0.42) - 0.42
+        Y = y - 0.5f;       // (y + 0.01) - 0.51
+        B = m2 - y;         // ((m2 - y) + 0.55) - 0.55
     }
 };
 
 template <>
 struct Xyb<double> {
+    static __device__ __forceinline__ double off(int) { return 0.0; }
     static __device__ __forceinline__ void eval(double r, double g, double b, double& X, double& Y, double& B) {
         const double M00 = 0.30, M02 = 0.078, M01 = 1.0 - M02 - M00;
         const double M10 = 0.23, M12 = 0.078, M11 = 1.0 - M12 - M10;
@@ -158,10 +165,11 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
         Xyb<T>::eval(T(la[0]), T(la[1]), T(la[2]), st.ra[S][0], st.ra[S][1], st.ra[S][2]);
         Xyb<T>::eval(T(lb[0]), T(lb[1]), T(lb[2]), st.rb[S][0], st.rb[S][1], st.rb[S][2]);
     } else {
+        // Beyond the left/right edges the reference's samples are 0, i.e. -offset in the shifted domain.
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            st.ra[S][c] = T(0);
-            st.rb[S][c] = T(0);
+            st.ra[S][c] = -Xyb<T>::off(c);
+            st.rb[S][c] = -Xyb<T>::off(c);
         }
     }
     // Next-scale 2x2 mean (uniform condition: the row is one of this tile's own rows).
