@@ -30,6 +30,10 @@ SSIM_BYTES_PER_PX = 18.0
 EVAL_BYTES_PER_PX = {3: 30.0, 1: 36.0}
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,7 +42,22 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-image search")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-image batched search")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--batch-images", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB mebibytes (HBM traffic counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(images shard across GPUs with no collective; see DESIGN.md)", file=sys.stderr)
@@ -72,6 +91,7 @@ def main():
     fin_ms, fin_n = ctx.profile_get("k_ssim_finalize")
     ctx.profile(False)
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    progress(f"headline workload timed: {ms_per_step:.3f} ms/step")
     mp = W * H / 1e6
     evals = len(cands)
     value = mp * evals / (ms_per_step / 1000.0)
@@ -120,6 +140,7 @@ def main():
                                "eval_mps": round(mp * f.steps / (t1 - t0), 2)}
 
     if not args.no_cpu:
+        progress("timing the CPU oracle sample")
         from oracle import oracle as O  # the CPU baseline is the oracle, timed, never the measured path
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sample = [(50, O.S420), (90, O.S444)]
@@ -131,8 +152,125 @@ def main():
                                "kind": "port",
                                "sample": f"{len(sample)} candidate evaluations (q50 4:2:0, q90 4:4:4) of the same "
                                          f"{W}x{H} image: oracle build_coefs + reconstruct + f64 SSIMULACRA2"}
+    if not args.no_batch:
+        progress("batched configuration")
+        out["batched"] = bench_batch(rc, ctx, torch, args)
+    if not args.no_serve:
+        progress("serving configuration")
+        out["serving"] = bench_serve(rc, args)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+CANDS_B = (40, 50, 60, 70, 80, 85, 90, 95)
+
+
+def bench_batch(rc, ctx, torch, args):
+    """256 synthetic RGB8 images, W and H uniform in [64, 3000], each swept over 8 candidate qualities at 4:2:0 in
+    one batched pass; choice = lowest q with SSIMULACRA2 >= 70 by the reference's bisection over the sorted list.
+    Compared with the plain per-image GPU search (serial bisection over q in 1..100) on the same images."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    sizes = [(int(rng.integers(64, 3001)), int(rng.integers(64, 3001))) for _ in range(args.batch_images)]
+    t0 = time.perf_counter()
+    dev = []
+    for i, (w, h) in enumerate(sizes):
+        dev.append(torch.from_numpy(rc.synth_rgb8(1000 + i, w, h)).to("cuda"))
+        if i % 64 == 63:
+            progress(f"generated {i + 1} batch images")
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    ptrs = [(t.data_ptr(), w, h, 3) for t, (w, h) in zip(dev, sizes)]
+    cands = [(q, rc.S420) for q in CANDS_B]
+    total_mp = sum(w * h for w, h in sizes) / 1e6
+    ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)  # warm-up (sizes the workspace)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    res = ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
+    ctx.profile(False)
+    batch_s = t1 - t0
+    out = {"images": len(sizes), "megapixels": round(total_mp, 2), "candidates": list(CANDS_B),
+           "seconds": round(batch_s, 4), "search_mps": round(total_mp / batch_s, 2),
+           "eval_mps": round(total_mp * len(CANDS_B) / batch_s, 2),
+           "hbm_frac": round(total_mp * 1e6 * (30.0 * len(CANDS_B) + 7.0) / batch_s / 1e9 / HBM_PEAK_GBS, 4),
+           "k_ssim_scale_ms": round(ssim_ms, 2), "k_ssim_scale_launches": ssim_n,
+           "settled_fp64": int(sum(sum(r.settled[:len(CANDS_B)]) for r in res)),
+           "chosen_equals_sweep": int(sum(r.chosen_quality == r.sweep_quality for r in res)),
+           "generate_s": round(gen_s, 1)}
+    progress(f"batched pass: {batch_s:.3f} s")
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        steps = 0
+        for p in ptrs:
+            f = ctx.find_quality_dev(p[0], p[1], p[2], 3, 70.0, 100, rc.S420)
+            steps += f.steps
+        t1 = time.perf_counter()
+        out["plain_search_mps"] = round(total_mp / (t1 - t0), 2)
+        out["plain_eval_mps"] = round(total_mp * steps / len(ptrs) / (t1 - t0), 2)
+        out["plain_seconds"] = round(t1 - t0, 3)
+        out["batched_vs_plain"] = round(out["search_mps"] / out["plain_search_mps"], 2)
+    del dev
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_serve(rc, args):
+    """Poisson arrivals at 200 req/s; megapixels log-uniform in [0.1, 24], aspect uniform in [0.5, 2]; targets
+    uniform over {60, 70, 80}. Images come from a pool of 32 synthetic images drawn from that distribution (the
+    host cannot hold one image per request); every request is an independent search. Latency = completion -
+    submission."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(77)
+    pool = []
+    for i in range(32):
+        mp = float(np.exp(rng.uniform(np.log(0.1), np.log(24.0))))
+        aspect = float(rng.uniform(0.5, 2.0))
+        w = max(8, int(round((mp * 1e6 * aspect) ** 0.5)))
+        h = max(8, int(round(mp * 1e6 / w)))
+        pool.append(rc.synth_rgb8(5000 + i, w, h))
+    n = args.serve_requests
+    picks = rng.integers(0, len(pool), n)
+    targets = rng.choice([60.0, 70.0, 80.0], n)
+    gaps = rng.exponential(1.0 / 200.0, n)
+    srv = rc.Server(0, speculate=3)
+    # Warm-up: one request of each pool image size class is not needed; run one request to initialise.
+    srv.submit(10 ** 9, pool[0], 70.0)
+    while not srv.poll(8, 500):
+        pass
+    done = {}
+
+    def submitter():
+        t = time.perf_counter()
+        for i in range(n):
+            t += gaps[i]
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(i, pool[picks[i]], targets[i])
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    while len(done) < n:
+        for c in srv.poll(128, 200):
+            done[c.id] = c
+    t1 = time.perf_counter()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([(c.done_s - c.submit_s) * 1000.0 for c in done.values()])
+    mp = sum(pool[picks[i]].shape[0] * pool[picks[i]].shape[1] for i in range(n)) / 1e6
+    return {"requests": n, "offered_req_s": 200.0, "seconds": round(t1 - t0, 2),
+            "achieved_req_s": round(n / (t1 - t0), 1), "search_mps": round(mp / (t1 - t0), 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "mean_batch": round(st.mean_batch, 1), "ticks": int(st.ticks),
+            "peak_footprint_gib": round(st.peak_footprint / 2 ** 30, 2),
+            "failed": int(sum(c.status != 0 for c in done.values()))}
 
 
 if __name__ == "__main__":

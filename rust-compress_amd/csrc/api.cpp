@@ -111,7 +111,7 @@ Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int s
     return L;
 }
 
-PairPlan plan_pair(uint32_t w, uint32_t h, int nt, int th) {
+PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
     PairPlan P{};
     P.w = w;
     P.h = h;
@@ -120,8 +120,15 @@ PairPlan plan_pair(uint32_t w, uint32_t h, int nt, int th) {
         if (sw < 8 || sh < 8) break;
         P.sw[s] = sw;
         P.sh[s] = sh;
-        P.tiles_x[s] = ssim_tiles_x(sw, nt);
-        P.tiles_y[s] = ssim_tiles_y(sh, th);
+        P.nt[s] = T.fixed_nt ? uint32_t(T.fixed_nt) : (sw <= T.narrow_w ? 128u : uint32_t(T.wide_nt));
+        P.th[s] = uint32_t(T.th[3]);
+        for (int i = 0; i < 4; i++)
+            if (sh >= T.th_min_h[i]) {
+                P.th[s] = uint32_t(T.th[i]);
+                break;
+            }
+        P.tiles_x[s] = ssim_tiles_x(sw, int(P.nt[s]));
+        P.tiles_y[s] = ssim_tiles_y(sh, int(P.th[s]));
         P.tiles[s] = P.tiles_x[s] * P.tiles_y[s];
         P.nscales++;
         sw = (sw + 1) / 2;
@@ -217,6 +224,7 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
         std::memset(&j, 0, sizeof(j));
         j.w = P.sw[s];
         j.h = P.sh[s];
+        j.th = P.th[s];
         j.tiles_x = P.tiles_x[s];
         j.tiles_y = P.tiles_y[s];
         j.tile_begin = 0;
@@ -250,9 +258,8 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
 }
 
 int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
-    const SsimTune& T = st.precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
     for (uint32_t s = 0; s < st.P.nscales; s++)
-        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], st.begins, 1, st.P.tiles[s], T.nt, T.th));
+        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], st.begins, 1, st.P.tiles[s], int(st.P.nt[s])));
     return launch_finalize(ctx, st.fin, 1);
 }
 
@@ -387,7 +394,7 @@ namespace {
 int ssim_pair_dev(rcg_ctx* ctx, Call& call, const uint8_t* d_a, const uint8_t* d_b, uint32_t w, uint32_t h,
                   uint32_t channels, int precision, double* score, double* per_scale, int* nscales) {
     const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
-    const PairPlan P = plan_pair(w, h, T.nt, T.th);
+    const PairPlan P = plan_pair(w, h, T);
     SsimBuffers B = take_ssim(ctx->arena, P);
     StagedSsim st = stage_ssim(call.D, P, bytes_src(d_a, channels), bytes_src(d_b, channels), B, false, precision);
     RCG_TRY(call.D.upload(ctx->stream));
@@ -451,10 +458,7 @@ int rcg_create(int device, rcg_ctx** out) {
     }
     make_taps(c->taps64);
     for (int i = 0; i < 6; i++) c->taps32[i] = float(c->taps64[i]);
-    c->tune32.nt = 256;
-    c->tune32.th = 64;
-    c->tune64.nt = 128;
-    c->tune64.th = 64;
+    c->tune64.fixed_nt = 128;
     int rc = upload_weights();
     if (rc != RCG_OK) {
         rcg_destroy(c);
@@ -672,7 +676,7 @@ int rcg_ssimulacra2_dev(rcg_ctx* ctx, const uint8_t* d_reference, const uint8_t*
         return RCG_INVALID;
     }
     const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
-    const PairPlan P = plan_pair(width, height, T.nt, T.th);
+    const PairPlan P = plan_pair(width, height, T);
     Call call(ctx);
     RCG_TRY(call.begin(ssim_bytes(P, false)));
     return ssim_pair_dev(ctx, call, d_reference, d_distorted, width, height, channels, precision, score, per_scale,
@@ -689,7 +693,7 @@ int rcg_ssimulacra2(rcg_ctx* ctx, const uint8_t* reference, const uint8_t* disto
         return RCG_INVALID;
     }
     const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
-    const PairPlan P = plan_pair(width, height, T.nt, T.th);
+    const PairPlan P = plan_pair(width, height, T);
     const size_t n = size_t(width) * height * channels;
     Call call(ctx);
     RCG_TRY(call.begin(ssim_bytes(P, false) + 2 * Arena::align(n)));
@@ -713,7 +717,7 @@ int score_candidate(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_
                     bool& a64_ready, double target, double& score, bool& settled) {
     call.D.off = 0;
     RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, nullptr, rec));
-    const PairPlan P = plan_pair(L.w, L.h, ctx->tune32.nt, ctx->tune32.th);
+    const PairPlan P = plan_pair(L.w, L.h, ctx->tune32);
     StagedSsim st = stage_ssim(call.D, P, bytes_src(d_src, L.channels), recon_src(L, rec), B32, a32_ready,
                                RCG_PREC_FP32);
     RCG_TRY(call.D.upload(ctx->stream));
@@ -725,7 +729,7 @@ int score_candidate(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_
     settled = false;
     if (std::isfinite(s) && std::fabs(s - target) < kGuard) {
         call.D.off = 0;
-        const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64.nt, ctx->tune64.th);
+        const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
         StagedSsim s64 = stage_ssim(call.D, P64, bytes_src(d_src, L.channels), recon_src(L, rec), B64, a64_ready,
                                     RCG_PREC_FP64);
         RCG_TRY(call.D.upload(ctx->stream));
@@ -751,8 +755,8 @@ int find_quality_impl(rcg_ctx* ctx, const uint8_t* pixels, bool host_src, uint32
     }
     // Auto subsampling depends on the quality; size for the larger (4:4:4) layout.
     const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
-    const PairPlan P32 = plan_pair(width, height, ctx->tune32.nt, ctx->tune32.th);
-    const PairPlan P64 = plan_pair(width, height, ctx->tune64.nt, ctx->tune64.th);
+    const PairPlan P32 = plan_pair(width, height, ctx->tune32);
+    const PairPlan P64 = plan_pair(width, height, ctx->tune64);
     const size_t src_bytes = size_t(width) * height * channels;
     const size_t need = Arena::align(src_bytes) + plane_bytes(Lmax) + coef_bytes(Lmax) + ssim_bytes(P32, false) +
                         ssim_bytes(P64, false);

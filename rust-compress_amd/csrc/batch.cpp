@@ -130,6 +130,7 @@ size_t desc_bytes(const Item* items, int n) {
     for (int i = 0; i < n; i++) {
         d += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * items[i].ncand + 64 * (RCG_MAX_CAND + 2);
         d += (sizeof(SsimJob) + 64) * 6 * items[i].ncand + sizeof(FinalDesc) * items[i].ncand + 256;
+        d += 2 * 6 * 128;
     }
     return d;
 }
@@ -139,60 +140,67 @@ size_t desc_bytes(const Item* items, int n) {
 int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& bufs, const std::vector<PairRef>& pairs,
               int precision, double* d_scores) {
     const bool f64 = precision == RCG_PREC_FP64;
-    const SsimTune& T = f64 ? ctx->tune64 : ctx->tune32;
     std::vector<FinalDesc> fins(pairs.size());
     for (size_t i = 0; i < pairs.size(); i++) std::memset(&fins[i], 0, sizeof(FinalDesc));
+    // One launch per (scale, thread count): the tile policy gives narrow scales 128-thread workgroups.
     struct Staged {
         const SsimJob* jobs;
         const uint32_t* begins;
         int n;
         uint32_t tiles;
+        int nt;
     };
-    Staged st[6] = {};
+    std::vector<Staged> st;
     for (int s = 0; s < 6; s++) {
-        std::vector<SsimJob> sj;
-        std::vector<uint32_t> begins;
-        uint32_t total = 0;
-        for (size_t i = 0; i < pairs.size(); i++) {
-            const Item& it = items[pairs[i].item];
-            ItemBufs& b = bufs[pairs[i].item];
-            const int c = pairs[i].cand;
-            const PairPlan& P = f64 ? b.P64 : b.P32;
-            if (uint32_t(s) >= P.nscales) continue;
-            SsimJob j;
-            std::memset(&j, 0, sizeof(j));
-            j.w = P.sw[s];
-            j.h = P.sh[s];
-            j.tiles_x = P.tiles_x[s];
-            j.tiles_y = P.tiles_y[s];
-            j.tile_begin = total;
-            if (s == 0) {
-                j.a = bytes_src(it.d_src, it.ch);
-                j.b = recon_src(b.L[c], b.rec[c]);
-            } else {
-                j.a = planes_src(it.ref_planes[s]);
-                j.b = planes_src(b.dist_planes[c][s]);
+        for (int nt : {256, 128}) {
+            std::vector<SsimJob> sj;
+            std::vector<uint32_t> begins;
+            uint32_t total = 0;
+            for (size_t i = 0; i < pairs.size(); i++) {
+                const Item& it = items[pairs[i].item];
+                ItemBufs& b = bufs[pairs[i].item];
+                const int c = pairs[i].cand;
+                const PairPlan& P = f64 ? b.P64 : b.P32;
+                if (uint32_t(s) >= P.nscales || int(P.nt[s]) != nt) continue;
+                SsimJob j;
+                std::memset(&j, 0, sizeof(j));
+                j.w = P.sw[s];
+                j.h = P.sh[s];
+                j.th = P.th[s];
+                j.tiles_x = P.tiles_x[s];
+                j.tiles_y = P.tiles_y[s];
+                j.tile_begin = total;
+                if (s == 0) {
+                    j.a = bytes_src(it.d_src, it.ch);
+                    j.b = recon_src(b.L[c], b.rec[c]);
+                } else {
+                    j.a = planes_src(it.ref_planes[s]);
+                    j.b = planes_src(b.dist_planes[c][s]);
+                }
+                const bool next = uint32_t(s + 1) < P.nscales;
+                if (!f64 && next) {
+                    j.b_next = b.dist_planes[c][s + 1];
+                    if (c == 0 && !it.ref_ready) j.a_next = it.ref_planes[s + 1];
+                }
+                j.partials = f64 ? b.part64[c][s] : b.part32[c][s];
+                FinalDesc& f = fins[i];
+                f.nscales = P.nscales;
+                f.ntiles[s] = P.tiles[s];
+                f.partials[s] = j.partials;
+                f.count[s] = double(size_t(P.sw[s]) * P.sh[s]);
+                sj.push_back(j);
+                begins.push_back(total);
+                total += P.tiles[s];
             }
-            const bool next = uint32_t(s + 1) < P.nscales;
-            if (!f64 && next) {
-                j.b_next = b.dist_planes[c][s + 1];
-                if (c == 0 && !it.ref_ready) j.a_next = it.ref_planes[s + 1];
+            if (!sj.empty()) {
+                Staged g;
+                g.jobs = D.put(sj.data(), sj.size());
+                g.begins = D.put(begins.data(), begins.size());
+                g.n = int(sj.size());
+                g.tiles = total;
+                g.nt = nt;
+                st.push_back(g);
             }
-            j.partials = f64 ? b.part64[c][s] : b.part32[c][s];
-            FinalDesc& f = fins[i];
-            f.nscales = P.nscales;
-            f.ntiles[s] = P.tiles[s];
-            f.partials[s] = j.partials;
-            f.count[s] = double(size_t(P.sw[s]) * P.sh[s]);
-            sj.push_back(j);
-            begins.push_back(total);
-            total += P.tiles[s];
-        }
-        if (!sj.empty()) {
-            st[s].jobs = D.put(sj.data(), sj.size());
-            st[s].begins = D.put(begins.data(), begins.size());
-            st[s].n = int(sj.size());
-            st[s].tiles = total;
         }
     }
     for (size_t i = 0; i < pairs.size(); i++) {
@@ -201,9 +209,7 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
     }
     const FinalDesc* dfin = D.put(fins.data(), fins.size());
     RCG_TRY(D.upload(ctx->stream));
-    for (int s = 0; s < 6; s++)
-        if (st[s].n)
-            RCG_TRY(launch_ssim_scale(ctx, precision, st[s].jobs, st[s].begins, st[s].n, st[s].tiles, T.nt, T.th));
+    for (const Staged& g : st) RCG_TRY(launch_ssim_scale(ctx, precision, g.jobs, g.begins, g.n, g.tiles, g.nt));
     return launch_finalize(ctx, dfin, int(pairs.size()));
 }
 
@@ -213,8 +219,8 @@ size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n) {
     size_t need = 0, npairs = 0;
     for (int i = 0; i < n; i++) {
         const Item& it = items[i];
-        const PairPlan P32 = plan_pair(it.w, it.h, ctx->tune32.nt, ctx->tune32.th);
-        const PairPlan P64 = plan_pair(it.w, it.h, ctx->tune64.nt, ctx->tune64.th);
+        const PairPlan P32 = plan_pair(it.w, it.h, ctx->tune32);
+        const PairPlan P64 = plan_pair(it.w, it.h, ctx->tune64);
         for (uint32_t c = 0; c < it.ncand; c++) {
             const Layout L = make_layout(it.w, it.h, it.ch, it.quality[c], it.subsampling[c]);
             for (uint32_t k = 0; k < L.ncomp; k++) need += Arena::align(L.plane_bytes(k));
@@ -248,8 +254,8 @@ int score_items(rcg_ctx* ctx, Item* items, int n) {
     for (int i = 0; i < n; i++) {
         Item& it = items[i];
         ItemBufs& b = bufs[i];
-        b.P32 = plan_pair(it.w, it.h, ctx->tune32.nt, ctx->tune32.th);
-        b.P64 = plan_pair(it.w, it.h, ctx->tune64.nt, ctx->tune64.th);
+        b.P32 = plan_pair(it.w, it.h, ctx->tune32);
+        b.P64 = plan_pair(it.w, it.h, ctx->tune64);
         const int first_group = int(ej.size());
         for (uint32_t c = 0; c < it.ncand; c++) {
             it.scores[c] = -DBL_MAX;
@@ -406,7 +412,7 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
         items.push_back(it);
         job_of.push_back(i);
         if (host_src) own += Arena::align(size_t(j.width) * j.height * j.channels);
-        own += ref_pyramid_bytes(plan_pair(j.width, j.height, ctx->tune32.nt, ctx->tune32.th));
+        own += ref_pyramid_bytes(plan_pair(j.width, j.height, ctx->tune32));
     }
     const size_t need = own + score_items_bytes(ctx, items.data(), int(items.size()));
     ctx->arena.reset();
@@ -419,7 +425,7 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
             RCG_HIP(hipMemcpyAsync(d, jobs[job_of[k]].pixels, n, hipMemcpyHostToDevice, ctx->stream));
             it.d_src = d;
         }
-        take_ref_pyramid(ctx->arena, plan_pair(it.w, it.h, ctx->tune32.nt, ctx->tune32.th), it.ref_planes);
+        take_ref_pyramid(ctx->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref_planes);
         it.ref_ready = false;
     }
     RCG_TRY(score_items(ctx, items.data(), int(items.size())));

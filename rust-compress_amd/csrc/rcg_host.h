@@ -68,10 +68,17 @@ struct Prof {
     void collect();  // stream must be synchronised
 };
 
-// Launch shapes of the SSIMULACRA2 scale kernel: threads per workgroup (columns + 12 halo) and rows per tile.
+// Tile policy of the SSIMULACRA2 scale kernel. A scale of w x h pixels runs with NT threads per workgroup
+// (NT - 12 output columns; 128 when w <= narrow_w, else wide_nt) and TH output rows per tile, the largest of
+// th[0..3] whose height threshold h >= th_min_h[i] holds. Small scales get short tiles: their cost is the latency of
+// one workgroup's row walk, not throughput. The policy depends only on the image's own size, so an image's tiling
+// (and its score, to the last bit) does not depend on what else is in a batch.
 struct SsimTune {
-    int nt = 256;
-    int th = 64;
+    int wide_nt = 256;
+    uint32_t narrow_w = 160;
+    int th[4] = {64, 32, 16, 8};
+    uint32_t th_min_h[4] = {2048, 512, 128, 0};
+    int fixed_nt = 0;  // FP64 kernel: one thread count for every scale
 };
 
 }  // namespace rcg
@@ -125,7 +132,7 @@ int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
                           uint8_t* out);
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
-                      uint32_t total_tiles, int nt, int th);
+                      uint32_t total_tiles, int nt);
 struct FinalDesc {
     uint32_t nscales;
     uint32_t ntiles[6];
@@ -145,9 +152,10 @@ inline uint32_t ssim_tiles_y(uint32_t h, int th) { return (h + th - 1) / th; }
 struct PairPlan {
     uint32_t w, h, nscales;
     uint32_t sw[6], sh[6];
+    uint32_t nt[6], th[6];
     uint32_t tiles[6], tiles_x[6], tiles_y[6];
 };
-PairPlan plan_pair(uint32_t w, uint32_t h, int nt, int th);
+PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& tune);
 
 // Descriptor staging: descriptors are written to pinned memory at the offsets they will have in a device region
 // and uploaded with one copy before the launches that read them.

@@ -56,6 +56,7 @@ struct SsimSrc {
 
 struct SsimJob {
     uint32_t w, h;          // size at this scale
+    uint32_t th;            // output rows per tile (even)
     uint32_t tiles_x, tiles_y;
     uint32_t tile_begin;
     SsimSrc a, b;           // a = reference, b = distorted

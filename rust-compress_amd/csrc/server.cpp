@@ -119,7 +119,7 @@ struct rcg_server {
     uint64_t batch_sum = 0;
 
     size_t footprint_of(const rcg_request& r) const {
-        const PairPlan P = plan_pair(r.width, r.height, ctx->tune32.nt, ctx->tune32.th);
+        const PairPlan P = plan_pair(r.width, r.height, ctx->tune32);
         size_t b = Arena::align(size_t(r.width) * r.height * r.channels) + ref_pyramid_bytes(P);
         Item it;
         std::memset(&it, 0, sizeof(it));
@@ -175,7 +175,7 @@ struct rcg_server {
         const size_t n = size_t(r->req.width) * r->req.height * r->req.channels;
         RCG_HIP(hipMalloc(reinterpret_cast<void**>(&r->d_src), n));
         RCG_HIP(hipMemcpyAsync(r->d_src, r->pixels.data(), n, hipMemcpyHostToDevice, ctx->stream));
-        const PairPlan P = plan_pair(r->req.width, r->req.height, ctx->tune32.nt, ctx->tune32.th);
+        const PairPlan P = plan_pair(r->req.width, r->req.height, ctx->tune32);
         const size_t pb = ref_pyramid_bytes(P);
         if (pb) {
             RCG_HIP(hipMalloc(&r->d_ref, pb));

@@ -37,6 +37,31 @@ __constant__ double c_weights[108];
 template <typename T>
 struct Xyb;
 
+// The two quotients of the maps (core/src/ssimulacra2.rs:431-433):
+//   d  = 1 - num_m*num_s/denom_s            and   d1 = (1 + |b - mu2|) / (1 + |a - mu1|) - 1.
+// f64 evaluates them as the reference writes them. f32 uses the equal forms
+//   d  = (denom_s - num_m*num_s) / denom_s  and   d1 = (|b - mu2| - |a - mu1|) / (1 + |a - mu1|)
+// with a hardware reciprocal: the numerators are exactly 0 for identical images (so identity scores exactly 100),
+// and no 1 - ratio cancellation is left.
+template <typename T>
+struct Maps;
+template <>
+struct Maps<float> {
+    static __device__ __forceinline__ float ssim(float num_m, float num_s, float denom_s) {
+        return fmaf(-num_m, num_s, denom_s) * __builtin_amdgcn_rcpf(denom_s);
+    }
+    static __device__ __forceinline__ float edge(float da, float db) {
+        return (db - da) * __builtin_amdgcn_rcpf(1.0f + da);
+    }
+};
+template <>
+struct Maps<double> {
+    static __device__ __forceinline__ double ssim(double num_m, double num_s, double denom_s) {
+        return 1.0 - num_m * num_s / denom_s;
+    }
+    static __device__ __forceinline__ double edge(double da, double db) { return (1.0 + db) / (1.0 + da) - 1.0; }
+};
+
 // The f32 form works on XYB shifted by per-channel constants (X - 0.42, Y - 0.51, B - 0.55). Every map depends on
 // the values only through differences and variances (mu1 - mu2, a - mu1, blur(a^2) - mu1^2, blur(ab) - mu1*mu2),
 // which a constant shift leaves unchanged provided the samples outside the image hold -offset instead of 0; the
@@ -264,9 +289,9 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
                 const T num_m = T(1) - (m1 - m2) * (m1 - m2);
                 const T num_s = T(2) * (s12 - m1 * m2) + C2;
                 const T denom_s = (s11 - m1 * m1) + (s22 - m2 * m2) + C2;
-                T d = T(1) - num_m * num_s / denom_s;
+                T d = Maps<T>::ssim(num_m, num_s, denom_s);
                 d = d > T(0) ? d : T(0);
-                const T d1 = (T(1) + fabs(st.rb[SC][c] - m2)) / (T(1) + fabs(st.ra[SC][c] - m1)) - T(1);
+                const T d1 = Maps<T>::edge(fabs(st.ra[SC][c] - m1), fabs(st.rb[SC][c] - m2));
                 const T art = d1 > T(0) ? d1 : T(0);
                 const T det = -d1 > T(0) ? -d1 : T(0);
                 const T d2 = d * d, a2 = art * art, e2 = det * det;
@@ -285,7 +310,7 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
 
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ jobs,
-                                                    const uint32_t* __restrict__ tile_begins, int njobs, int TH,
+                                                    const uint32_t* __restrict__ tile_begins, int njobs,
                                                     Taps<T> taps, const float* __restrict__ lut_g) {
     constexpr int OW = NT - 12;
     constexpr int NWAVES = NT / 64;
@@ -308,6 +333,7 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     st.w = int(J.w);
     st.h = int(J.h);
     st.x0 = int(tx) * OW;
+    const int TH = int(J.th);
     st.y0 = int(ty) * TH;
     st.gx = st.x0 - 6 + tid;
     st.in_img = st.gx >= 0 && st.gx < st.w;
@@ -350,61 +376,68 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     }
 }
 
-template __global__ void k_ssim_scale<float, 256>(const SsimJob*, const uint32_t*, int, int, Taps<float>, const float*);
-template __global__ void k_ssim_scale<float, 128>(const SsimJob*, const uint32_t*, int, int, Taps<float>, const float*);
-template __global__ void k_ssim_scale<double, 128>(const SsimJob*, const uint32_t*, int, int, Taps<double>,
-                                                   const float*);
+template __global__ void k_ssim_scale<float, 256>(const SsimJob*, const uint32_t*, int, Taps<float>, const float*);
+template __global__ void k_ssim_scale<float, 128>(const SsimJob*, const uint32_t*, int, Taps<float>, const float*);
+template __global__ void k_ssim_scale<double, 128>(const SsimJob*, const uint32_t*, int, Taps<double>, const float*);
 
 // One workgroup (64 threads) per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean
 // and 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
 __global__ __launch_bounds__(64) void k_ssim_finalize(const FinalDesc* __restrict__ fj) {
     const FinalDesc& F = fj[blockIdx.x];
-    __shared__ double s_stats[6][18];
+    __shared__ double s_sum[6][18];
+    __shared__ double s_stat[6][18];
     const int lane = threadIdx.x;
-    for (uint32_t sc = 0; sc < F.nscales; sc++) {
-        double part[18];
-        for (int i = 0; i < 18; i++) part[i] = 0.0;
-        for (uint32_t t = lane; t < F.ntiles[sc]; t += 64) {
-            const double* p = F.partials[sc] + uint64_t(t) * 18;
-            for (int i = 0; i < 18; i++) part[i] += p[i];
+    const uint32_t ns = F.nscales;
+    // Lane l sums tiles l, l+64, ... of every scale (18 accumulators in registers, each tile's 18 sums read as
+    // one contiguous 144-byte record), then a fixed butterfly combines the lanes.
+    for (uint32_t sc = 0; sc < ns; sc++) {
+        const double* p = F.partials[sc];
+        const uint32_t nt = F.ntiles[sc];
+        double acc[18];
+#pragma unroll
+        for (int i = 0; i < 18; i++) acc[i] = 0.0;
+        for (uint32_t t = lane; t < nt; t += 64) {
+            const double2* rec = reinterpret_cast<const double2*>(p + uint64_t(t) * 18);
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const double2 v = rec[i];
+                acc[2 * i] += v.x;
+                acc[2 * i + 1] += v.y;
+            }
         }
+#pragma unroll
         for (int i = 0; i < 18; i++) {
-            double v = part[i];
+            double v = acc[i];
+#pragma unroll
             for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) s_stats[sc][i] = v;
+            if (lane == 0) s_sum[sc][i] = v;
         }
     }
     __syncthreads();
-    if (lane != 0) return;
-    double stats[6][3][6];
-    for (uint32_t sc = 0; sc < F.nscales; sc++) {
-        const double count = F.count[sc];
-        for (int c = 0; c < 3; c++)
-            for (int m = 0; m < 3; m++) {
-                stats[sc][c][2 * m] = s_stats[sc][c * 6 + 2 * m] / count;
-                stats[sc][c][2 * m + 1] = pow(s_stats[sc][c * 6 + 2 * m + 1] / count, 0.25);
-            }
+    // One lane per (scale, statistic): mean for even m, 4-norm for odd.
+    for (int k = lane; k < int(ns) * 18; k += 64) {
+        const int sc = k / 18, i = k % 18;
+        const double v = s_sum[sc][i] / F.count[sc];
+        s_stat[sc][i] = (i & 1) ? pow(v, 0.25) : v;
     }
+    __syncthreads();
+    if (lane != 0) return;
     double ssim = 0.0;
-    int i = 0;
+    int w = 0;
     for (int c = 0; c < 3; c++)
-        for (uint32_t sc = 0; sc < F.nscales; sc++)
+        for (uint32_t sc = 0; sc < ns; sc++)
             for (int n = 0; n < 2; n++) {
-                ssim += c_weights[i] * fabs(stats[sc][c][n]);
-                i++;
-                ssim += c_weights[i] * fabs(stats[sc][c][2 + n]);
-                i++;
-                ssim += c_weights[i] * fabs(stats[sc][c][4 + n]);
-                i++;
+                ssim += c_weights[w++] * fabs(s_stat[sc][c * 6 + n]);
+                ssim += c_weights[w++] * fabs(s_stat[sc][c * 6 + 2 + n]);
+                ssim += c_weights[w++] * fabs(s_stat[sc][c * 6 + 4 + n]);
             }
     ssim *= 0.9562382616834844;
     ssim = 2.326765642916932 * ssim - 0.020884521182843837 * ssim * ssim + 6.248496625763138e-5 * ssim * ssim * ssim;
     const double score = ssim > 0.0 ? 100.0 - 10.0 * pow(ssim, 0.6276336467831387) : 100.0;
     *F.score_out = score;
     if (F.stats_out)
-        for (uint32_t sc = 0; sc < F.nscales; sc++)
-            for (int c = 0; c < 3; c++)
-                for (int m = 0; m < 6; m++) F.stats_out[(sc * 3 + c) * 6 + m] = stats[sc][c][m];
+        for (uint32_t sc = 0; sc < ns; sc++)
+            for (int i = 0; i < 18; i++) F.stats_out[sc * 18 + i] = s_stat[sc][i];
 }
 
 // ----------------------------------------------------------------------------------------------------------------
@@ -419,23 +452,23 @@ int upload_weights() {
 }
 
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
-                      uint32_t total_tiles, int nt, int th) {
+                      uint32_t total_tiles, int nt) {
     if (total_tiles == 0) return RCG_OK;
     ProfScope ps(ctx, "k_ssim_scale");
     if (precision == RCG_PREC_FP64) {
         Taps<double> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps64[i];
         hipLaunchKernelGGL((k_ssim_scale<double, 128>), dim3(total_tiles), dim3(128), 0, ctx->stream, d_jobs,
-                           d_begins, njobs, th, t, ctx->d_lut);
+                           d_begins, njobs, t, ctx->d_lut);
     } else {
         Taps<float> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
         if (nt == 128)
             hipLaunchKernelGGL((k_ssim_scale<float, 128>), dim3(total_tiles), dim3(128), 0, ctx->stream, d_jobs,
-                               d_begins, njobs, th, t, ctx->d_lut);
+                               d_begins, njobs, t, ctx->d_lut);
         else
             hipLaunchKernelGGL((k_ssim_scale<float, 256>), dim3(total_tiles), dim3(256), 0, ctx->stream, d_jobs,
-                               d_begins, njobs, th, t, ctx->d_lut);
+                               d_begins, njobs, t, ctx->d_lut);
     }
     RCG_HIP(hipGetLastError());
     return RCG_OK;

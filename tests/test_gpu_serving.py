@@ -49,7 +49,7 @@ def test_results_alone_equal_results_in_the_mix(rc, ctx):
 
 
 def test_budget_admits_oversized_request_alone(rc):
-    srv = rc.Server(0, hbm_budget=1 << 20)  # 1 MiB: every request is oversized
+    srv = rc.Server(0, hbm_budget=4096)  # 4 KiB: every request is larger than the whole budget
     imgs = [rc.synth_rgb8(400 + i, 128, 96) for i in range(3)]
     for i, img in enumerate(imgs):
         srv.submit(i, img, 70.0)
