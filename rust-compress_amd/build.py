@@ -17,10 +17,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RCG_OFFLOAD_ARCH", "gfx950")
 
 
+FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
 def command(out=LIB):
-    return [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result",
-            "-o", out] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lpthread"]
+    """Single-invocation form of the build (documentation and debugging)."""
+    return [HIPCC] + FLAGS + ["-shared", "-o", out] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lpthread"]
 
 
 ROOT = os.path.dirname(HERE)
@@ -42,10 +46,16 @@ def _run(cmd, verbose):
 
 
 def build(verbose=False):
-    cmd = command()
+    """Compile every source to an object in parallel, then link librcgpu.so (and the C++ API test binary)."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    jobs = [[HIPCC] + FLAGS + ["-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        for f in [ex.submit(_run, j, verbose) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
-    cmd[cmd.index("-o") + 1] = tmp
-    _run(cmd, verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)
     # C++ tests of the host API (host/rc_gpu.hpp), linked against the library.
     _run(["g++", "-std=c++17", "-O2", "-Wall", API_TEST_SRC, "-o", API_TEST_BIN, "-L" + HERE, "-lrcgpu",

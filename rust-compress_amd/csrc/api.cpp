@@ -205,6 +205,7 @@ SsimBuffers take_ssim(Arena& A, const PairPlan& P) {
 // Desc::upload(). a0/b0 are the scale-0 sources. With a_cached, scales >= 1 of `a` read B.a_planes (built by an
 // earlier evaluation of the same reference) instead of writing them.
 struct StagedSsim {
+    uint32_t kind_a0, kind_b0;
     const SsimJob* jobs[6];
     const uint32_t* begins;
     const FinalDesc* fin;
@@ -217,6 +218,8 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
     StagedSsim st;
     st.P = P;
     st.precision = precision;
+    st.kind_a0 = a0.kind;
+    st.kind_b0 = b0.kind;
     const uint32_t zero = 0;
     st.begins = D.put(&zero, 1);
     for (uint32_t s = 0; s < P.nscales; s++) {
@@ -259,7 +262,8 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
 
 int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
     for (uint32_t s = 0; s < st.P.nscales; s++)
-        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], st.begins, 1, st.P.tiles[s], int(st.P.nt[s])));
+        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], st.begins, 1, st.P.tiles[s], int(st.P.nt[s]),
+                                  s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes), s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes)));
     return launch_finalize(ctx, st.fin, 1);
 }
 
@@ -353,7 +357,8 @@ int rcg::check_image(uint32_t w, uint32_t h, uint32_t channels) {
     return RCG_OK;
 }
 
-// Recon planes of the encode stage seen as reconstruct's sample planes (jpeg/src/decode.rs:1404-1415).
+// Recon planes of the encode stage seen as reconstruct's sample planes (jpeg/src/decode.rs:1404-1415). Luma is
+// full resolution and both chroma planes share one geometry, which the SSIMULACRA2 loader relies on.
 SsimSrc rcg::recon_src(const Layout& L, uint8_t* const rec[3]) {
     SsimSrc s;
     std::memset(&s, 0, sizeof(s));

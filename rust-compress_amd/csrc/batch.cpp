@@ -149,6 +149,7 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
         int n;
         uint32_t tiles;
         int nt;
+        uint32_t ka, kb;
     };
     std::vector<Staged> st;
     for (int s = 0; s < 6; s++) {
@@ -199,6 +200,8 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
                 g.n = int(sj.size());
                 g.tiles = total;
                 g.nt = nt;
+                g.ka = s == 0 ? uint32_t(kSrcBytes) : uint32_t(kSrcPlanes);
+                g.kb = s == 0 ? uint32_t(kSrcRecon) : uint32_t(kSrcPlanes);
                 st.push_back(g);
             }
         }
@@ -209,7 +212,8 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
     }
     const FinalDesc* dfin = D.put(fins.data(), fins.size());
     RCG_TRY(D.upload(ctx->stream));
-    for (const Staged& g : st) RCG_TRY(launch_ssim_scale(ctx, precision, g.jobs, g.begins, g.n, g.tiles, g.nt));
+    for (const Staged& g : st)
+        RCG_TRY(launch_ssim_scale(ctx, precision, g.jobs, g.begins, g.n, g.tiles, g.nt, g.ka, g.kb));
     return launch_finalize(ctx, dfin, int(pairs.size()));
 }
 

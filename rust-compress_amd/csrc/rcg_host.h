@@ -132,7 +132,7 @@ int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
                           uint8_t* out);
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
-                      uint32_t total_tiles, int nt);
+                      uint32_t total_tiles, int nt, uint32_t kind_a, uint32_t kind_b);
 struct FinalDesc {
     uint32_t nscales;
     uint32_t ntiles[6];

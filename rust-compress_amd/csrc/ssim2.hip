@@ -110,43 +110,134 @@ struct Xyb<double> {
     }
 };
 
-// Linear-light RGB of one image at (x, y), in range.
-__device__ __forceinline__ void load_linear(const SsimSrc& S, uint32_t w, uint32_t h, uint32_t x, uint32_t y,
-                                            const float* lut, float out[3]) {
-    if (S.kind == kSrcPlanes) {
-        const uint64_t i = uint64_t(y) * w + x, n = uint64_t(w) * h;
-        out[0] = S.planes[i];
-        out[1] = S.planes[n + i];
-        out[2] = S.planes[2 * n + i];
-    } else if (S.kind == kSrcBytes) {
-        if (S.channels == 1) {
-            const float l = lut[S.bytes[uint64_t(y) * w + x]];
+#define RCG_GLOBAL __attribute__((address_space(1)))
+
+// Loads and stores through the global address space (global_load/global_store rather than flat accesses: the
+// pointers live in job tables, so the compiler cannot infer their address space by itself).
+template <class V>
+__device__ __forceinline__ V gload(const V* p) {
+    return *(const RCG_GLOBAL V*)p;
+}
+template <class V>
+__device__ __forceinline__ void gstore(V* p, V v) {
+    *(RCG_GLOBAL V*)p = v;
+}
+
+// Row loaders of linear-light RGB. begin_row() takes the row the whole workgroup is on (uniform), so row pointers
+// are scalar; load() takes the thread's column. The source kind is a template parameter: each kernel instance
+// carries only the loader it uses.
+template <uint32_t K>
+struct Loader;
+
+// Gray8/RGB8 bytes through the sRGB table (core/src/ssimulacra2.rs:186-207).
+template <>
+struct Loader<kSrcBytes> {
+    const uint8_t* base;
+    const uint8_t* row;
+    uint32_t w, ch;
+    __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t) {
+        base = S.bytes;
+        ch = S.channels;
+        w = w_;
+    }
+    __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w * ch; }
+    __device__ __forceinline__ void load(uint32_t x, const float* lut, float out[3]) const {
+        if (ch == 1) {
+            const float l = lut[gload(row + x)];
             out[0] = out[1] = out[2] = l;
         } else {
-            const uint8_t* p = S.bytes + (uint64_t(y) * w + x) * 3;
-            out[0] = lut[p[0]];
-            out[1] = lut[p[1]];
-            out[2] = lut[p[2]];
-        }
-    } else {
-        if (S.ncomp == 1) {
-            const float l = lut[upsample_at(S.comp[0], x, y)];
-            out[0] = out[1] = out[2] = l;
-        } else {
-            const int32_t p0 = upsample_at(S.comp[0], x, y);
-            const int32_t p1 = upsample_at(S.comp[1], x, y);
-            const int32_t p2 = upsample_at(S.comp[2], x, y);
-            int32_t r, g, b;
-            if (S.rgb_direct) {
-                r = p0; g = p1; b = p2;
-            } else {
-                ycc_to_rgb(p0, p1, p2, r, g, b);
-            }
-            out[0] = lut[r];
-            out[1] = lut[g];
-            out[2] = lut[b];
+            const uint8_t* p = row + 3 * x;
+            out[0] = lut[gload(p)];
+            out[1] = lut[gload(p + 1)];
+            out[2] = lut[gload(p + 2)];
         }
     }
+};
+
+// The previous scale's planar f32 pyramid level.
+template <>
+struct Loader<kSrcPlanes> {
+    const float* base;
+    const float* row;
+    uint64_t n;
+    uint32_t w;
+    __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t h) {
+        base = S.planes;
+        w = w_;
+        n = uint64_t(w_) * h;
+    }
+    __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w; }
+    __device__ __forceinline__ void load(uint32_t x, const float*, float out[3]) const {
+        out[0] = gload(row + x);
+        out[1] = gload(row + n + x);
+        out[2] = gload(row + 2 * n + x);
+    }
+};
+
+// Reconstructed JPEG sample planes of the search path with reconstruct's upsampling (jpeg/src/decode.rs:1769-1829)
+// and fixed-point colour conversion (decode.rs:1434-1441) applied on the fly, then the sRGB table. These planes come
+// from build_coefs layouts: luma is full resolution and both chroma planes share one geometry (1x1, 2x1 or 2x2).
+template <>
+struct Loader<kSrcRecon> {
+    const uint8_t* py;
+    const uint8_t* pc[2];
+    uint32_t stride_y, stride_c, cw, ch, fx, fy, ncomp;
+    const uint8_t* ry;
+    uint32_t c0, c1, yodd;  // chroma row offsets (near, far) of the current row
+    __device__ __forceinline__ void init(const SsimSrc& S, uint32_t, uint32_t) {
+        ncomp = S.ncomp;
+        py = S.comp[0].p;
+        stride_y = S.comp[0].stride;
+        pc[0] = S.comp[1].p;
+        pc[1] = S.comp[2].p;
+        stride_c = S.comp[1].stride;
+        cw = S.comp[1].cw;
+        ch = S.comp[1].ch;
+        fx = S.comp[1].fx;
+        fy = S.comp[1].fy;
+    }
+    __device__ __forceinline__ void begin_row(uint32_t r) {
+        yodd = r & 1;
+        ry = py + uint64_t(r) * stride_y;
+        if (fy == 2) {
+            const uint32_t jj = r >> 1;
+            const uint32_t far = yodd == 0 ? (jj > 0 ? jj - 1 : 0) : min(jj + 1, ch - 1);
+            c0 = jj * stride_c;
+            c1 = far * stride_c;
+        } else {
+            c0 = c1 = r * stride_c;
+        }
+    }
+    __device__ __forceinline__ int32_t chroma(int k, uint32_t x) const {
+        const uint8_t* a = pc[k] + c0;
+        const uint8_t* b = pc[k] + c1;
+        if (fx == 1) return gload(a + x);
+        const uint32_t i = x >> 1;
+        const uint32_t fi = (x & 1) == 0 ? (i > 0 ? i - 1 : 0) : min(i + 1, cw - 1);
+        if (fy == 2) {
+            const int32_t nearv = 3 * int32_t(gload(a + i)) + int32_t(gload(b + i));
+            const int32_t far = 3 * int32_t(gload(a + fi)) + int32_t(gload(b + fi));
+            return (3 * nearv + far + ((x & 1) ? 7 : 8)) >> 4;
+        }
+        return (3 * int32_t(gload(a + i)) + int32_t(gload(a + fi)) + ((x & 1) ? 2 : 1)) >> 2;
+    }
+    __device__ __forceinline__ void load(uint32_t x, const float* lut, float out[3]) const {
+        const int32_t y = gload(ry + x);
+        if (ncomp == 1) {
+            out[0] = out[1] = out[2] = lut[y];
+            return;
+        }
+        int32_t r, g, b;
+        ycc_to_rgb(y, chroma(0, x), chroma(1, x), r, g, b);
+        out[0] = lut[r];
+        out[1] = lut[g];
+        out[2] = lut[b];
+    }
+};
+
+// Exchange with the neighbouring lane of the pair (x ^ 1) through DPP (quad_perm [1,0,3,2]).
+__device__ __forceinline__ float swap_pair(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 
 __device__ __forceinline__ int find_job(const uint32_t* begins, int njobs, uint32_t bid) {
@@ -160,9 +251,12 @@ __device__ __forceinline__ int find_job(const uint32_t* begins, int njobs, uint3
 }
 
 // Per-workgroup constants and per-thread state of the scale kernel's row walk.
-template <typename T, int NT>
+template <typename T, int NT, uint32_t KA, uint32_t KB>
 struct Walk {
-    const SsimJob* J;
+    Loader<KA> la;
+    Loader<KB> lb;
+    float* a_next;
+    float* b_next;
     const float* lut;
     const T* taps;
     T (*s_q)[15][NT];
@@ -176,17 +270,19 @@ struct Walk {
 
 // One input row of the walk, in ring slot S (compile-time, so the ring stays in registers). Returns false once the
 // walk is past its last row.
-template <int S, typename T, int NT>
-__device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
+template <int S, typename T, int NT, uint32_t KA, uint32_t KB>
+__device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
 #pragma clang fp contract(fast)
     const int yy = st.yy;
     if (yy > st.ylast) return false;
     const int h = st.h, w = st.w;
     const int r = yy < 0 ? -yy - 1 : (yy >= h ? 2 * h - 1 - yy : yy);
     float la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
+    st.la.begin_row(uint32_t(r));
+    st.lb.begin_row(uint32_t(r));
     if (st.in_img) {
-        load_linear(st.J->a, st.J->w, st.J->h, st.gx, r, st.lut, la);
-        load_linear(st.J->b, st.J->w, st.J->h, st.gx, r, st.lut, lb);
+        st.la.load(uint32_t(st.gx), st.lut, la);
+        st.lb.load(uint32_t(st.gx), st.lut, lb);
         Xyb<T>::eval(T(la[0]), T(la[1]), T(la[2]), st.ra[S][0], st.ra[S][1], st.ra[S][2]);
         Xyb<T>::eval(T(lb[0]), T(lb[1]), T(lb[2]), st.rb[S][0], st.rb[S][1], st.rb[S][2]);
     } else {
@@ -202,8 +298,8 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
         float qa[3], qb[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            qa[c] = __shfl_xor(la[c], 1);
-            qb[c] = __shfl_xor(lb[c], 1);
+            qa[c] = swap_pair(la[c]);
+            qb[c] = swap_pair(lb[c]);
         }
         const bool first_row = ((yy - st.y0) & 1) == 0;
         if (first_row) {
@@ -237,8 +333,8 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
                         sb += double(qb[c]);
                     }
                 }
-                if (st.want_a) st.J->a_next[c * n2 + o] = float(sa * inv);
-                if (st.want_b) st.J->b_next[c * n2 + o] = float(sb * inv);
+                if (st.want_a) gstore(st.a_next + c * n2 + o, float(sa * inv));
+                if (st.want_b) gstore(st.b_next + c * n2 + o, float(sb * inv));
             }
         }
     }
@@ -308,7 +404,7 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT>& st) {
     return true;
 }
 
-template <typename T, int NT>
+template <typename T, int NT, uint32_t KA, uint32_t KB>
 __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ tile_begins, int njobs,
                                                     Taps<T> taps, const float* __restrict__ lut_g) {
@@ -321,11 +417,14 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += NT) s_lut[i] = lut_g[i];
 
-    Walk<T, NT> st;
-    st.J = &jobs[find_job(tile_begins, njobs, blockIdx.x)];
-    const SsimJob& J = *st.J;
+    const SsimJob& J = jobs[find_job(tile_begins, njobs, blockIdx.x)];
     const uint32_t tile = blockIdx.x - J.tile_begin;
     const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+    Walk<T, NT, KA, KB> st;
+    st.la.init(J.a, J.w, J.h);
+    st.lb.init(J.b, J.w, J.h);
+    st.a_next = J.a_next;
+    st.b_next = J.b_next;
     st.lut = s_lut;
     st.taps = taps.k;
     st.s_q = s_q;
@@ -340,8 +439,8 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     st.is_out = tid >= 6 && tid < 6 + OW && st.gx < st.w;
     st.y_end = min(st.y0 + TH, st.h);
     st.ylast = st.y_end + 4;
-    st.want_a = J.a_next != nullptr;
-    st.want_b = J.b_next != nullptr;
+    st.want_a = st.a_next != nullptr;
+    st.want_b = st.b_next != nullptr;
     st.want_next = st.want_a || st.want_b;
     st.nw = (st.w + 1) >> 1;
     st.nh = (st.h + 1) >> 1;
@@ -372,13 +471,11 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
         double v = 0.0;
 #pragma unroll
         for (int wv = 0; wv < NWAVES; wv++) v += s_red[wv][tid];
-        J.partials[uint64_t(tile) * 18 + tid] = v;
+        gstore(J.partials + uint64_t(tile) * 18 + tid, v);
     }
 }
 
-template __global__ void k_ssim_scale<float, 256>(const SsimJob*, const uint32_t*, int, Taps<float>, const float*);
-template __global__ void k_ssim_scale<float, 128>(const SsimJob*, const uint32_t*, int, Taps<float>, const float*);
-template __global__ void k_ssim_scale<double, 128>(const SsimJob*, const uint32_t*, int, Taps<double>, const float*);
+
 
 // One workgroup (64 threads) per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean
 // and 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
@@ -451,25 +548,45 @@ int upload_weights() {
     return RCG_OK;
 }
 
+template <typename T, int NT, uint32_t KA, uint32_t KB>
+static void launch_one(rcg_ctx* ctx, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t tiles,
+                       const Taps<T>& t) {
+    hipLaunchKernelGGL((k_ssim_scale<T, NT, KA, KB>), dim3(tiles), dim3(NT), 0, ctx->stream, d_jobs, d_begins, njobs, t,
+                       ctx->d_lut);
+}
+
+template <typename T, int NT>
+static int launch_kinds(rcg_ctx* ctx, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t tiles,
+                        const Taps<T>& t, uint32_t ka, uint32_t kb) {
+    if (ka == kSrcBytes && kb == kSrcRecon)
+        launch_one<T, NT, kSrcBytes, kSrcRecon>(ctx, d_jobs, d_begins, njobs, tiles, t);
+    else if (ka == kSrcBytes && kb == kSrcBytes)
+        launch_one<T, NT, kSrcBytes, kSrcBytes>(ctx, d_jobs, d_begins, njobs, tiles, t);
+    else if (ka == kSrcPlanes && kb == kSrcPlanes)
+        launch_one<T, NT, kSrcPlanes, kSrcPlanes>(ctx, d_jobs, d_begins, njobs, tiles, t);
+    else {
+        set_error("internal: unsupported SSIMULACRA2 source combination");
+        return RCG_DEVICE;
+    }
+    return RCG_OK;
+}
+
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
-                      uint32_t total_tiles, int nt) {
+                      uint32_t total_tiles, int nt, uint32_t kind_a, uint32_t kind_b) {
     if (total_tiles == 0) return RCG_OK;
     ProfScope ps(ctx, "k_ssim_scale");
+    int rc;
     if (precision == RCG_PREC_FP64) {
         Taps<double> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps64[i];
-        hipLaunchKernelGGL((k_ssim_scale<double, 128>), dim3(total_tiles), dim3(128), 0, ctx->stream, d_jobs,
-                           d_begins, njobs, t, ctx->d_lut);
+        rc = launch_kinds<double, 128>(ctx, d_jobs, d_begins, njobs, total_tiles, t, kind_a, kind_b);
     } else {
         Taps<float> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
-        if (nt == 128)
-            hipLaunchKernelGGL((k_ssim_scale<float, 128>), dim3(total_tiles), dim3(128), 0, ctx->stream, d_jobs,
-                               d_begins, njobs, t, ctx->d_lut);
-        else
-            hipLaunchKernelGGL((k_ssim_scale<float, 256>), dim3(total_tiles), dim3(256), 0, ctx->stream, d_jobs,
-                               d_begins, njobs, t, ctx->d_lut);
+        rc = nt == 128 ? launch_kinds<float, 128>(ctx, d_jobs, d_begins, njobs, total_tiles, t, kind_a, kind_b)
+                       : launch_kinds<float, 256>(ctx, d_jobs, d_begins, njobs, total_tiles, t, kind_a, kind_b);
     }
+    if (rc != RCG_OK) return rc;
     RCG_HIP(hipGetLastError());
     return RCG_OK;
 }
