@@ -66,6 +66,8 @@ def main():
     import torch
     rc = importlib.import_module("rust-compress_amd")
     ctx = rc.Context(0)
+    tune = ctx.autotune(8)  # tile heights per scale-height class, measured on this GPU
+    progress(f"autotuned tile policy: {tune.as_dict()}")
     W = H = args.size
     img = rc.synth_rgb8(2, W, H)
     d_img = torch.from_numpy(img).to("cuda")
@@ -123,6 +125,7 @@ def main():
         "kernels_ms_per_step": {"k_ssim_scale": round(ssim_ms / args.steps, 3),
                                 "k_encode": round(enc_ms / args.steps, 3),
                                 "k_ssim_finalize": round(fin_ms / args.steps, 3)},
+        "tile_policy": tune.as_dict(),
         "scores": [round(x, 4) for x in list(r0.scores)[:evals]],
         "settled_fp64": int(sum(r0.settled[:evals])),
     }
@@ -157,7 +160,7 @@ def main():
         out["batched"] = bench_batch(rc, ctx, torch, args)
     if not args.no_serve:
         progress("serving configuration")
-        out["serving"] = bench_serve(rc, args)
+        out["serving"] = bench_serve(rc, args, tune)
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -218,7 +221,7 @@ def bench_batch(rc, ctx, torch, args):
     return out
 
 
-def bench_serve(rc, args):
+def bench_serve(rc, args, tune=None):
     """Poisson arrivals at 200 req/s; megapixels log-uniform in [0.1, 24], aspect uniform in [0.5, 2]; targets
     uniform over {60, 70, 80}. Images come from a pool of 32 synthetic images drawn from that distribution (the
     host cannot hold one image per request); every request is an independent search. Latency = completion -
@@ -237,7 +240,7 @@ def bench_serve(rc, args):
     picks = rng.integers(0, len(pool), n)
     targets = rng.choice([60.0, 70.0, 80.0], n)
     gaps = rng.exponential(1.0 / 200.0, n)
-    srv = rc.Server(0, speculate=3)
+    srv = rc.Server(0, speculate=3, tune=tune)
     # Warm-up: one request of each pool image size class is not needed; run one request to initialise.
     srv.submit(10 ** 9, pool[0], 70.0)
     while not srv.poll(8, 500):

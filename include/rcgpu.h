@@ -169,6 +169,25 @@ int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_
 int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
 
 /*
+ * Tile policy of the SSIMULACRA2 scale kernel: a scale of height h uses th[i] output rows per tile for the first
+ * class i with h >= min_h[i] (classes in descending min_h, the last with min_h = 0), and 128-thread workgroups when
+ * its width is at most narrow_w (else wide_nt = 256). The policy depends only on an image's own size, so a score
+ * never depends on what else shares a launch; different policies change scores only in the last bits.
+ * rcg_autotune times the scale kernel for th in {8,16,32,64,128} on `candidates` synthetic pairs at each size
+ * class (4096 down to 64 rows) and installs the fastest per class; it returns the installed policy.
+ */
+typedef struct {
+    int wide_nt;
+    uint32_t narrow_w;
+    int nclass;
+    uint32_t min_h[8];
+    int th[8];
+} rcg_tune;
+int rcg_get_tune(rcg_ctx* ctx, rcg_tune* out);
+int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune);
+int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out);
+
+/*
  * Serving loop with continuous batching. A worker thread owns one context. Requests (an image and a
  * --target-ssimulacra2 goal) are admitted in arrival order while their device footprint fits the HBM budget, the
  * GPU analogue of cli/src/budget.rs:11-50: a request that alone exceeds the budget runs, but by itself. Every tick
@@ -185,6 +204,7 @@ typedef struct {
     size_t hbm_budget;              /* bytes for admitted requests; 0 = 16 GiB */
     int max_active;                 /* requests in flight at once; 0 = no limit */
     int speculate;                  /* 1 or 3 */
+    const rcg_tune* tune;           /* tile policy of the worker's context, or NULL for the default one */
 } rcg_server_opts;
 typedef struct {
     uint64_t id;                    /* caller's tag, returned in the completion */

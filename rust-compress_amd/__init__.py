@@ -70,8 +70,19 @@ class BatchResult(ctypes.Structure):
                 ("sweep_quality", ctypes.c_int), ("status", ctypes.c_int)]
 
 
+class Tune(ctypes.Structure):
+    """rcg_tune: tile policy of the SSIMULACRA2 scale kernel (rows per tile by scale height)."""
+    _fields_ = [("wide_nt", ctypes.c_int), ("narrow_w", ctypes.c_uint32), ("nclass", ctypes.c_int),
+                ("min_h", ctypes.c_uint32 * 8), ("th", ctypes.c_int * 8)]
+
+    def as_dict(self):
+        return {"wide_nt": self.wide_nt, "narrow_w": self.narrow_w,
+                "classes": [[int(self.min_h[i]), int(self.th[i])] for i in range(self.nclass)]}
+
+
 class ServerOpts(ctypes.Structure):
-    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("speculate", ctypes.c_int)]
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("speculate", ctypes.c_int),
+                ("tune", ctypes.c_void_p)]
 
 
 class Request(ctypes.Structure):
@@ -126,6 +137,9 @@ def lib():
         L.rcg_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
         L.rcg_server_destroy.argtypes = [P]
         L.rcg_server_destroy.restype = None
+        L.rcg_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
+        L.rcg_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
+        L.rcg_autotune.argtypes = [P, I, ctypes.POINTER(Tune)]
         L.rcg_profile.argtypes = [P, I]
         L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
         _LIB = L
@@ -319,6 +333,21 @@ class Context:
         _check(fn(self._h, jobs, n, res))
         return list(res)
 
+    # ------------------------------------------------------------------ tile policy
+
+    def get_tune(self):
+        t = Tune()
+        _check(lib().rcg_get_tune(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tune(self, tune):
+        _check(lib().rcg_set_tune(self._h, ctypes.byref(tune)))
+
+    def autotune(self, candidates=8):
+        t = Tune()
+        _check(lib().rcg_autotune(self._h, candidates, ctypes.byref(t)))
+        return t
+
     # ------------------------------------------------------------------ profiling
 
     def profile(self, enable):
@@ -382,9 +411,9 @@ def _coef_image_struct(ci, keep, device):
 class Server:
     """Continuous-batching target-quality service on one GPU (rcg_server_*)."""
 
-    def __init__(self, device=0, hbm_budget=0, max_active=0, speculate=1):
+    def __init__(self, device=0, hbm_budget=0, max_active=0, speculate=1, tune=None):
         self._h = ctypes.c_void_p()
-        o = ServerOpts(hbm_budget, max_active, speculate)
+        o = ServerOpts(hbm_budget, max_active, speculate, ctypes.addressof(tune) if tune is not None else None)
         _check(lib().rcg_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
 
     def submit(self, req_id, img, target, max_quality=100, subsampling=S420):

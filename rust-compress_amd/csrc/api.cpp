@@ -121,8 +121,8 @@ PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
         P.sw[s] = sw;
         P.sh[s] = sh;
         P.nt[s] = T.fixed_nt ? uint32_t(T.fixed_nt) : (sw <= T.narrow_w ? 128u : uint32_t(T.wide_nt));
-        P.th[s] = uint32_t(T.th[3]);
-        for (int i = 0; i < 4; i++)
+        P.th[s] = uint32_t(T.th[T.nclass - 1]);
+        for (int i = 0; i < T.nclass; i++)
             if (sh >= T.th_min_h[i]) {
                 P.th[s] = uint32_t(T.th[i]);
                 break;
@@ -843,3 +843,166 @@ int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, 
 }
 
 }  // extern "C"
+
+// ================================================================================================================
+// Tile policy and autotuning
+
+namespace {
+
+void tune_to_c(const SsimTune& T, rcg_tune* out) {
+    out->wide_nt = T.wide_nt;
+    out->narrow_w = T.narrow_w;
+    out->nclass = T.nclass;
+    for (int i = 0; i < 8; i++) {
+        out->min_h[i] = T.th_min_h[i];
+        out->th[i] = T.th[i];
+    }
+}
+
+int tune_from_c(const rcg_tune* in, SsimTune& T) {
+    if (!in || in->nclass < 1 || in->nclass > 8 || (in->wide_nt != 128 && in->wide_nt != 256)) return RCG_INVALID;
+    for (int i = 0; i < in->nclass; i++) {
+        if (in->th[i] < 2 || in->th[i] > 512 || (in->th[i] & 1)) return RCG_INVALID;
+        if (i > 0 && in->min_h[i] > in->min_h[i - 1]) return RCG_INVALID;
+    }
+    if (in->min_h[in->nclass - 1] != 0) return RCG_INVALID;
+    T.wide_nt = in->wide_nt;
+    T.narrow_w = in->narrow_w;
+    T.nclass = in->nclass;
+    for (int i = 0; i < 8; i++) {
+        T.th[i] = i < in->nclass ? in->th[i] : in->th[in->nclass - 1];
+        T.th_min_h[i] = i < in->nclass ? in->min_h[i] : 0;
+    }
+    return RCG_OK;
+}
+
+}  // namespace
+
+extern "C" int rcg_get_tune(rcg_ctx* ctx, rcg_tune* out) {
+    if (!ctx || !out) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    tune_to_c(ctx->tune32, out);
+    return RCG_OK;
+}
+
+extern "C" int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune) {
+    if (!ctx) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    SsimTune T = ctx->tune32;
+    const int rc = tune_from_c(tune, T);
+    if (rc != RCG_OK) {
+        set_error("invalid tile policy");
+        return rc;
+    }
+    ctx->tune32 = T;
+    return RCG_OK;
+}
+
+// Times k_ssim_scale (f32, f32 planes in, no pyramid out) on `candidates` pairs of S x S synthetic planes for every
+// tile height, for S = 4096 ... 64, and installs the fastest tile height of each size class.
+extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
+    if (!ctx || candidates < 1 || candidates > 64) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
+    const int ths[] = {8, 16, 32, 64, 128};
+    const int nsz = 7;
+    const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
+    ctx->arena.reset();
+    RCG_TRY(ctx->arena.ensure(2 * Arena::align(plane_bytes) + (size_t(4) << 20) + (size_t(1) << 20)));
+    RCG_TRY(ctx->pinned.ensure(size_t(1) << 20));
+    float* a = static_cast<float*>(ctx->arena.take(plane_bytes));
+    float* b = static_cast<float*>(ctx->arena.take(plane_bytes));
+    double* partials = static_cast<double*>(ctx->arena.take(size_t(4) << 20));
+    char* ddesc = static_cast<char*>(ctx->arena.take(size_t(1) << 20));
+    // Synthetic linear-light planes: a smooth pattern and a perturbed copy (the timing does not depend on values).
+    {
+        std::vector<float> h(size_t(4096) * 4096 * 3);
+        uint64_t z = 0x9E3779B97F4A7C15ull;
+        for (size_t i = 0; i < h.size(); i++) {
+            z = z * 6364136223846793005ull + 1442695040888963407ull;
+            h[i] = float((z >> 40) & 0xFFFF) / 65535.0f;
+        }
+        RCG_HIP(hipMemcpy(a, h.data(), plane_bytes, hipMemcpyHostToDevice));
+        for (size_t i = 0; i < h.size(); i++) h[i] = std::min(1.0f, h[i] * 0.97f + 0.01f);
+        RCG_HIP(hipMemcpy(b, h.data(), plane_bytes, hipMemcpyHostToDevice));
+    }
+    hipEvent_t e0, e1;
+    RCG_HIP(hipEventCreate(&e0));
+    RCG_HIP(hipEventCreate(&e1));
+    SsimTune T = ctx->tune32;
+    int best_th[nsz];
+    const bool prof = ctx->prof.on;
+    ctx->prof.on = false;
+    for (int si = 0; si < nsz; si++) {
+        const uint32_t S = sizes[si];
+        double best = 1e30;
+        best_th[si] = 8;
+        for (int th : ths) {
+            if (uint32_t(th) > S) continue;
+            const int nt = S <= T.narrow_w ? 128 : T.wide_nt;
+            const uint32_t tx = ssim_tiles_x(S, nt), ty = ssim_tiles_y(S, th);
+            if (size_t(tx) * ty * 18 * 8 * candidates > (size_t(4) << 20)) continue;
+            std::vector<SsimJob> jobs(candidates);
+            std::vector<uint32_t> begins(candidates);
+            for (int c = 0; c < candidates; c++) {
+                SsimJob& j = jobs[c];
+                std::memset(&j, 0, sizeof(j));
+                j.w = S;
+                j.h = S;
+                j.th = uint32_t(th);
+                j.tiles_x = tx;
+                j.tiles_y = ty;
+                j.tile_begin = uint32_t(c) * tx * ty;
+                j.a = planes_src(a);
+                j.b = planes_src(b);
+                j.partials = partials + size_t(c) * tx * ty * 18;
+                begins[c] = j.tile_begin;
+            }
+            Desc D;
+            D.host = ctx->pinned.base;
+            D.dev = ddesc;
+            D.cap = size_t(1) << 20;
+            const SsimJob* dj = D.put(jobs.data(), jobs.size());
+            const uint32_t* db = D.put(begins.data(), begins.size());
+            RCG_TRY(D.upload(ctx->stream));
+            const uint32_t tiles = tx * ty * uint32_t(candidates);
+            RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, db, candidates, tiles, nt, kSrcPlanes, kSrcPlanes));
+            const int reps = S >= 2048 ? 3 : 10;
+            RCG_HIP(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < reps; r++)
+                RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, db, candidates, tiles, nt, kSrcPlanes, kSrcPlanes));
+            RCG_HIP(hipEventRecord(e1, ctx->stream));
+            RCG_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            RCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / reps < best) {
+                best = ms / reps;
+                best_th[si] = th;
+            }
+        }
+    }
+    ctx->prof.on = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // Size classes: class i covers heights in [sizes[i], sizes[i-1]); adjacent classes with the same choice merge.
+    SsimTune N = T;
+    N.nclass = 0;
+    for (int si = 0; si < nsz; si++) {
+        const uint32_t lo = si == nsz - 1 ? 0 : sizes[si];
+        if (N.nclass > 0 && N.th[N.nclass - 1] == best_th[si]) {
+            N.th_min_h[N.nclass - 1] = lo;
+            continue;
+        }
+        N.th[N.nclass] = best_th[si];
+        N.th_min_h[N.nclass] = lo;
+        N.nclass++;
+    }
+    for (int i = N.nclass; i < 8; i++) {
+        N.th[i] = N.th[N.nclass - 1];
+        N.th_min_h[i] = 0;
+    }
+    ctx->tune32 = N;
+    if (out) tune_to_c(N, out);
+    return RCG_OK;
+}

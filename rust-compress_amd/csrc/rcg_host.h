@@ -76,8 +76,9 @@ struct Prof {
 struct SsimTune {
     int wide_nt = 256;
     uint32_t narrow_w = 160;
-    int th[4] = {64, 32, 16, 8};
-    uint32_t th_min_h[4] = {2048, 512, 128, 0};
+    int nclass = 4;
+    int th[8] = {64, 32, 16, 8, 8, 8, 8, 8};
+    uint32_t th_min_h[8] = {2048, 512, 128, 0, 0, 0, 0, 0};  // descending; the last class must have 0
     int fixed_nt = 0;  // FP64 kernel: one thread count for every scale
 };
 

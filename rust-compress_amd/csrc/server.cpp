@@ -289,10 +289,19 @@ int rcg_server_create(int device, const rcg_server_opts* opts, rcg_server** out)
     if (opts) s->opts = *opts;
     if (s->opts.hbm_budget == 0) s->opts.hbm_budget = size_t(16) << 30;
     if (s->opts.speculate != 3) s->opts.speculate = 1;
-    const int rc = rcg_create(device, &s->ctx);
+    int rc = rcg_create(device, &s->ctx);
     if (rc != RCG_OK) {
         delete s;
         return rc;
+    }
+    if (s->opts.tune) {
+        rc = rcg_set_tune(s->ctx, s->opts.tune);
+        if (rc != RCG_OK) {
+            rcg_destroy(s->ctx);
+            delete s;
+            return rc;
+        }
+        s->opts.tune = nullptr;
     }
     s->worker = std::thread([s] { s->run(); });
     *out = s;

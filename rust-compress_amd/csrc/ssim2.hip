@@ -37,28 +37,88 @@ __constant__ double c_weights[108];
 template <typename T>
 struct Xyb;
 
-// The two quotients of the maps (core/src/ssimulacra2.rs:431-433):
-//   d  = 1 - num_m*num_s/denom_s            and   d1 = (1 + |b - mu2|) / (1 + |a - mu1|) - 1.
-// f64 evaluates them as the reference writes them. f32 uses the equal forms
-//   d  = (denom_s - num_m*num_s) / denom_s  and   d1 = (|b - mu2| - |a - mu1|) / (1 + |a - mu1|)
-// with a hardware reciprocal: the numerators are exactly 0 for identical images (so identity scores exactly 100),
-// and no 1 - ratio cancellation is left.
+// Quantities blurred per channel and the SSIM map computed from them (core/src/ssimulacra2.rs:364-382, 424-433).
+//
+// f64 follows the reference: mu_a, mu_b, blur(a^2), blur(b^2), blur(ab) and
+//   d = 1 - (1 - (mu_a - mu_b)^2) * (2*sigma_ab + C2) / (sigma_aa + sigma_bb + C2).
+// f32 blurs four quantities, mu_a, mu_b, Q = blur(a^2 + b^2) and R = blur((a - b)^2). With dmu = mu_a - mu_b,
+// V = R - dmu^2 (= sigma_aa + sigma_bb - 2*sigma_ab, the local variance of a - b) and
+// D = Q - mu_a^2 - mu_b^2 + C2 (= sigma_aa + sigma_bb + C2), the same map is
+//   d = (V + dmu^2 * (D - V)) / D,
+// since 2*sigma_ab + C2 = D - V. V is formed from differences of the two images, not by cancelling large blurred
+// products, so the f32 form keeps its precision where the images are close, and V = 0 exactly for identical ones.
+template <typename T>
+struct Form;
+template <>
+struct Form<double> {
+    static constexpr int NQ = 5;
+    static __device__ __forceinline__ void centre(double k0, double a0, double b0, double q[5]) {
+        q[0] = k0 * a0;
+        q[1] = k0 * b0;
+        q[2] = k0 * (a0 * a0);
+        q[3] = k0 * (b0 * b0);
+        q[4] = k0 * (a0 * b0);
+    }
+    static __device__ __forceinline__ void pair(double kt, double au, double ad, double bu, double bd, double q[5]) {
+        q[0] += kt * (au + ad);
+        q[1] += kt * (bu + bd);
+        q[2] += kt * (au * au + ad * ad);
+        q[3] += kt * (bu * bu + bd * bd);
+        q[4] += kt * (au * bu + ad * bd);
+    }
+    static __device__ __forceinline__ double ssim(const double v[5]) {
+        const double m1 = v[0], m2 = v[1], s11 = v[2], s22 = v[3], s12 = v[4];
+        const double C2 = 0.0009;
+        const double num_m = 1.0 - (m1 - m2) * (m1 - m2);
+        const double num_s = 2.0 * (s12 - m1 * m2) + C2;
+        const double denom_s = (s11 - m1 * m1) + (s22 - m2 * m2) + C2;
+        return 1.0 - num_m * num_s / denom_s;
+    }
+};
+template <>
+struct Form<float> {
+    static constexpr int NQ = 4;
+    static __device__ __forceinline__ void centre(float k0, float a0, float b0, float q[4]) {
+#pragma clang fp contract(fast)
+        const float d0 = a0 - b0;
+        q[0] = k0 * a0;
+        q[1] = k0 * b0;
+        q[2] = k0 * (a0 * a0 + b0 * b0);
+        q[3] = k0 * (d0 * d0);
+    }
+    static __device__ __forceinline__ void pair(float kt, float au, float ad, float bu, float bd, float q[4]) {
+#pragma clang fp contract(fast)
+        const float du = au - bu, dd = ad - bd;
+        q[0] += kt * (au + ad);
+        q[1] += kt * (bu + bd);
+        q[2] += kt * ((au * au + bu * bu) + (ad * ad + bd * bd));
+        q[3] += kt * (du * du + dd * dd);
+    }
+    static __device__ __forceinline__ float ssim(const float v[4]) {
+#pragma clang fp contract(fast)
+        const float m1 = v[0], m2 = v[1], Q = v[2], R = v[3];
+        const float dmu = m1 - m2, dmu2 = dmu * dmu;
+        const float V = R - dmu2;
+        const float D = (Q - (m1 * m1 + m2 * m2)) + 0.0009f;
+        return fmaf(dmu2, D - V, V) * __builtin_amdgcn_rcpf(D);
+    }
+};
+
+// The edge quotient of the maps (core/src/ssimulacra2.rs:433):
+//   d1 = (1 + |b - mu2|) / (1 + |a - mu1|) - 1.
+// f64 evaluates it as the reference writes it. f32 uses the equal form (|b - mu2| - |a - mu1|) / (1 + |a - mu1|)
+// with a hardware reciprocal: exactly 0 for identical images, and no 1 - ratio cancellation.
 template <typename T>
 struct Maps;
 template <>
 struct Maps<float> {
-    static __device__ __forceinline__ float ssim(float num_m, float num_s, float denom_s) {
-        return fmaf(-num_m, num_s, denom_s) * __builtin_amdgcn_rcpf(denom_s);
-    }
     static __device__ __forceinline__ float edge(float da, float db) {
+#pragma clang fp contract(fast)
         return (db - da) * __builtin_amdgcn_rcpf(1.0f + da);
     }
 };
 template <>
 struct Maps<double> {
-    static __device__ __forceinline__ double ssim(double num_m, double num_s, double denom_s) {
-        return 1.0 - num_m * num_s / denom_s;
-    }
     static __device__ __forceinline__ double edge(double da, double db) { return (1.0 + db) / (1.0 + da) - 1.0; }
 };
 
@@ -259,13 +319,13 @@ struct Walk {
     float* b_next;
     const float* lut;
     const T* taps;
-    T (*s_q)[15][NT];
+    T (*s_q)[3 * Form<T>::NQ][NT];
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
     bool in_img, is_out, want_a, want_b, want_next, ds_writer;
     int yy;
     T ra[11][3], rb[11][3];
     T acc[18];
-    float pa_own[3], pa_par[3], pb_own[3], pb_par[3];
+    float v0[3], v1[3];  // first row of the current 2x2 blocks: left and right column, this lane's image
 };
 
 // One input row of the walk, in ring slot S (compile-time, so the ring stays in registers). Returns false once the
@@ -293,99 +353,84 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
             st.rb[S][c] = -Xyb<T>::off(c);
         }
     }
-    // Next-scale 2x2 mean (uniform condition: the row is one of this tile's own rows).
+    // Next-scale 2x2 mean (uniform condition: the row is one of this tile's own rows). The two lanes of a column
+    // pair split the work: the even lane sums the reference's block, the odd lane the distorted one's, each after
+    // one DPP exchange per channel of the value its partner needs.
     if (st.want_next && yy >= st.y0 && yy < st.y_end) {
-        float qa[3], qb[3];
+        const bool odd = (st.gx & 1) != 0;
+        float x0v[3], x1v[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            qa[c] = swap_pair(la[c]);
-            qb[c] = swap_pair(lb[c]);
+            const float recv = swap_pair(odd ? la[c] : lb[c]);
+            const float own = odd ? lb[c] : la[c];
+            x0v[c] = odd ? recv : own;
+            x1v[c] = odd ? own : recv;
         }
         const bool first_row = ((yy - st.y0) & 1) == 0;
         if (first_row) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                st.pa_own[c] = la[c];
-                st.pa_par[c] = qa[c];
-                st.pb_own[c] = lb[c];
-                st.pb_par[c] = qb[c];
+                st.v0[c] = x0v[c];
+                st.v1[c] = x1v[c];
             }
         }
         const bool emit = !first_row || yy == h - 1;
         if (emit && st.ds_writer) {
-            const bool two_cols = st.gx + 1 < w;
-            const int oy = yy >> 1, ox = st.gx >> 1;
+            const int gx0 = st.gx & ~1;
+            const bool two_cols = gx0 + 1 < w;
+            const int oy = yy >> 1, ox = gx0 >> 1;
             const uint64_t n2 = uint64_t(st.nw) * st.nh, o = uint64_t(oy) * st.nw + ox;
             const double inv = 1.0 / double((first_row ? 1 : 2) * (two_cols ? 2 : 1));
+            float* dst = odd ? st.b_next : st.a_next;
+            if (dst) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                // core/src/ssimulacra2.rs:236-243: row by row, left to right, in f64.
-                double sa = double(st.pa_own[c]), sb = double(st.pb_own[c]);
-                if (two_cols) {
-                    sa += double(st.pa_par[c]);
-                    sb += double(st.pb_par[c]);
-                }
-                if (!first_row) {
-                    sa += double(la[c]);
-                    sb += double(lb[c]);
-                    if (two_cols) {
-                        sa += double(qa[c]);
-                        sb += double(qb[c]);
+                for (int c = 0; c < 3; c++) {
+                    // core/src/ssimulacra2.rs:236-243: row by row, left to right, in f64.
+                    double sum = double(st.v0[c]);
+                    if (two_cols) sum += double(st.v1[c]);
+                    if (!first_row) {
+                        sum += double(x0v[c]);
+                        if (two_cols) sum += double(x1v[c]);
                     }
+                    gstore(dst + c * n2 + o, float(sum * inv));
                 }
-                if (st.want_a) gstore(st.a_next + c * n2 + o, float(sa * inv));
-                if (st.want_b) gstore(st.b_next + c * n2 + o, float(sb * inv));
             }
         }
     }
     if (yy >= st.y0 + 5) {
         // Vertical blur for output row yy-5: centre in slot (S+6)%11, distance t in (S+6-t)%11 and (S+6+t)%11.
         constexpr int SC = (S + 6) % 11;
+        constexpr int NQ = Form<T>::NQ;
         const int buf = (yy - 5) & 1;
         const T* k = st.taps;
         const int tid = st.tid;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const T a0 = st.ra[SC][c], b0 = st.rb[SC][c];
-            T ma = k[0] * a0, mb = k[0] * b0;
-            T saa = k[0] * (a0 * a0), sbb = k[0] * (b0 * b0), sab = k[0] * (a0 * b0);
+            T q[NQ];
+            Form<T>::centre(k[0], st.ra[SC][c], st.rb[SC][c], q);
 #pragma unroll
             for (int t = 1; t <= 5; t++) {
                 const int su = (S + 6 - t + 11) % 11, sd = (S + 6 + t) % 11;
-                const T au = st.ra[su][c], ad = st.ra[sd][c], bu = st.rb[su][c], bd = st.rb[sd][c];
-                const T kt = k[t];
-                ma += kt * (au + ad);
-                mb += kt * (bu + bd);
-                saa += kt * (au * au + ad * ad);
-                sbb += kt * (bu * bu + bd * bd);
-                sab += kt * (au * bu + ad * bd);
+                Form<T>::pair(k[t], st.ra[su][c], st.ra[sd][c], st.rb[su][c], st.rb[sd][c], q);
             }
-            st.s_q[buf][c * 5 + 0][tid] = ma;
-            st.s_q[buf][c * 5 + 1][tid] = mb;
-            st.s_q[buf][c * 5 + 2][tid] = saa;
-            st.s_q[buf][c * 5 + 3][tid] = sbb;
-            st.s_q[buf][c * 5 + 4][tid] = sab;
+#pragma unroll
+            for (int i = 0; i < NQ; i++) st.s_q[buf][c * NQ + i][tid] = q[i];
         }
         __syncthreads();
         if (st.is_out) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                T v[5];
+                T v[NQ];
 #pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    const T* row = &st.s_q[buf][c * 5 + q][tid];
+                for (int i = 0; i < NQ; i++) {
+                    const T* row = &st.s_q[buf][c * NQ + i][tid];
                     T sum = k[0] * row[0];
 #pragma unroll
                     for (int t = 1; t <= 5; t++) sum += k[t] * (row[-t] + row[t]);
-                    v[q] = sum;
+                    v[i] = sum;
                 }
-                // core/src/ssimulacra2.rs:424-440
-                const T m1 = v[0], m2 = v[1], s11 = v[2], s22 = v[3], s12 = v[4];
-                const T C2 = T(0.0009);
-                const T num_m = T(1) - (m1 - m2) * (m1 - m2);
-                const T num_s = T(2) * (s12 - m1 * m2) + C2;
-                const T denom_s = (s11 - m1 * m1) + (s22 - m2 * m2) + C2;
-                T d = Maps<T>::ssim(num_m, num_s, denom_s);
+                const T m1 = v[0], m2 = v[1];
+                T d = Form<T>::ssim(v);
                 d = d > T(0) ? d : T(0);
                 const T d1 = Maps<T>::edge(fabs(st.ra[SC][c] - m1), fabs(st.rb[SC][c] - m2));
                 const T art = d1 > T(0) ? d1 : T(0);
@@ -410,7 +455,7 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
                                                     Taps<T> taps, const float* __restrict__ lut_g) {
     constexpr int OW = NT - 12;
     constexpr int NWAVES = NT / 64;
-    __shared__ T s_q[2][15][NT];
+    __shared__ T s_q[2][3 * Form<T>::NQ][NT];
     __shared__ float s_lut[256];
     __shared__ double s_red[NWAVES][18];
 
@@ -444,12 +489,14 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     st.want_next = st.want_a || st.want_b;
     st.nw = (st.w + 1) >> 1;
     st.nh = (st.h + 1) >> 1;
-    st.ds_writer = st.want_next && st.is_out && ((st.gx & 1) == 0);
+    // Both lanes of a column pair write (the even one the reference's plane, the odd one the distorted one's) when
+    // the pair's left column is one of this tile's output columns inside the image.
+    st.ds_writer = st.want_next && tid >= 6 && tid < 6 + OW && (st.gx & ~1) < st.w;
     st.yy = st.y0 - 5;
 #pragma unroll
     for (int i = 0; i < 18; i++) st.acc[i] = T(0);
 #pragma unroll
-    for (int c = 0; c < 3; c++) st.pa_own[c] = st.pa_par[c] = st.pb_own[c] = st.pb_par[c] = 0.0f;
+    for (int c = 0; c < 3; c++) st.v0[c] = st.v1[c] = 0.0f;
     __syncthreads();
 
     while (walk_row<0>(st) && walk_row<1>(st) && walk_row<2>(st) && walk_row<3>(st) && walk_row<4>(st) &&
@@ -477,23 +524,26 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
 
 
 
-// One workgroup (64 threads) per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean
-// and 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
-__global__ __launch_bounds__(64) void k_ssim_finalize(const FinalDesc* __restrict__ fj) {
+// One 256-thread workgroup per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean and
+// 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
+constexpr int kFinThreads = 256;
+
+__global__ __launch_bounds__(kFinThreads) void k_ssim_finalize(const FinalDesc* __restrict__ fj) {
     const FinalDesc& F = fj[blockIdx.x];
+    __shared__ double s_wave[kFinThreads / 64][18];
     __shared__ double s_sum[6][18];
     __shared__ double s_stat[6][18];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t ns = F.nscales;
-    // Lane l sums tiles l, l+64, ... of every scale (18 accumulators in registers, each tile's 18 sums read as
-    // one contiguous 144-byte record), then a fixed butterfly combines the lanes.
+    // Thread t sums tiles t, t+256, ... of a scale (each tile's 18 sums are one contiguous 144-byte record); a
+    // fixed butterfly per wave and a fixed-order sum over the waves combine the threads.
     for (uint32_t sc = 0; sc < ns; sc++) {
         const double* p = F.partials[sc];
         const uint32_t nt = F.ntiles[sc];
         double acc[18];
 #pragma unroll
         for (int i = 0; i < 18; i++) acc[i] = 0.0;
-        for (uint32_t t = lane; t < nt; t += 64) {
+        for (uint32_t t = tid; t < nt; t += kFinThreads) {
             const double2* rec = reinterpret_cast<const double2*>(p + uint64_t(t) * 18);
 #pragma unroll
             for (int i = 0; i < 9; i++) {
@@ -507,18 +557,24 @@ __global__ __launch_bounds__(64) void k_ssim_finalize(const FinalDesc* __restric
             double v = acc[i];
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) s_sum[sc][i] = v;
+            if (lane == 0) s_wave[wave][i] = v;
         }
+        __syncthreads();
+        if (tid < 18) {
+            double v = 0.0;
+            for (int w = 0; w < kFinThreads / 64; w++) v += s_wave[w][tid];
+            s_sum[sc][tid] = v;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    // One lane per (scale, statistic): mean for even m, 4-norm for odd.
-    for (int k = lane; k < int(ns) * 18; k += 64) {
+    // One thread per (scale, statistic): mean for even entries, 4-norm for odd ones.
+    for (int k = tid; k < int(ns) * 18; k += kFinThreads) {
         const int sc = k / 18, i = k % 18;
         const double v = s_sum[sc][i] / F.count[sc];
         s_stat[sc][i] = (i & 1) ? pow(v, 0.25) : v;
     }
     __syncthreads();
-    if (lane != 0) return;
+    if (tid != 0) return;
     double ssim = 0.0;
     int w = 0;
     for (int c = 0; c < 3; c++)
@@ -594,7 +650,7 @@ int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const 
 int launch_finalize(rcg_ctx* ctx, const FinalDesc* d_jobs, int n) {
     if (n == 0) return RCG_OK;
     ProfScope ps(ctx, "k_ssim_finalize");
-    hipLaunchKernelGGL(k_ssim_finalize, dim3(n), dim3(64), 0, ctx->stream, d_jobs);
+    hipLaunchKernelGGL(k_ssim_finalize, dim3(n), dim3(kFinThreads), 0, ctx->stream, d_jobs);
     RCG_HIP(hipGetLastError());
     return RCG_OK;
 }

@@ -225,3 +225,24 @@ def test_score_batch_reports_per_job_errors(ctx, rc):
     res = ctx.score_batch([good, np.zeros((7, 64, 3), np.uint8)], CANDS, 70.0)
     assert res[0].status == 0 and res[0].chosen_quality > 0
     assert res[1].status == rc.UNSUPPORTED
+
+
+def test_tile_policy_roundtrip_and_validation(ctx, rc):
+    t = ctx.get_tune()
+    assert t.nclass >= 1 and t.min_h[t.nclass - 1] == 0
+    bad = rc.Tune.from_buffer_copy(t)
+    bad.th[0] = 7  # odd tile heights break the 2x2 pyramid pairing
+    with pytest.raises(rc.RcgError):
+        ctx.set_tune(bad)
+    ctx.set_tune(t)
+
+
+def test_autotuned_policy_keeps_parity(rc):
+    c = rc.Context(0)
+    t = c.autotune(4)
+    assert 1 <= t.nclass <= 7
+    assert all(t.th[i] in (8, 16, 32, 64, 128) for i in range(t.nclass))
+    img = rc.synth_rgb8(12, 700, 530)
+    dist = O.reconstruct(O.build_coefs(img, 60, O.S420))
+    assert abs(c.ssimulacra2(img, dist, rc.FP32) - O.ssimulacra2(img, dist)) <= TOL32
+    c.close()
