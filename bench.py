@@ -9,6 +9,9 @@ value = candidate-evaluation throughput in MP/s (image megapixels x candidates /
 the plain per-image GPU search (serial bisection over q in 1..100, rcg_find_quality_dev) in search-MP/s, the roofline
 of the dominant kernel (k_ssim_scale) from HIP events recorded on the library's own stream around every launch in
 the timed region, and the CPU oracle (C restatement of the reference CPU path, OpenMP threads) on a bounded sample.
+
+All device memory and synchronisation go through librcgpu (rcg_device_alloc, rcg_synchronize on the stream every
+kernel runs on); PyTorch is not loaded, so the process holds a single HIP runtime (torch bundles its own).
 """
 import argparse
 import importlib
@@ -50,12 +53,14 @@ def main():
                     help="only run a device-to-device copy of MIB mebibytes (HBM traffic counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
+        # One device-to-device copy of a known size (the traffic counters of a plain copy kernel).
+        rc = importlib.import_module("rust-compress_amd")
+        ctx = rc.Context(0)
         n = args.copy_calibration << 20
-        src = torch.empty(n, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        a = rc.DeviceBuffer(ctx, n)
+        b = rc.DeviceBuffer(ctx, n)
+        rc.lib().rcg_device_copy(ctx.handle, b.ptr, a.ptr, n)
+        ctx.synchronize()
         print(json.dumps({"copy_calibration_bytes": n}), flush=True)
         return
     if args.gpus != 1:
@@ -63,16 +68,14 @@ def main():
               "(images shard across GPUs with no collective; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     rc = importlib.import_module("rust-compress_amd")
     ctx = rc.Context(0)
     tune = ctx.autotune(8)  # tile heights per scale-height class, measured on this GPU
     progress(f"autotuned tile policy: {tune.as_dict()}")
     W = H = args.size
     img = rc.synth_rgb8(2, W, H)
-    d_img = torch.from_numpy(img).to("cuda")
-    torch.cuda.synchronize()
-    ptr = d_img.data_ptr()
+    d_img = ctx.upload(img)  # resident in HBM before the timed region
+    ptr = d_img.ptr
     cands = [(q, s) for s in (rc.S420, rc.S444) for q in (50, 75, 90)]
     target = 70.0
 
@@ -81,12 +84,12 @@ def main():
 
     for _ in range(args.warmup):
         res = step()
-    torch.cuda.synchronize()
+    ctx.synchronize()
     ctx.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = step()
-    torch.cuda.synchronize()
+    ctx.synchronize()
     t1 = time.perf_counter()
     ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
     enc_ms, enc_n = ctx.profile_get("k_encode")
@@ -134,7 +137,7 @@ def main():
 
     if not args.no_plain:
         ctx.find_quality_dev(ptr, W, H, 3, target, 100, rc.S420)
-        torch.cuda.synchronize()
+        ctx.synchronize()
         t0 = time.perf_counter()
         f = ctx.find_quality_dev(ptr, W, H, 3, target, 100, rc.S420)
         t1 = time.perf_counter()
@@ -157,7 +160,7 @@ def main():
                                          f"{W}x{H} image: oracle build_coefs + reconstruct + f64 SSIMULACRA2"}
     if not args.no_batch:
         progress("batched configuration")
-        out["batched"] = bench_batch(rc, ctx, torch, args)
+        out["batched"] = bench_batch(rc, ctx, args)
     if not args.no_serve:
         progress("serving configuration")
         out["serving"] = bench_serve(rc, args, tune)
@@ -168,7 +171,7 @@ def main():
 CANDS_B = (40, 50, 60, 70, 80, 85, 90, 95)
 
 
-def bench_batch(rc, ctx, torch, args):
+def bench_batch(rc, ctx, args):
     """256 synthetic RGB8 images, W and H uniform in [64, 3000], each swept over 8 candidate qualities at 4:2:0 in
     one batched pass; choice = lowest q with SSIMULACRA2 >= 70 by the reference's bisection over the sorted list.
     Compared with the plain per-image GPU search (serial bisection over q in 1..100) on the same images."""
@@ -178,20 +181,20 @@ def bench_batch(rc, ctx, torch, args):
     t0 = time.perf_counter()
     dev = []
     for i, (w, h) in enumerate(sizes):
-        dev.append(torch.from_numpy(rc.synth_rgb8(1000 + i, w, h)).to("cuda"))
+        dev.append(ctx.upload(rc.synth_rgb8(1000 + i, w, h)))
         if i % 64 == 63:
             progress(f"generated {i + 1} batch images")
-    torch.cuda.synchronize()
+    ctx.synchronize()
     gen_s = time.perf_counter() - t0
-    ptrs = [(t.data_ptr(), w, h, 3) for t, (w, h) in zip(dev, sizes)]
+    ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
     cands = [(q, rc.S420) for q in CANDS_B]
     total_mp = sum(w * h for w, h in sizes) / 1e6
     ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)  # warm-up (sizes the workspace)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     ctx.profile(True)
     t0 = time.perf_counter()
     res = ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     t1 = time.perf_counter()
     ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
     ctx.profile(False)
@@ -205,6 +208,18 @@ def bench_batch(rc, ctx, torch, args):
            "chosen_equals_sweep": int(sum(r.chosen_quality == r.sweep_quality for r in res)),
            "generate_s": round(gen_s, 1)}
     progress(f"batched pass: {batch_s:.3f} s")
+    # The same pass captured once as a hipGraph and replayed.
+    plan = ctx.batch_plan(ptrs, cands, 70.0)
+    plan.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    gres = plan.run()
+    ctx.synchronize()
+    graph_s = time.perf_counter() - t0
+    plan.close()
+    out["graph_seconds"] = round(graph_s, 4)
+    out["graph_search_mps"] = round(total_mp / graph_s, 2)
+    out["graph_matches_eager"] = int(all(g.chosen_quality == r.chosen_quality for g, r in zip(gres, res)))
     if not args.no_plain:
         t0 = time.perf_counter()
         steps = 0
@@ -216,8 +231,8 @@ def bench_batch(rc, ctx, torch, args):
         out["plain_eval_mps"] = round(total_mp * steps / len(ptrs) / (t1 - t0), 2)
         out["plain_seconds"] = round(t1 - t0, 3)
         out["batched_vs_plain"] = round(out["search_mps"] / out["plain_search_mps"], 2)
-    del dev
-    torch.cuda.empty_cache()
+    for d in dev:
+        d.free()
     return out
 
 

@@ -53,6 +53,16 @@ const char* rcg_last_error(void);
 /* Library version string. */
 const char* rcg_version(void);
 
+/* Device buffers for the _dev entry points (resident sources, outputs): plain hipMalloc'd memory on the
+ * context's device, copies ordered on the context's stream and complete on return. */
+int rcg_device_alloc(rcg_ctx* ctx, size_t bytes, void** d_ptr);
+int rcg_device_free(rcg_ctx* ctx, void* d_ptr);
+int rcg_memcpy_h2d(rcg_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int rcg_memcpy_d2h(rcg_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int rcg_device_copy(rcg_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+/* Wait for everything enqueued on the context's stream. */
+int rcg_synchronize(rcg_ctx* ctx);
+
 /* Coefficient layout build_coefs produces (jpeg/src/encode.rs:428-443): sampling factors and block grids. */
 typedef struct {
     uint32_t width, height;
@@ -167,6 +177,18 @@ typedef struct {
 } rcg_batch_result;
 int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
 int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
+
+/*
+ * The batched scoring of rcg_score_batch_dev captured once as a hipGraph: rcg_batch_plan_create allocates every
+ * buffer, uploads every descriptor and records the launch sequence (encode, six scale launches, pooling, one
+ * copy of the scores to pinned host memory); rcg_batch_plan_run replays it with one graph launch, then settles
+ * scores near a target in FP64 and applies the selection. The jobs' device pixels must stay valid and unchanged in
+ * place between runs (new pixel values at the same addresses are fine). No counterpart in the reference.
+ */
+typedef struct rcg_plan rcg_plan;
+int rcg_batch_plan_create(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_plan** out);
+int rcg_batch_plan_run(rcg_plan* plan, rcg_batch_result* results);
+void rcg_batch_plan_destroy(rcg_plan* plan);
 
 /*
  * Tile policy of the SSIMULACRA2 scale kernel: a scale of height h uses th[i] output rows per tile for the first

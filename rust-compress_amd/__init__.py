@@ -137,6 +137,16 @@ def lib():
         L.rcg_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
         L.rcg_server_destroy.argtypes = [P]
         L.rcg_server_destroy.restype = None
+        L.rcg_device_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
+        L.rcg_device_free.argtypes = [P, P]
+        L.rcg_memcpy_h2d.argtypes = [P, P, P, ctypes.c_size_t]
+        L.rcg_memcpy_d2h.argtypes = [P, P, P, ctypes.c_size_t]
+        L.rcg_synchronize.argtypes = [P]
+        L.rcg_device_copy.argtypes = [P, P, P, ctypes.c_size_t]
+        L.rcg_batch_plan_create.argtypes = [P, ctypes.POINTER(BatchJob), I, ctypes.POINTER(P)]
+        L.rcg_batch_plan_run.argtypes = [P, ctypes.POINTER(BatchResult)]
+        L.rcg_batch_plan_destroy.argtypes = [P]
+        L.rcg_batch_plan_destroy.restype = None
         L.rcg_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
         L.rcg_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
         L.rcg_autotune.argtypes = [P, I, ctypes.POINTER(Tune)]
@@ -301,18 +311,27 @@ class Context:
             qts = [qt[:64].copy(), qt[64:].copy(), None, None]
         return Found(f.quality, f.score, CoefImage(w, h, comps, qts), f.steps)
 
+    # ------------------------------------------------------------------ device buffers
+
+    def upload(self, array):
+        """Copy a numpy array into a new device buffer; returns a DeviceBuffer."""
+        a = np.ascontiguousarray(array)
+        buf = DeviceBuffer(self, a.nbytes)
+        _check(lib().rcg_memcpy_h2d(self._h, buf.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+        return buf
+
+    def synchronize(self):
+        _check(lib().rcg_synchronize(self._h))
+
     # ------------------------------------------------------------------ batched scoring
 
-    def score_batch(self, images, candidates, targets, device_ptrs=None):
-        """Score every image against its candidate list [(quality, subsampling), ...] in one launch sequence.
-
-        images: list of numpy arrays (host) or, with device_ptrs, a list of (ptr, width, height, channels).
-        Returns a list of BatchResult (scores, settled flags, chosen_* = bisection replay, sweep_* = exhaustive)."""
+    def _batch_jobs(self, images, candidates, targets, device_ptrs):
         n = len(images) if device_ptrs is None else len(device_ptrs)
         jobs = (BatchJob * n)()
         keep = []
+        per_image = isinstance(candidates[0], (list, tuple)) and isinstance(candidates[0][0], (list, tuple))
         for i in range(n):
-            cand = candidates[i] if isinstance(candidates[0], (list, tuple)) and isinstance(candidates[0][0], (list, tuple)) else candidates
+            cand = candidates[i] if per_image else candidates
             if device_ptrs is None:
                 img = np.ascontiguousarray(images[i], dtype=np.uint8)
                 keep.append(img)
@@ -328,9 +347,25 @@ class Context:
                 jobs[i].quality[k] = q
                 jobs[i].subsampling[k] = sub
             jobs[i].target = float(targets[i] if isinstance(targets, (list, tuple, np.ndarray)) else targets)
+        return jobs, keep
+
+    def batch_plan(self, device_ptrs, candidates, targets):
+        """Capture the batched scoring of device-resident images as a hipGraph (see BatchPlan)."""
+        jobs, _ = self._batch_jobs(None, candidates, targets, device_ptrs)
+        return BatchPlan(self, jobs)
+
+    def score_batch(self, images, candidates, targets, device_ptrs=None):
+        """Score every image against its candidate list [(quality, subsampling), ...] in one launch sequence.
+
+        images: list of numpy arrays (host) or, with device_ptrs, a list of (ptr, width, height, channels).
+        candidates: one list for every image, or a list of lists. Returns a list of BatchResult (scores, settled
+        flags, chosen_* = the reference's bisection over the sorted list, sweep_* = exhaustive choice)."""
+        jobs, keep = self._batch_jobs(images, candidates, targets, device_ptrs)
+        n = len(jobs)
         res = (BatchResult * n)()
         fn = lib().rcg_score_batch if device_ptrs is None else lib().rcg_score_batch_dev
         _check(fn(self._h, jobs, n, res))
+        del keep
         return list(res)
 
     # ------------------------------------------------------------------ tile policy
@@ -406,6 +441,60 @@ def _coef_image_struct(ci, keep, device):
             s.qtables[slot] = qa.ctypes.data
     s.adobe_transform = ci.adobe_transform
     return s
+
+
+class DeviceBuffer:
+    """A device allocation on the context's GPU (rcg_device_alloc); .ptr is the raw device address."""
+
+    def __init__(self, ctx, nbytes):
+        self._ctx = ctx
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        _check(lib().rcg_device_alloc(ctx.handle, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def download(self, out):
+        assert out.nbytes <= self.nbytes and out.flags["C_CONTIGUOUS"]
+        _check(lib().rcg_memcpy_d2h(self._ctx.handle, out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(lib().rcg_device_free(self._ctx.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """rcg_batch_plan: the batched launch sequence captured once as a hipGraph, replayed by run()."""
+
+    def __init__(self, ctx, jobs):
+        self._ctx = ctx
+        self._n = len(jobs)
+        self._jobs = jobs
+        self._h = ctypes.c_void_p()
+        _check(lib().rcg_batch_plan_create(ctx.handle, jobs, self._n, ctypes.byref(self._h)))
+
+    def run(self):
+        res = (BatchResult * self._n)()
+        _check(lib().rcg_batch_plan_run(self._h, res))
+        return list(res)
+
+    def close(self):
+        if self._h:
+            lib().rcg_batch_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

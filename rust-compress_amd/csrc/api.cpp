@@ -1006,3 +1006,59 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
     if (out) tune_to_c(N, out);
     return RCG_OK;
 }
+
+// ================================================================================================================
+// Device buffers
+
+extern "C" int rcg_device_alloc(rcg_ctx* ctx, size_t bytes, void** d_ptr) {
+    if (!ctx || !d_ptr) return RCG_INVALID;
+    *d_ptr = nullptr;
+    RCG_HIP(hipSetDevice(ctx->device));
+    RCG_HIP(hipMalloc(d_ptr, bytes ? bytes : 1));
+    return RCG_OK;
+}
+
+extern "C" int rcg_device_free(rcg_ctx* ctx, void* d_ptr) {
+    if (!ctx) return RCG_INVALID;
+    if (!d_ptr) return RCG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    RCG_HIP(hipFree(d_ptr));
+    return RCG_OK;
+}
+
+extern "C" int rcg_memcpy_h2d(rcg_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+    if (!ctx || (!d_dst && bytes) || (!src && bytes)) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    RCG_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}
+
+extern "C" int rcg_memcpy_d2h(rcg_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!d_src && bytes)) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    RCG_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}
+
+extern "C" int rcg_synchronize(rcg_ctx* ctx) {
+    if (!ctx) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}
+
+extern "C" int rcg_device_copy(rcg_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx || (bytes && (!d_dst || !d_src))) return RCG_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    RCG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return RCG_OK;
+}

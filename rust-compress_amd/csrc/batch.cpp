@@ -129,29 +129,39 @@ size_t desc_bytes(const Item* items, int n) {
     size_t d = 1 << 16;
     for (int i = 0; i < n; i++) {
         d += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * items[i].ncand + 64 * (RCG_MAX_CAND + 2);
-        d += (sizeof(SsimJob) + 64) * 6 * items[i].ncand + sizeof(FinalDesc) * items[i].ncand + 256;
-        d += 2 * 6 * 128;
+        // FP32 pass plus, after it, an FP64 settling pass of up to every pair.
+        d += 2 * ((sizeof(SsimJob) + 64) * 6 * items[i].ncand + sizeof(FinalDesc) * items[i].ncand + 256);
+        d += 4 * 6 * 128;
     }
     return d;
 }
 
-// Scores every pair in `pairs` at `precision`. FP32 writes the pyramids (the reference's by each item's first
-// candidate unless already built); FP64 reads both from the FP32 pass.
-int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& bufs, const std::vector<PairRef>& pairs,
-              int precision, double* d_scores) {
+// Staged SSIMULACRA2 launches of a set of pairs: one launch per (scale, thread count) and one pooling launch.
+struct StagedLaunch {
+    const SsimJob* jobs;
+    const uint32_t* begins;
+    int n;
+    uint32_t tiles;
+    int nt;
+    uint32_t ka, kb;
+};
+struct StagedPairs {
+    std::vector<StagedLaunch> scales;
+    const FinalDesc* fin = nullptr;
+    int npairs = 0;
+    int precision = RCG_PREC_FP32;
+};
+
+// Stages the descriptors to score every pair in `pairs` at `precision` into D. FP32 writes the pyramids (the
+// reference's by each item's first candidate unless already built); FP64 reads both from the FP32 pass.
+StagedPairs stage_pairs(Desc& D, const Item* items, std::vector<ItemBufs>& bufs, const std::vector<PairRef>& pairs,
+                        int precision, double* d_scores) {
     const bool f64 = precision == RCG_PREC_FP64;
+    StagedPairs out;
+    out.precision = precision;
+    out.npairs = int(pairs.size());
     std::vector<FinalDesc> fins(pairs.size());
     for (size_t i = 0; i < pairs.size(); i++) std::memset(&fins[i], 0, sizeof(FinalDesc));
-    // One launch per (scale, thread count): the tile policy gives narrow scales 128-thread workgroups.
-    struct Staged {
-        const SsimJob* jobs;
-        const uint32_t* begins;
-        int n;
-        uint32_t tiles;
-        int nt;
-        uint32_t ka, kb;
-    };
-    std::vector<Staged> st;
     for (int s = 0; s < 6; s++) {
         for (int nt : {256, 128}) {
             std::vector<SsimJob> sj;
@@ -194,7 +204,7 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
                 total += P.tiles[s];
             }
             if (!sj.empty()) {
-                Staged g;
+                StagedLaunch g;
                 g.jobs = D.put(sj.data(), sj.size());
                 g.begins = D.put(begins.data(), begins.size());
                 g.n = int(sj.size());
@@ -202,7 +212,7 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
                 g.nt = nt;
                 g.ka = s == 0 ? uint32_t(kSrcBytes) : uint32_t(kSrcPlanes);
                 g.kb = s == 0 ? uint32_t(kSrcRecon) : uint32_t(kSrcPlanes);
-                st.push_back(g);
+                out.scales.push_back(g);
             }
         }
     }
@@ -210,11 +220,14 @@ int run_pairs(rcg_ctx* ctx, Desc& D, const Item* items, std::vector<ItemBufs>& b
         fins[i].score_out = d_scores + i;
         fins[i].stats_out = nullptr;
     }
-    const FinalDesc* dfin = D.put(fins.data(), fins.size());
-    RCG_TRY(D.upload(ctx->stream));
-    for (const Staged& g : st)
-        RCG_TRY(launch_ssim_scale(ctx, precision, g.jobs, g.begins, g.n, g.tiles, g.nt, g.ka, g.kb));
-    return launch_finalize(ctx, dfin, int(pairs.size()));
+    out.fin = D.put(fins.data(), fins.size());
+    return out;
+}
+
+int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp) {
+    for (const StagedLaunch& g : sp.scales)
+        RCG_TRY(launch_ssim_scale(ctx, sp.precision, g.jobs, g.begins, g.n, g.tiles, g.nt, g.ka, g.kb));
+    return launch_finalize(ctx, sp.fin, sp.npairs);
 }
 
 }  // namespace
@@ -235,29 +248,44 @@ size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n) {
     return need + Arena::align(desc_bytes(items, n)) + 2 * Arena::align(npairs * sizeof(double) + 8) + (1 << 12);
 }
 
-// Score every candidate of every item: one encode launch, one launch per scale, one pooling launch; then FP64
-// settling of the scores inside the guard band of each item's target. The arena must already hold
-// score_items_bytes() beyond what the caller took.
-int score_items(rcg_ctx* ctx, Item* items, int n) {
+// The device state of one scoring pass: per-item buffers, staged descriptors (already uploaded) and the FP32
+// launch sequence. prepare_items() does all allocation and uploading; enqueue_items() only enqueues kernels and one
+// device-to-host copy, so a stream capture of it replays as a graph; finish_items() reads the scores back and
+// settles the ones near a target with the FP64 kernel.
+struct ScoreState {
+    std::vector<ItemBufs> bufs;
+    std::vector<PairRef> pairs;
+    const EncJob* d_ej = nullptr;
+    const uint32_t* d_eb = nullptr;
+    int nej = 0;
+    uint32_t etiles = 0;
+    StagedPairs fp32;
+    double* d_s32 = nullptr;
+    double* d_s64 = nullptr;
+    double* h_scores = nullptr;
+    Desc D;
+    size_t fp32_desc_end = 0;
+};
+
+int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, ScoreState& S) {
     size_t npairs = 0;
     for (int i = 0; i < n; i++) npairs += items[i].ncand;
     const size_t dbytes = desc_bytes(items, n);
-    RCG_TRY(ctx->pinned.ensure(Arena::align(dbytes) + Arena::align(npairs * sizeof(double) + 8)));
-    Desc D;
-    D.host = ctx->pinned.base;
-    D.dev = static_cast<char*>(ctx->arena.take(dbytes));
-    D.cap = dbytes;
-    double* d_s32 = static_cast<double*>(ctx->arena.take(npairs * sizeof(double) + 8));
-    double* d_s64 = static_cast<double*>(ctx->arena.take(npairs * sizeof(double) + 8));
-    double* h_scores = reinterpret_cast<double*>(ctx->pinned.base + Arena::align(dbytes));
-
-    std::vector<ItemBufs> bufs(n);
+    RCG_TRY(pin.ensure(Arena::align(dbytes) + Arena::align(npairs * sizeof(double) + 8)));
+    S.D.host = pin.base;
+    S.D.dev = static_cast<char*>(A.take(dbytes));
+    S.D.cap = dbytes;
+    S.D.off = 0;
+    S.d_s32 = static_cast<double*>(A.take(npairs * sizeof(double) + 8));
+    S.d_s64 = static_cast<double*>(A.take(npairs * sizeof(double) + 8));
+    S.h_scores = reinterpret_cast<double*>(pin.base + Arena::align(dbytes));
+    S.bufs.assign(n, ItemBufs());
+    S.pairs.clear();
     std::vector<EncJob> ej;
     std::vector<std::vector<EncCand>> ecands;
-    std::vector<PairRef> pairs;
     for (int i = 0; i < n; i++) {
         Item& it = items[i];
-        ItemBufs& b = bufs[i];
+        ItemBufs& b = S.bufs[i];
         b.P32 = plan_pair(it.w, it.h, ctx->tune32);
         b.P64 = plan_pair(it.w, it.h, ctx->tune64);
         const int first_group = int(ej.size());
@@ -266,15 +294,15 @@ int score_items(rcg_ctx* ctx, Item* items, int n) {
             it.settled[c] = 0;
             b.L[c] = make_layout(it.w, it.h, it.ch, it.quality[c], it.subsampling[c]);
             for (uint32_t k = 0; k < b.L[c].ncomp; k++)
-                b.rec[c][k] = static_cast<uint8_t*>(ctx->arena.take(b.L[c].plane_bytes(k)));
+                b.rec[c][k] = static_cast<uint8_t*>(A.take(b.L[c].plane_bytes(k)));
             for (uint32_t s = 1; s < b.P32.nscales; s++)
                 b.dist_planes[c][s] =
-                    static_cast<float*>(ctx->arena.take(size_t(b.P32.sw[s]) * b.P32.sh[s] * 3 * sizeof(float)));
+                    static_cast<float*>(A.take(size_t(b.P32.sw[s]) * b.P32.sh[s] * 3 * sizeof(float)));
             for (uint32_t s = 0; s < b.P32.nscales; s++)
-                b.part32[c][s] = static_cast<double*>(ctx->arena.take(size_t(b.P32.tiles[s]) * 18 * 8));
+                b.part32[c][s] = static_cast<double*>(A.take(size_t(b.P32.tiles[s]) * 18 * 8));
             for (uint32_t s = 0; s < b.P64.nscales; s++)
-                b.part64[c][s] = static_cast<double*>(ctx->arena.take(size_t(b.P64.tiles[s]) * 18 * 8));
-            pairs.push_back(PairRef{i, int(c)});
+                b.part64[c][s] = static_cast<double*>(A.take(size_t(b.P64.tiles[s]) * 18 * 8));
+            S.pairs.push_back(PairRef{i, int(c)});
             // Candidates with the same sampling layout share one encode job (colour + FDCT once).
             int g = -1;
             for (int e = first_group; e < int(ej.size()); e++)
@@ -308,45 +336,70 @@ int score_items(rcg_ctx* ctx, Item* items, int n) {
         }
     }
     std::vector<uint32_t> ebegins(ej.size());
-    uint32_t etiles = 0;
+    S.etiles = 0;
     for (size_t e = 0; e < ej.size(); e++) {
-        ej[e].cand = D.put(ecands[e].data(), ecands[e].size());
-        ej[e].tile_begin = etiles;
-        ebegins[e] = etiles;
-        etiles += ej[e].tiles_x * ej[e].tiles_y;
+        ej[e].cand = S.D.put(ecands[e].data(), ecands[e].size());
+        ej[e].tile_begin = S.etiles;
+        ebegins[e] = S.etiles;
+        S.etiles += ej[e].tiles_x * ej[e].tiles_y;
     }
+    S.nej = int(ej.size());
     if (!ej.empty()) {
-        const EncJob* d_ej = D.put(ej.data(), ej.size());
-        const uint32_t* d_eb = D.put(ebegins.data(), ebegins.size());
-        RCG_TRY(D.upload(ctx->stream));
-        RCG_TRY(launch_encode(ctx, d_ej, d_eb, int(ej.size()), etiles));
+        S.d_ej = S.D.put(ej.data(), ej.size());
+        S.d_eb = S.D.put(ebegins.data(), ebegins.size());
     }
-    if (!pairs.empty()) {
-        RCG_TRY(run_pairs(ctx, D, items, bufs, pairs, RCG_PREC_FP32, d_s32));
-        RCG_HIP(hipMemcpyAsync(h_scores, d_s32, pairs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    }
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<PairRef> settle;
-    for (size_t i = 0; i < pairs.size(); i++) {
-        const double s = h_scores[i];
-        Item& it = items[pairs[i].item];
-        it.scores[pairs[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
-        if (std::isfinite(s) && std::fabs(s - it.target) < kGuard) settle.push_back(pairs[i]);
-    }
-    for (int i = 0; i < n; i++) items[i].ref_ready = true;
-    if (!settle.empty()) {
-        D.off = 0;
-        RCG_TRY(run_pairs(ctx, D, items, bufs, settle, RCG_PREC_FP64, d_s64));
-        RCG_HIP(hipMemcpyAsync(h_scores, d_s64, settle.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        RCG_HIP(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < settle.size(); i++) {
-            const double s = h_scores[i];
-            Item& it = items[settle[i].item];
-            it.scores[settle[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
-            it.settled[settle[i].cand] = 1;
-        }
+    S.fp32 = stage_pairs(S.D, items, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32);
+    S.fp32_desc_end = S.D.off;
+    return S.D.upload(ctx->stream);
+}
+
+int enqueue_items(rcg_ctx* ctx, const ScoreState& S) {
+    if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
+    if (!S.pairs.empty()) {
+        RCG_TRY(launch_pairs(ctx, S.fp32));
+        RCG_HIP(hipMemcpyAsync(S.h_scores, S.d_s32, S.pairs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
     }
     return RCG_OK;
+}
+
+// The stream must be synchronised (the FP32 scores are in S.h_scores).
+int finish_items(rcg_ctx* ctx, ScoreState& S, Item* items, int n) {
+    std::vector<PairRef> settle;
+    for (size_t i = 0; i < S.pairs.size(); i++) {
+        const double s = S.h_scores[i];
+        Item& it = items[S.pairs[i].item];
+        it.scores[S.pairs[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
+        it.settled[S.pairs[i].cand] = 0;
+        if (std::isfinite(s) && std::fabs(s - it.target) < kGuard) settle.push_back(S.pairs[i]);
+    }
+    for (int i = 0; i < n; i++) items[i].ref_ready = true;
+    if (settle.empty()) return RCG_OK;
+    // FP64 descriptors go after the FP32 ones, which stay valid for later replays.
+    S.D.off = S.fp32_desc_end;
+    const StagedPairs f64 = stage_pairs(S.D, items, S.bufs, settle, RCG_PREC_FP64, S.d_s64);
+    RCG_TRY(S.D.upload(ctx->stream));
+    RCG_TRY(launch_pairs(ctx, f64));
+    RCG_HIP(hipMemcpyAsync(S.h_scores, S.d_s64, settle.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < settle.size(); i++) {
+        const double s = S.h_scores[i];
+        Item& it = items[settle[i].item];
+        it.scores[settle[i].cand] = std::isfinite(s) ? s : -DBL_MAX;
+        it.settled[settle[i].cand] = 1;
+    }
+    return RCG_OK;
+}
+
+// Score every candidate of every item: one encode launch, one launch per scale, one pooling launch; then FP64
+// settling of the scores inside the guard band of each item's target. The arena must already hold
+// score_items_bytes() beyond what the caller took.
+int score_items(rcg_ctx* ctx, Item* items, int n) {
+    ScoreState S;
+    RCG_TRY(prepare_items(ctx, ctx->arena, ctx->pinned, items, n, S));
+    RCG_TRY(enqueue_items(ctx, S));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return finish_items(ctx, S, items, n);
 }
 
 // Selection of jpeg/src/target.rs:37-55 over a candidate list sorted by quality: the highest first, then bisection
@@ -455,4 +508,153 @@ extern "C" int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int 
 
 extern "C" int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results) {
     return score_batch_impl(ctx, jobs, njobs, results, true);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// rcg_batch_plan: the batched launch sequence captured once as a hipGraph and replayed per run.
+
+struct rcg_plan {
+    rcg_ctx* ctx = nullptr;
+    Arena arena;
+    Pinned pinned;
+    std::vector<rcg_batch_job> jobs;
+    std::vector<Item> items;
+    std::vector<int> job_of;
+    std::vector<int> status;
+    ScoreState S;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" void rcg_batch_plan_destroy(rcg_plan* p) {
+    if (!p) return;
+    if (p->ctx) {
+        std::lock_guard<std::mutex> lock(p->ctx->mu);
+        (void)hipSetDevice(p->ctx->device);
+        (void)hipStreamSynchronize(p->ctx->stream);
+    }
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->arena.base) (void)hipFree(p->arena.base);
+    if (p->pinned.base) (void)hipHostFree(p->pinned.base);
+    delete p;
+}
+
+extern "C" int rcg_batch_plan_create(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_plan** out) {
+    if (!ctx || !out || njobs < 0 || (!jobs && njobs)) return RCG_INVALID;
+    *out = nullptr;
+    rcg_plan* p = new (std::nothrow) rcg_plan();
+    if (!p) return RCG_OOM;
+    p->ctx = ctx;
+    p->jobs.assign(jobs, jobs + njobs);
+    p->status.assign(njobs, RCG_OK);
+    size_t own = 0;
+    for (int i = 0; i < njobs; i++) {
+        const rcg_batch_job& j = jobs[i];
+        int st = check_image(j.width, j.height, j.channels);
+        if (st == RCG_OK && (j.width < 8 || j.height < 8)) st = RCG_UNSUPPORTED;
+        if (st == RCG_OK && (j.ncand == 0 || j.ncand > RCG_MAX_CAND || !j.pixels)) st = RCG_INVALID;
+        p->status[i] = st;
+        if (st != RCG_OK) continue;
+        Item it;
+        std::memset(&it, 0, sizeof(it));
+        it.d_src = j.pixels;
+        it.w = j.width;
+        it.h = j.height;
+        it.ch = j.channels;
+        it.ncand = j.ncand;
+        for (uint32_t c = 0; c < j.ncand; c++) {
+            it.quality[c] = j.quality[c];
+            it.subsampling[c] = j.subsampling[c];
+        }
+        it.target = j.target;
+        p->items.push_back(it);
+        p->job_of.push_back(i);
+        own += ref_pyramid_bytes(plan_pair(j.width, j.height, ctx->tune32));
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = RCG_OK;
+    {
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) rc = RCG_DEVICE;
+    }
+    if (rc == RCG_OK) rc = p->arena.ensure(own + score_items_bytes(ctx, p->items.data(), int(p->items.size())));
+    if (rc == RCG_OK) {
+        for (Item& it : p->items) {
+            take_ref_pyramid(p->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref_planes);
+            it.ref_ready = false;
+        }
+        rc = prepare_items(ctx, p->arena, p->pinned, p->items.data(), int(p->items.size()), p->S);
+    }
+    if (rc == RCG_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RCG_DEVICE;
+    if (rc == RCG_OK) {
+        // Capture the FP32 launch sequence (kernels and one device-to-host copy; no memsets, no allocation).
+        const bool prof = ctx->prof.on;
+        ctx->prof.on = false;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = enqueue_items(ctx, p->S);
+            hipError_t e2 = hipStreamEndCapture(ctx->stream, &p->graph);
+            if (rc == RCG_OK && e2 != hipSuccess) {
+                set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+                rc = RCG_DEVICE;
+            }
+        } else {
+            set_error(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+            rc = RCG_DEVICE;
+        }
+        ctx->prof.on = prof;
+        if (rc == RCG_OK) {
+            e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                rc = RCG_DEVICE;
+            }
+        }
+    }
+    if (rc != RCG_OK) {
+        p->ctx = nullptr;  // the lock is held: destroy without re-locking
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+        if (p->arena.base) (void)hipFree(p->arena.base);
+        if (p->pinned.base) (void)hipHostFree(p->pinned.base);
+        p->arena.base = nullptr;
+        p->pinned.base = nullptr;
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return RCG_OK;
+}
+
+extern "C" int rcg_batch_plan_run(rcg_plan* p, rcg_batch_result* results) {
+    if (!p || (!results && !p->jobs.empty())) return RCG_INVALID;
+    rcg_ctx* ctx = p->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RCG_HIP(hipSetDevice(ctx->device));
+    if (p->exec) {
+        ProfScope ps(ctx, "batch_graph");
+        RCG_HIP(hipGraphLaunch(p->exec, ctx->stream));
+    }
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    RCG_TRY(finish_items(ctx, p->S, p->items.data(), int(p->items.size())));
+    for (size_t i = 0; i < p->jobs.size(); i++) {
+        std::memset(&results[i], 0, sizeof(rcg_batch_result));
+        results[i].status = p->status[i];
+        results[i].chosen_index = results[i].sweep_index = -1;
+    }
+    for (size_t k = 0; k < p->items.size(); k++) {
+        const Item& it = p->items[k];
+        rcg_batch_result& r = results[p->job_of[k]];
+        for (uint32_t c = 0; c < it.ncand; c++) {
+            r.scores[c] = it.scores[c];
+            r.settled[c] = it.settled[c];
+        }
+        r.chosen_index = select_candidate(it.quality, it.scores, int(it.ncand), it.target, &r.sweep_index);
+        r.chosen_quality = it.quality[r.chosen_index];
+        r.sweep_quality = it.quality[r.sweep_index];
+    }
+    return RCG_OK;
 }

@@ -246,3 +246,43 @@ def test_autotuned_policy_keeps_parity(rc):
     dist = O.reconstruct(O.build_coefs(img, 60, O.S420))
     assert abs(c.ssimulacra2(img, dist, rc.FP32) - O.ssimulacra2(img, dist)) <= TOL32
     c.close()
+
+
+def test_graph_plan_replays_match_eager(ctx, rc):
+    # The captured batch is replayed several times with eager work and synchronisations in between; every replay
+    # must give exactly the eager scores and choices.
+    imgs = [rc.synth_rgb8(700 + i, w, h) for i, (w, h) in enumerate([(300, 200), (97, 131), (640, 480), (64, 64)])]
+    dev = [ctx.upload(im) for im in imgs]
+    ptrs = [(d.ptr, im.shape[1], im.shape[0], 3) for d, im in zip(dev, imgs)]
+    eager = ctx.score_batch(None, CANDS, 70.0, device_ptrs=ptrs)
+    plan = ctx.batch_plan(ptrs, CANDS, 70.0)
+    for rep in range(3):
+        got = plan.run()
+        for g, e in zip(got, eager):
+            assert list(g.scores[:len(CANDS)]) == list(e.scores[:len(CANDS)]), rep
+            assert g.chosen_quality == e.chosen_quality and g.sweep_quality == e.sweep_quality
+        # eager work on the same context between replays
+        again = ctx.score_batch(None, CANDS, 70.0, device_ptrs=ptrs)
+        assert list(again[0].scores[:len(CANDS)]) == list(eager[0].scores[:len(CANDS)])
+        ctx.synchronize()
+    plan.close()
+    for d in dev:
+        d.free()
+
+
+def test_device_entry_points_match_host_ones(ctx, rc):
+    img = rc.synth_rgb8(31, 333, 250)
+    d = ctx.upload(img)
+    L = rc.coef_layout(333, 250, 3, 80, rc.S420)
+    outs = [rc.DeviceBuffer(ctx, L.blocks_w[c] * L.blocks_h[c] * 64 * 2) for c in range(3)]
+    ctx.build_coefs_dev(d.ptr, 333, 250, 3, 80, rc.S420, [o.ptr for o in outs])
+    host = ctx.build_coefs(img, 80, rc.S420)
+    for o, comp in zip(outs, host.comps):
+        got = o.download(np.empty(len(comp.coefs), np.int16))
+        np.testing.assert_array_equal(got, comp.coefs)
+    dist = O.reconstruct(O.build_coefs(img, 50, O.S420))
+    dd = ctx.upload(dist)
+    assert ctx.ssimulacra2_dev(d.ptr, dd.ptr, 333, 250, 3) == ctx.ssimulacra2(img, dist)
+    f1 = ctx.find_quality_dev(d.ptr, 333, 250, 3, 70.0)
+    f2 = ctx.find_quality(img, 70.0)
+    assert (f1.quality, f1.score, f1.steps) == (f2.quality, f2.score, f2.steps)
