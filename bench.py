@@ -27,7 +27,8 @@ METRIC = "MP/s of JPEG DCT+quant+SSIMULACRA2 search, batched vs plain; p99 laten
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per pixel per candidate of the dominant kernel (SSIMULACRA2 scales): read the reference and the
 # reconstruction at scale 1 (6), write and read back the reconstruction's pyramid (4 + 4), read the reference's
-# cached pyramid (4).
+# cached pyramid (4). The kernel's real fetches are larger: it reads the reference side from the per-image cache
+# (9 f32 per pixel per scale) in place of the reference's pixels, trading HBM bytes for VALU work.
 SSIM_BYTES_PER_PX = 18.0
 # Whole candidate evaluation: + read RGB8 (3), write/read coefficients (3 + 3, or 6 + 6 at 4:4:4), write RGB8 (3).
 EVAL_BYTES_PER_PX = {3: 30.0, 1: 36.0}
@@ -65,7 +66,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(images shard across GPUs with no collective; see DESIGN.md)", file=sys.stderr)
+              "(images would shard across GPUs with no collective; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
     rc = importlib.import_module("rust-compress_amd")
@@ -94,6 +95,7 @@ def main():
     ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
     enc_ms, enc_n = ctx.profile_get("k_encode")
     fin_ms, fin_n = ctx.profile_get("k_ssim_finalize")
+    ref_ms, ref_n = ctx.profile_get("k_ssim_ref")
     ctx.profile(False)
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     progress(f"headline workload timed: {ms_per_step:.3f} ms/step")
@@ -126,6 +128,7 @@ def main():
                      "avg_launch_ms": round(ssim_ms / max(1, ssim_n), 4), "launches": ssim_n,
                      "bytes_per_px_per_candidate": SSIM_BYTES_PER_PX},
         "kernels_ms_per_step": {"k_ssim_scale": round(ssim_ms / args.steps, 3),
+                                "k_ssim_ref": round(ref_ms / args.steps, 3),
                                 "k_encode": round(enc_ms / args.steps, 3),
                                 "k_ssim_finalize": round(fin_ms / args.steps, 3)},
         "tile_policy": tune.as_dict(),

@@ -112,6 +112,7 @@ Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int s
 }
 
 PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
+    constexpr uint32_t kRefTiles = 2048;
     PairPlan P{};
     P.w = w;
     P.h = h;
@@ -130,6 +131,12 @@ PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
         P.tiles_x[s] = ssim_tiles_x(sw, int(P.nt[s]));
         P.tiles_y[s] = ssim_tiles_y(sh, int(P.th[s]));
         P.tiles[s] = P.tiles_x[s] * P.tiles_y[s];
+        // The reference-cache kernel's output does not depend on its tiling: one image's cache is built with shorter
+        // tiles, down to 16 rows, until the scale has kRefTiles workgroups (or the tiles are 16 rows high).
+        uint32_t tr = P.th[s];
+        while (tr > 16 && P.tiles_x[s] * ssim_tiles_y(sh, int(tr)) < kRefTiles) tr = std::max(16u, (tr / 2 + 1) & ~1u);
+        P.th_ref[s] = tr;
+        P.tiles_ref[s] = P.tiles_x[s] * ssim_tiles_y(sh, int(tr));
         P.nscales++;
         sw = (sw + 1) / 2;
         sh = (sh + 1) / 2;
@@ -166,34 +173,38 @@ void make_taps(double out[6]) {
     for (int t = 0; t <= 5; t++) out[t] = k[5 + t];
 }
 
-// Bytes one SSIMULACRA2 evaluation of a w x h pair needs in the arena (pyramid of both images, partials).
-size_t ssim_bytes(const PairPlan& P, bool cache_a) {
+// Bytes one SSIMULACRA2 evaluation of a w x h pair needs in the arena: the distorted image's pyramid, the reference
+// side (FP32: its pyramid and cache; FP64: its pyramid), the tile partials and the results.
+size_t ssim_bytes(const PairPlan& P, int precision) {
     size_t b = 0;
-    for (uint32_t s = 1; s < P.nscales; s++) {
-        const size_t n = size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float);
-        b += Arena::align(n) * 2;  // both images (the reference pyramid is kept even when cached)
-        (void)cache_a;
-    }
+    for (uint32_t s = 1; s < P.nscales; s++) b += Arena::align(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float));
+    if (precision == RCG_PREC_FP64)
+        b *= 2;
+    else
+        b += ref_state_bytes(P);
     for (uint32_t s = 0; s < P.nscales; s++) b += Arena::align(size_t(P.tiles[s]) * 18 * sizeof(double));
     b += 4096;
     return b;
 }
 
 struct SsimBuffers {
-    float* a_planes[6] = {};
+    RefState ref{};  // the reference side (FP64 uses ref.planes only)
     float* b_planes[6] = {};
     double* partials[6] = {};
     double* d_score = nullptr;
     double* d_stats = nullptr;
 };
 
-SsimBuffers take_ssim(Arena& A, const PairPlan& P) {
+SsimBuffers take_ssim(Arena& A, const PairPlan& P, int precision) {
     SsimBuffers B;
-    for (uint32_t s = 1; s < P.nscales; s++) {
-        const size_t n = size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float);
-        B.a_planes[s] = static_cast<float*>(A.take(n));
-        B.b_planes[s] = static_cast<float*>(A.take(n));
+    if (precision == RCG_PREC_FP64) {
+        for (uint32_t s = 1; s < P.nscales; s++)
+            B.ref.planes[s] = static_cast<float*>(A.take(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float)));
+    } else {
+        take_ref_state(A, P, B.ref);
     }
+    for (uint32_t s = 1; s < P.nscales; s++)
+        B.b_planes[s] = static_cast<float*>(A.take(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float)));
     for (uint32_t s = 0; s < P.nscales; s++)
         B.partials[s] = static_cast<double*>(A.take(size_t(P.tiles[s]) * 18 * sizeof(double)));
     B.d_score = static_cast<double*>(A.take(sizeof(double)));
@@ -202,27 +213,34 @@ SsimBuffers take_ssim(Arena& A, const PairPlan& P) {
 }
 
 // Stage the descriptors for every scale of one pair and its final pooling; launch_staged_ssim() enqueues them after
-// Desc::upload(). a0/b0 are the scale-0 sources. With a_cached, scales >= 1 of `a` read B.a_planes (built by an
-// earlier evaluation of the same reference) instead of writing them.
+// Desc::upload(). a0/b0 are the scale-0 sources (a0 as pixels or pyramid, never the cache). FP32 builds the reference
+// state (pyramid and cache) unless ref_ready and reads the reference side from the cache; FP64 computes both sides
+// and writes the reference pyramid unless ref_ready.
 struct StagedSsim {
     uint32_t kind_a0, kind_b0;
+    const RefJob* ref_jobs[6];
     const SsimJob* jobs[6];
-    const uint32_t* begins;
     const FinalDesc* fin;
     PairPlan P;
     int precision;
+    bool build_ref;
 };
 
 StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimSrc& b0, const SsimBuffers& B,
-                      bool a_cached, int precision) {
+                      bool ref_ready, int precision) {
+    const bool f64 = precision == RCG_PREC_FP64;
     StagedSsim st;
     st.P = P;
     st.precision = precision;
     st.kind_a0 = a0.kind;
     st.kind_b0 = b0.kind;
-    const uint32_t zero = 0;
-    st.begins = D.put(&zero, 1);
+    st.build_ref = !f64 && !ref_ready;
     for (uint32_t s = 0; s < P.nscales; s++) {
+        st.ref_jobs[s] = nullptr;
+        if (st.build_ref) {
+            const RefJob r = make_ref_job(P, s, a0, B.ref);
+            st.ref_jobs[s] = D.put(&r, 1);
+        }
         SsimJob j;
         std::memset(&j, 0, sizeof(j));
         j.w = P.sw[s];
@@ -230,18 +248,13 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
         j.th = P.th[s];
         j.tiles_x = P.tiles_x[s];
         j.tiles_y = P.tiles_y[s];
-        j.tile_begin = 0;
-        if (s == 0) {
-            j.a = a0;
-            j.b = b0;
-        } else {
-            j.a.kind = kSrcPlanes;
-            j.a.planes = B.a_planes[s];
-            j.b.kind = kSrcPlanes;
-            j.b.planes = B.b_planes[s];
-        }
+        if (!f64)
+            j.a = cache_src(B.ref.cache[s]);
+        else
+            j.a = s == 0 ? a0 : planes_src(B.ref.planes[s]);
+        j.b = s == 0 ? b0 : planes_src(B.b_planes[s]);
         const bool next = s + 1 < P.nscales;
-        j.a_next = (next && !a_cached) ? B.a_planes[s + 1] : nullptr;
+        j.a_next = (f64 && next && !ref_ready) ? B.ref.planes[s + 1] : nullptr;
         j.b_next = next ? B.b_planes[s + 1] : nullptr;
         j.partials = B.partials[s];
         st.jobs[s] = D.put(&j, 1);
@@ -261,9 +274,15 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
 }
 
 int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
-    for (uint32_t s = 0; s < st.P.nscales; s++)
-        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], st.begins, 1, st.P.tiles[s], int(st.P.nt[s]),
-                                  s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes), s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes)));
+    const bool f64 = st.precision == RCG_PREC_FP64;
+    for (uint32_t s = 0; s < st.P.nscales; s++) {
+        if (st.build_ref)
+            RCG_TRY(launch_ssim_ref(ctx, st.ref_jobs[s], nullptr, st.P.tiles_ref[s], int(st.P.nt[s]),
+                                    s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes)));
+        const uint32_t ka = !f64 ? uint32_t(kSrcRefCache) : (s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes));
+        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], nullptr, st.P.tiles[s], int(st.P.nt[s]), ka,
+                                  s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes)));
+    }
     return launch_finalize(ctx, st.fin, 1);
 }
 
@@ -394,13 +413,21 @@ SsimSrc rcg::planes_src(const float* planes) {
     return s;
 }
 
+SsimSrc rcg::cache_src(const float* cache) {
+    SsimSrc s;
+    std::memset(&s, 0, sizeof(s));
+    s.kind = kSrcRefCache;
+    s.planes = cache;
+    return s;
+}
+
 namespace {
 
 int ssim_pair_dev(rcg_ctx* ctx, Call& call, const uint8_t* d_a, const uint8_t* d_b, uint32_t w, uint32_t h,
                   uint32_t channels, int precision, double* score, double* per_scale, int* nscales) {
     const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
     const PairPlan P = plan_pair(w, h, T);
-    SsimBuffers B = take_ssim(ctx->arena, P);
+    SsimBuffers B = take_ssim(ctx->arena, P, precision);
     StagedSsim st = stage_ssim(call.D, P, bytes_src(d_a, channels), bytes_src(d_b, channels), B, false, precision);
     RCG_TRY(call.D.upload(ctx->stream));
     RCG_TRY(launch_staged_ssim(ctx, st));
@@ -683,7 +710,7 @@ int rcg_ssimulacra2_dev(rcg_ctx* ctx, const uint8_t* d_reference, const uint8_t*
     const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
     const PairPlan P = plan_pair(width, height, T);
     Call call(ctx);
-    RCG_TRY(call.begin(ssim_bytes(P, false)));
+    RCG_TRY(call.begin(ssim_bytes(P, precision)));
     return ssim_pair_dev(ctx, call, d_reference, d_distorted, width, height, channels, precision, score, per_scale,
                          nscales);
 }
@@ -701,7 +728,7 @@ int rcg_ssimulacra2(rcg_ctx* ctx, const uint8_t* reference, const uint8_t* disto
     const PairPlan P = plan_pair(width, height, T);
     const size_t n = size_t(width) * height * channels;
     Call call(ctx);
-    RCG_TRY(call.begin(ssim_bytes(P, false) + 2 * Arena::align(n)));
+    RCG_TRY(call.begin(ssim_bytes(P, precision) + 2 * Arena::align(n)));
     uint8_t* d_a = static_cast<uint8_t*>(ctx->arena.take(n));
     uint8_t* d_b = static_cast<uint8_t*>(ctx->arena.take(n));
     RCG_HIP(hipMemcpyAsync(d_a, reference, n, hipMemcpyHostToDevice, ctx->stream));
@@ -716,7 +743,7 @@ namespace {
 constexpr double kGuard = 0.05;  // |f32 score - target| below this is settled with the f64 scorer
 
 // Score one candidate quality on the device: encode + reconstruct planes, SSIMULACRA2 against the source.
-// The reference pyramid is built by the first evaluation and reused (a_cached) by the later ones.
+// The reference state of each precision is built by its first evaluation and reused by the later ones.
 int score_candidate(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_src, int quality,
                     uint8_t* const rec[3], const SsimBuffers& B32, const SsimBuffers& B64, bool& a32_ready,
                     bool& a64_ready, double target, double& score, bool& settled) {
@@ -763,8 +790,8 @@ int find_quality_impl(rcg_ctx* ctx, const uint8_t* pixels, bool host_src, uint32
     const PairPlan P32 = plan_pair(width, height, ctx->tune32);
     const PairPlan P64 = plan_pair(width, height, ctx->tune64);
     const size_t src_bytes = size_t(width) * height * channels;
-    const size_t need = Arena::align(src_bytes) + plane_bytes(Lmax) + coef_bytes(Lmax) + ssim_bytes(P32, false) +
-                        ssim_bytes(P64, false);
+    const size_t need = Arena::align(src_bytes) + plane_bytes(Lmax) + coef_bytes(Lmax) +
+                        ssim_bytes(P32, RCG_PREC_FP32) + ssim_bytes(P64, RCG_PREC_FP64);
     Call call(ctx);
     RCG_TRY(call.begin(need));
     const uint8_t* d_src = pixels;
@@ -778,8 +805,8 @@ int find_quality_impl(rcg_ctx* ctx, const uint8_t* pixels, bool host_src, uint32
     int16_t* d_co[3] = {nullptr, nullptr, nullptr};
     for (uint32_t c = 0; c < Lmax.ncomp; c++)
         d_co[c] = static_cast<int16_t*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(int16_t)));
-    const SsimBuffers B32 = take_ssim(ctx->arena, P32);
-    const SsimBuffers B64 = take_ssim(ctx->arena, P64);
+    const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
+    const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
     bool a32 = false, a64 = false;
 
     // jpeg/src/target.rs:37-55
@@ -898,21 +925,25 @@ extern "C" int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune) {
     return RCG_OK;
 }
 
-// Times k_ssim_scale (f32, f32 planes in, no pyramid out) on `candidates` pairs of S x S synthetic planes for every
-// tile height, for S = 4096 ... 64, and installs the fastest tile height of each size class.
+// Times k_ssim_scale (f32, the reference cache and f32 planes in, no pyramid out) on `candidates` pairs of S x S
+// synthetic planes for every tile height, for S = 4096 ... 64, and installs the fastest tile height of each size
+// class.
 extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
     if (!ctx || candidates < 1 || candidates > 64) return RCG_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     RCG_HIP(hipSetDevice(ctx->device));
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
-    const int ths[] = {8, 16, 32, 64, 128};
+    // Tile heights: powers of two and the steps between them (a launch's last round of workgroups is the less idle
+    // the closer its tile count comes to a multiple of the GPU's workgroup slots).
+    const int ths[] = {8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256};
     const int nsz = 7;
     const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
     ctx->arena.reset();
-    RCG_TRY(ctx->arena.ensure(2 * Arena::align(plane_bytes) + (size_t(4) << 20) + (size_t(1) << 20)));
+    RCG_TRY(ctx->arena.ensure(5 * Arena::align(plane_bytes) + (size_t(4) << 20) + (size_t(1) << 20)));
     RCG_TRY(ctx->pinned.ensure(size_t(1) << 20));
     float* a = static_cast<float*>(ctx->arena.take(plane_bytes));
     float* b = static_cast<float*>(ctx->arena.take(plane_bytes));
+    float* cache = static_cast<float*>(ctx->arena.take(3 * plane_bytes));
     double* partials = static_cast<double*>(ctx->arena.take(size_t(4) << 20));
     char* ddesc = static_cast<char*>(ctx->arena.take(size_t(1) << 20));
     // Synthetic linear-light planes: a smooth pattern and a perturbed copy (the timing does not depend on values).
@@ -943,8 +974,17 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
             const int nt = S <= T.narrow_w ? 128 : T.wide_nt;
             const uint32_t tx = ssim_tiles_x(S, nt), ty = ssim_tiles_y(S, th);
             if (size_t(tx) * ty * 18 * 8 * candidates > (size_t(4) << 20)) continue;
+            // The reference cache of the S x S plane `a` (the same for every tile height; rebuilt per size).
+            RefJob rj;
+            std::memset(&rj, 0, sizeof(rj));
+            rj.w = S;
+            rj.h = S;
+            rj.th = uint32_t(th);
+            rj.tiles_x = tx;
+            rj.tiles_y = ty;
+            rj.a = planes_src(a);
+            rj.cache = cache;
             std::vector<SsimJob> jobs(candidates);
-            std::vector<uint32_t> begins(candidates);
             for (int c = 0; c < candidates; c++) {
                 SsimJob& j = jobs[c];
                 std::memset(&j, 0, sizeof(j));
@@ -953,25 +993,27 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
                 j.th = uint32_t(th);
                 j.tiles_x = tx;
                 j.tiles_y = ty;
-                j.tile_begin = uint32_t(c) * tx * ty;
-                j.a = planes_src(a);
+                j.a = cache_src(cache);
                 j.b = planes_src(b);
                 j.partials = partials + size_t(c) * tx * ty * 18;
-                begins[c] = j.tile_begin;
             }
+            // The candidates of a tile side by side on one XCD, as the scoring passes launch them.
+            const std::vector<Unit> units = xcd_units({UnitGroup{0, uint32_t(candidates), tx * ty}});
             Desc D;
             D.host = ctx->pinned.base;
             D.dev = ddesc;
             D.cap = size_t(1) << 20;
             const SsimJob* dj = D.put(jobs.data(), jobs.size());
-            const uint32_t* db = D.put(begins.data(), begins.size());
+            const Unit* du = D.put(units.data(), units.size());
+            const RefJob* drj = D.put(&rj, 1);
             RCG_TRY(D.upload(ctx->stream));
-            const uint32_t tiles = tx * ty * uint32_t(candidates);
-            RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, db, candidates, tiles, nt, kSrcPlanes, kSrcPlanes));
+            RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes));
+            const uint32_t nu = uint32_t(units.size());
+            RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
             const int reps = S >= 2048 ? 3 : 10;
             RCG_HIP(hipEventRecord(e0, ctx->stream));
             for (int r = 0; r < reps; r++)
-                RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, db, candidates, tiles, nt, kSrcPlanes, kSrcPlanes));
+                RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
             RCG_HIP(hipEventRecord(e1, ctx->stream));
             RCG_HIP(hipEventSynchronize(e1));
             float ms = 0;

@@ -4,9 +4,10 @@
 // batch is scored by a fixed sequence of launches whose workgroups are spread over every image at once:
 //   k_encode        one launch; per image one job per distinct sampling layout, whose candidates share the
 //                   colour conversion and FDCT and differ only in quantisation + IDCT;
-//   k_ssim_scale    one launch per scale (6), covering every (image, candidate) pair that has that scale; the
-//                   first candidate of an image also writes the reference pyramid, which every candidate of the
-//                   image then reads;
+//   k_ssim_ref      per scale, one launch over the images whose reference state is not built yet: the reference
+//                   image's cache and next pyramid level;
+//   k_ssim_scale    per scale, one launch covering every (image, candidate) pair that has that scale, reading the
+//                   reference side from the cache;
 //   k_ssim_finalize one launch pooling every pair.
 // Pairs whose FP32 score falls within the guard band of the image's target are re-scored with the FP64 kernel
 // (reading the already-built pyramids), then the selection of jpeg/src/target.rs:37-55 is replayed per image.
@@ -89,9 +90,16 @@ extern "C" int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_m
 
 namespace rcg {
 
-size_t ref_pyramid_bytes(const PairPlan& P) {
+// Linear-light planes of scales 1.. of one image.
+static size_t pyramid_bytes(const PairPlan& P) {
     size_t b = 0;
     for (uint32_t s = 1; s < P.nscales; s++) b += Arena::align(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float));
+    return b;
+}
+
+size_t ref_state_bytes(const PairPlan& P) {
+    size_t b = pyramid_bytes(P);
+    for (uint32_t s = 0; s < P.nscales; s++) b += Arena::align(size_t(P.sw[s]) * P.sh[s] * 9 * sizeof(float));
     return b;
 }
 
@@ -101,10 +109,47 @@ static size_t partial_bytes(const PairPlan& P) {
     return b;
 }
 
-void take_ref_pyramid(Arena& A, const PairPlan& P, float* planes[6]) {
-    for (int s = 0; s < 6; s++) planes[s] = nullptr;
+void take_ref_state(Arena& A, const PairPlan& P, RefState& R) {
+    for (int s = 0; s < 6; s++) R.planes[s] = R.cache[s] = nullptr;
     for (uint32_t s = 1; s < P.nscales; s++)
-        planes[s] = static_cast<float*>(A.take(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float)));
+        R.planes[s] = static_cast<float*>(A.take(size_t(P.sw[s]) * P.sh[s] * 3 * sizeof(float)));
+    for (uint32_t s = 0; s < P.nscales; s++)
+        R.cache[s] = static_cast<float*>(A.take(size_t(P.sw[s]) * P.sh[s] * 9 * sizeof(float)));
+}
+
+RefJob make_ref_job(const PairPlan& P, uint32_t s, const SsimSrc& a0, const RefState& R) {
+    RefJob j;
+    std::memset(&j, 0, sizeof(j));
+    j.w = P.sw[s];
+    j.h = P.sh[s];
+    j.th = P.th_ref[s];
+    j.tiles_x = P.tiles_x[s];
+    j.tiles_y = ssim_tiles_y(P.sh[s], int(P.th_ref[s]));
+    j.a = s == 0 ? a0 : planes_src(R.planes[s]);
+    j.a_next = s + 1 < P.nscales ? R.planes[s + 1] : nullptr;
+    j.cache = R.cache[s];
+    return j;
+}
+
+std::vector<Unit> xcd_units(const std::vector<UnitGroup>& groups) {
+    std::vector<Unit> lists[kXcds];
+    size_t total = 0;
+    for (const UnitGroup& g : groups) total += size_t(g.njobs) * g.tiles;
+    for (auto& l : lists) l.reserve(total / kXcds + 64);
+    for (const UnitGroup& g : groups) {
+        for (uint32_t t = 0; t < g.tiles; t++) {
+            int x = 0;
+            for (int i = 1; i < kXcds; i++)
+                if (lists[i].size() < lists[x].size()) x = i;
+            for (uint32_t c = 0; c < g.njobs; c++) lists[x].push_back(Unit{g.job0 + c, t});
+        }
+    }
+    size_t len = 0;
+    for (auto& l : lists) len = std::max(len, l.size());
+    std::vector<Unit> out(len * kXcds, Unit{kIdleJob, 0});
+    for (int x = 0; x < kXcds; x++)
+        for (size_t p = 0; p < lists[x].size(); p++) out[p * kXcds + x] = lists[x][p];
+    return out;
 }
 
 namespace {
@@ -125,23 +170,34 @@ struct PairRef {
     int item, cand;
 };
 
-size_t desc_bytes(const Item* items, int n) {
-    size_t d = 1 << 16;
+size_t desc_bytes(rcg_ctx* ctx, const Item* items, int n) {
+    // Fixed part: alignment of every put and the idle units that pad each launch's unit table (at most
+    // kXcds * RCG_MAX_CAND per launch, 24 launches per pass).
+    size_t d = (1 << 16) + 2 * 24 * kXcds * RCG_MAX_CAND * sizeof(Unit);
     for (int i = 0; i < n; i++) {
-        d += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * items[i].ncand + 64 * (RCG_MAX_CAND + 2);
+        const Item& it = items[i];
+        d += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * it.ncand + 64 * (RCG_MAX_CAND + 2);
         // FP32 pass plus, after it, an FP64 settling pass of up to every pair.
-        d += 2 * ((sizeof(SsimJob) + 64) * 6 * items[i].ncand + sizeof(FinalDesc) * items[i].ncand + 256);
+        d += 2 * ((sizeof(SsimJob) + 64) * 6 * it.ncand + sizeof(FinalDesc) * it.ncand + 256);
+        d += (sizeof(RefJob) + 64) * 6;
         d += 4 * 6 * 128;
+        const PairPlan P32 = plan_pair(it.w, it.h, ctx->tune32);
+        const PairPlan P64 = plan_pair(it.w, it.h, ctx->tune64);
+        size_t units = 0;
+        for (uint32_t s = 0; s < P32.nscales; s++) units += size_t(P32.tiles[s]) * it.ncand + P32.tiles_ref[s];
+        for (uint32_t s = 0; s < P64.nscales; s++) units += size_t(P64.tiles[s]) * it.ncand;
+        d += units * sizeof(Unit);
     }
     return d;
 }
 
-// Staged SSIMULACRA2 launches of a set of pairs: one launch per (scale, thread count) and one pooling launch.
+// Staged SSIMULACRA2 launches of a set of pairs: per (scale, thread count) one reference-cache launch (FP32, for the
+// items whose reference state is not built) and one scale launch; then one pooling launch.
 struct StagedLaunch {
-    const SsimJob* jobs;
-    const uint32_t* begins;
-    int n;
-    uint32_t tiles;
+    bool ref;
+    const void* jobs;
+    const Unit* units;
+    uint32_t nunits;
     int nt;
     uint32_t ka, kb;
 };
@@ -152,21 +208,49 @@ struct StagedPairs {
     int precision = RCG_PREC_FP32;
 };
 
-// Stages the descriptors to score every pair in `pairs` at `precision` into D. FP32 writes the pyramids (the
-// reference's by each item's first candidate unless already built); FP64 reads both from the FP32 pass.
-StagedPairs stage_pairs(Desc& D, const Item* items, std::vector<ItemBufs>& bufs, const std::vector<PairRef>& pairs,
-                        int precision, double* d_scores) {
+// Stages the descriptors to score every pair in `pairs` at `precision` into D. FP32 first builds the reference state
+// of every item that lacks it, then writes each candidate's pyramid; FP64 reads both pyramids from the FP32 pass.
+// The pairs of one item must be consecutive in `pairs` (they form one unit group per scale).
+StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<ItemBufs>& bufs,
+                        const std::vector<PairRef>& pairs, int precision, double* d_scores) {
     const bool f64 = precision == RCG_PREC_FP64;
     StagedPairs out;
     out.precision = precision;
     out.npairs = int(pairs.size());
     std::vector<FinalDesc> fins(pairs.size());
     for (size_t i = 0; i < pairs.size(); i++) std::memset(&fins[i], 0, sizeof(FinalDesc));
+    // Items whose reference state this pass builds: those with a pair here and no state yet.
+    std::vector<char> build(nitems, 0);
+    if (!f64)
+        for (const PairRef& pr : pairs) build[pr.item] = !items[pr.item].ref_ready;
     for (int s = 0; s < 6; s++) {
         for (int nt : {256, 128}) {
+            if (!f64) {
+                std::vector<RefJob> rj;
+                std::vector<UnitGroup> groups;
+                for (int i = 0; i < nitems; i++) {
+                    if (!build[i]) continue;
+                    const Item& it = items[i];
+                    const PairPlan& P = bufs[i].P32;
+                    if (uint32_t(s) >= P.nscales || int(P.nt[s]) != nt) continue;
+                    groups.push_back(UnitGroup{uint32_t(rj.size()), 1, P.tiles_ref[s]});
+                    rj.push_back(make_ref_job(P, uint32_t(s), bytes_src(it.d_src, it.ch), it.ref));
+                }
+                if (!rj.empty()) {
+                    const std::vector<Unit> units = xcd_units(groups);
+                    StagedLaunch g;
+                    g.ref = true;
+                    g.jobs = D.put(rj.data(), rj.size());
+                    g.units = D.put(units.data(), units.size());
+                    g.nunits = uint32_t(units.size());
+                    g.nt = nt;
+                    g.ka = s == 0 ? uint32_t(kSrcBytes) : uint32_t(kSrcPlanes);
+                    g.kb = 0;
+                    out.scales.push_back(g);
+                }
+            }
             std::vector<SsimJob> sj;
-            std::vector<uint32_t> begins;
-            uint32_t total = 0;
+            std::vector<UnitGroup> groups;
             for (size_t i = 0; i < pairs.size(); i++) {
                 const Item& it = items[pairs[i].item];
                 ItemBufs& b = bufs[pairs[i].item];
@@ -180,37 +264,37 @@ StagedPairs stage_pairs(Desc& D, const Item* items, std::vector<ItemBufs>& bufs,
                 j.th = P.th[s];
                 j.tiles_x = P.tiles_x[s];
                 j.tiles_y = P.tiles_y[s];
-                j.tile_begin = total;
-                if (s == 0) {
-                    j.a = bytes_src(it.d_src, it.ch);
-                    j.b = recon_src(b.L[c], b.rec[c]);
-                } else {
-                    j.a = planes_src(it.ref_planes[s]);
-                    j.b = planes_src(b.dist_planes[c][s]);
-                }
-                const bool next = uint32_t(s + 1) < P.nscales;
-                if (!f64 && next) {
-                    j.b_next = b.dist_planes[c][s + 1];
-                    if (c == 0 && !it.ref_ready) j.a_next = it.ref_planes[s + 1];
-                }
+                if (!f64)
+                    j.a = cache_src(it.ref.cache[s]);
+                else
+                    j.a = s == 0 ? bytes_src(it.d_src, it.ch) : planes_src(it.ref.planes[s]);
+                j.b = s == 0 ? recon_src(b.L[c], b.rec[c]) : planes_src(b.dist_planes[c][s]);
+                if (!f64 && uint32_t(s + 1) < P.nscales) j.b_next = b.dist_planes[c][s + 1];
                 j.partials = f64 ? b.part64[c][s] : b.part32[c][s];
                 FinalDesc& f = fins[i];
                 f.nscales = P.nscales;
                 f.ntiles[s] = P.tiles[s];
                 f.partials[s] = j.partials;
                 f.count[s] = double(size_t(P.sw[s]) * P.sh[s]);
+                // The FP32 candidates of one item share the item's cache: one group, so they run side by side on one
+                // XCD. FP64 jobs read no shared data and stay single.
+                const bool same_item = !f64 && !groups.empty() && i > 0 && pairs[i - 1].item == pairs[i].item &&
+                                       groups.back().job0 + groups.back().njobs == sj.size();
+                if (same_item)
+                    groups.back().njobs++;
+                else
+                    groups.push_back(UnitGroup{uint32_t(sj.size()), 1, P.tiles[s]});
                 sj.push_back(j);
-                begins.push_back(total);
-                total += P.tiles[s];
             }
             if (!sj.empty()) {
+                const std::vector<Unit> units = xcd_units(groups);
                 StagedLaunch g;
+                g.ref = false;
                 g.jobs = D.put(sj.data(), sj.size());
-                g.begins = D.put(begins.data(), begins.size());
-                g.n = int(sj.size());
-                g.tiles = total;
+                g.units = D.put(units.data(), units.size());
+                g.nunits = uint32_t(units.size());
                 g.nt = nt;
-                g.ka = s == 0 ? uint32_t(kSrcBytes) : uint32_t(kSrcPlanes);
+                g.ka = !f64 ? uint32_t(kSrcRefCache) : (s == 0 ? uint32_t(kSrcBytes) : uint32_t(kSrcPlanes));
                 g.kb = s == 0 ? uint32_t(kSrcRecon) : uint32_t(kSrcPlanes);
                 out.scales.push_back(g);
             }
@@ -225,8 +309,13 @@ StagedPairs stage_pairs(Desc& D, const Item* items, std::vector<ItemBufs>& bufs,
 }
 
 int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp) {
-    for (const StagedLaunch& g : sp.scales)
-        RCG_TRY(launch_ssim_scale(ctx, sp.precision, g.jobs, g.begins, g.n, g.tiles, g.nt, g.ka, g.kb));
+    for (const StagedLaunch& g : sp.scales) {
+        if (g.ref)
+            RCG_TRY(launch_ssim_ref(ctx, static_cast<const RefJob*>(g.jobs), g.units, g.nunits, g.nt, g.ka));
+        else
+            RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
+                                      g.ka, g.kb));
+    }
     return launch_finalize(ctx, sp.fin, sp.npairs);
 }
 
@@ -241,11 +330,11 @@ size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n) {
         for (uint32_t c = 0; c < it.ncand; c++) {
             const Layout L = make_layout(it.w, it.h, it.ch, it.quality[c], it.subsampling[c]);
             for (uint32_t k = 0; k < L.ncomp; k++) need += Arena::align(L.plane_bytes(k));
-            need += ref_pyramid_bytes(P32) + partial_bytes(P32) + partial_bytes(P64);
+            need += pyramid_bytes(P32) + partial_bytes(P32) + partial_bytes(P64);
         }
         npairs += it.ncand;
     }
-    return need + Arena::align(desc_bytes(items, n)) + 2 * Arena::align(npairs * sizeof(double) + 8) + (1 << 12);
+    return need + Arena::align(desc_bytes(ctx, items, n)) + 2 * Arena::align(npairs * sizeof(double) + 8) + (1 << 12);
 }
 
 // The device state of one scoring pass: per-item buffers, staged descriptors (already uploaded) and the FP32
@@ -270,7 +359,7 @@ struct ScoreState {
 int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, ScoreState& S) {
     size_t npairs = 0;
     for (int i = 0; i < n; i++) npairs += items[i].ncand;
-    const size_t dbytes = desc_bytes(items, n);
+    const size_t dbytes = desc_bytes(ctx, items, n);
     RCG_TRY(pin.ensure(Arena::align(dbytes) + Arena::align(npairs * sizeof(double) + 8)));
     S.D.host = pin.base;
     S.D.dev = static_cast<char*>(A.take(dbytes));
@@ -348,7 +437,7 @@ int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, Score
         S.d_ej = S.D.put(ej.data(), ej.size());
         S.d_eb = S.D.put(ebegins.data(), ebegins.size());
     }
-    S.fp32 = stage_pairs(S.D, items, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32);
+    S.fp32 = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32);
     S.fp32_desc_end = S.D.off;
     return S.D.upload(ctx->stream);
 }
@@ -377,7 +466,7 @@ int finish_items(rcg_ctx* ctx, ScoreState& S, Item* items, int n) {
     if (settle.empty()) return RCG_OK;
     // FP64 descriptors go after the FP32 ones, which stay valid for later replays.
     S.D.off = S.fp32_desc_end;
-    const StagedPairs f64 = stage_pairs(S.D, items, S.bufs, settle, RCG_PREC_FP64, S.d_s64);
+    const StagedPairs f64 = stage_pairs(S.D, items, n, S.bufs, settle, RCG_PREC_FP64, S.d_s64);
     RCG_TRY(S.D.upload(ctx->stream));
     RCG_TRY(launch_pairs(ctx, f64));
     RCG_HIP(hipMemcpyAsync(S.h_scores, S.d_s64, settle.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -469,7 +558,7 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
         items.push_back(it);
         job_of.push_back(i);
         if (host_src) own += Arena::align(size_t(j.width) * j.height * j.channels);
-        own += ref_pyramid_bytes(plan_pair(j.width, j.height, ctx->tune32));
+        own += ref_state_bytes(plan_pair(j.width, j.height, ctx->tune32));
     }
     const size_t need = own + score_items_bytes(ctx, items.data(), int(items.size()));
     ctx->arena.reset();
@@ -482,7 +571,7 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
             RCG_HIP(hipMemcpyAsync(d, jobs[job_of[k]].pixels, n, hipMemcpyHostToDevice, ctx->stream));
             it.d_src = d;
         }
-        take_ref_pyramid(ctx->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref_planes);
+        take_ref_state(ctx->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref);
         it.ref_ready = false;
     }
     RCG_TRY(score_items(ctx, items.data(), int(items.size())));
@@ -570,7 +659,7 @@ extern "C" int rcg_batch_plan_create(rcg_ctx* ctx, const rcg_batch_job* jobs, in
         it.target = j.target;
         p->items.push_back(it);
         p->job_of.push_back(i);
-        own += ref_pyramid_bytes(plan_pair(j.width, j.height, ctx->tune32));
+        own += ref_state_bytes(plan_pair(j.width, j.height, ctx->tune32));
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     int rc = RCG_OK;
@@ -581,7 +670,7 @@ extern "C" int rcg_batch_plan_create(rcg_ctx* ctx, const rcg_batch_job* jobs, in
     if (rc == RCG_OK) rc = p->arena.ensure(own + score_items_bytes(ctx, p->items.data(), int(p->items.size())));
     if (rc == RCG_OK) {
         for (Item& it : p->items) {
-            take_ref_pyramid(p->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref_planes);
+            take_ref_state(p->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref);
             it.ref_ready = false;
         }
         rc = prepare_items(ctx, p->arena, p->pinned, p->items.data(), int(p->items.size()), p->S);

@@ -132,8 +132,20 @@ struct IdctPlane {
 int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
                           uint8_t* out);
-int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
-                      uint32_t total_tiles, int nt, uint32_t kind_a, uint32_t kind_b);
+int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
+                      uint32_t kind_a, uint32_t kind_b);
+int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
+                    uint32_t kind_a);
+
+// Jobs job0 .. job0+njobs-1 of a launch that share one tiling of `tiles` tiles (the candidates of one image at one
+// scale, or a single job).
+struct UnitGroup {
+    uint32_t job0, njobs, tiles;
+};
+// The unit table of a launch (see Unit): each tile of a group becomes a chunk of njobs consecutive units on one XCD,
+// and chunks go to the XCD with the fewest units so far (round-robin when the groups are alike). XCDs left shorter
+// than the longest are padded with idle units. A launch of single-job groups gets units in plain tile order.
+std::vector<Unit> xcd_units(const std::vector<UnitGroup>& groups);
 struct FinalDesc {
     uint32_t nscales;
     uint32_t ntiles[6];
@@ -155,6 +167,7 @@ struct PairPlan {
     uint32_t sw[6], sh[6];
     uint32_t nt[6], th[6];
     uint32_t tiles[6], tiles_x[6], tiles_y[6];
+    uint32_t th_ref[6], tiles_ref[6];  // tiling of the reference-cache kernel (same columns, its own rows)
 };
 PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& tune);
 
@@ -191,10 +204,23 @@ struct Desc {
 SsimSrc recon_src(const Layout& L, uint8_t* const rec[3]);
 SsimSrc bytes_src(const uint8_t* d, uint32_t channels);
 SsimSrc planes_src(const float* planes);
+SsimSrc cache_src(const float* cache);
 int check_image(uint32_t w, uint32_t h, uint32_t channels);
 
-// One image of a scoring pass: its resident source, its reference pyramid (scales 1..5, built by the pass when
-// ref_ready is false) and the candidates to score. Scores come back in scores[]; settled[c] marks FP64 ones.
+// The reference side of an image's f32 scoring, built once per image and read by every candidate: its linear-light
+// pyramid (planes[s], [3][h_s][w_s] for scales 1..; the f64 kernel reads it too) and its per-scale cache (cache[s],
+// [9][h_s][w_s], written by k_ssim_ref).
+struct RefState {
+    float* planes[6];
+    float* cache[6];
+};
+size_t ref_state_bytes(const PairPlan& P);
+void take_ref_state(Arena& A, const PairPlan& P, RefState& R);
+// The reference-cache job of scale s; a0 is the scale-0 source of the reference image.
+RefJob make_ref_job(const PairPlan& P, uint32_t s, const SsimSrc& a0, const RefState& R);
+
+// One image of a scoring pass: its resident source, its reference state (built by the pass when ref_ready is false)
+// and the candidates to score. Scores come back in scores[]; settled[c] marks FP64 ones.
 struct Item {
     const uint8_t* d_src;
     uint32_t w, h, ch;
@@ -202,13 +228,11 @@ struct Item {
     int quality[RCG_MAX_CAND];
     int subsampling[RCG_MAX_CAND];
     double target;
-    float* ref_planes[6];
+    RefState ref;
     bool ref_ready;
     double scores[RCG_MAX_CAND];
     int settled[RCG_MAX_CAND];
 };
-size_t ref_pyramid_bytes(const PairPlan& P);
-void take_ref_pyramid(Arena& A, const PairPlan& P, float* planes[6]);
 size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n);
 int score_items(rcg_ctx* ctx, Item* items, int n);
 int select_candidate(const int* quality, const double* scores, int n, double target, int* sweep);

@@ -1,9 +1,10 @@
 // Work descriptors shared by the host launchers and the kernels.
 //
-// A launch processes a list of jobs (one image, or one image at one scale) whose tiles are numbered
-// consecutively: job j owns workgroups [tile_begin, tile_begin + tiles). A workgroup finds its job by a binary
-// search over tile_begin, so one launch can cover many images of uneven size (the batched search) or a single one
-// (the plain drop-in), with the same kernel.
+// A launch processes a list of jobs (one image, or one image at one scale), so one launch can cover many images of
+// uneven size (the batched search) or a single one (the plain drop-in), with the same kernel. The encode kernel
+// numbers the tiles of its jobs consecutively (job j owns workgroups [tile_begin, tile_begin + tiles), found by a
+// binary search). The SSIMULACRA2 kernels read their (job, tile) from a per-launch unit table instead, which the
+// host orders for the L2 caches: see Unit.
 #pragma once
 #include <stdint.h>
 
@@ -42,7 +43,9 @@ struct PlaneView {
 };
 
 // Source of linear-light RGB for one image at one SSIMULACRA2 scale.
-enum SrcKind : uint32_t { kSrcBytes = 0, kSrcRecon = 1, kSrcPlanes = 2 };
+// kSrcRefCache: the reference's per-scale cache (planes [9][h][w]: shifted XYB, blurred XYB, blurred XYB^2), read
+// by candidate kernels in place of the reference's pixels.
+enum SrcKind : uint32_t { kSrcBytes = 0, kSrcRecon = 1, kSrcPlanes = 2, kSrcRefCache = 3 };
 
 struct SsimSrc {
     uint32_t kind;
@@ -58,11 +61,30 @@ struct SsimJob {
     uint32_t w, h;          // size at this scale
     uint32_t th;            // output rows per tile (even)
     uint32_t tiles_x, tiles_y;
-    uint32_t tile_begin;
     SsimSrc a, b;           // a = reference, b = distorted
     float* a_next;          // next scale's planes [3][ceil(h/2)][ceil(w/2)] or null
     float* b_next;
     double* partials;       // [tiles_x*tiles_y][18] per-tile sums, written by this launch
 };
+
+// One image at one scale for the reference-cache kernel: its pixels in, its cache (and next pyramid level) out.
+struct RefJob {
+    uint32_t w, h, th;
+    uint32_t tiles_x, tiles_y;
+    SsimSrc a;
+    float* a_next;          // next scale's planes [3][ceil(h/2)][ceil(w/2)] or null
+    float* cache;           // [9][h][w]
+};
+
+// Workgroup b of an SSIMULACRA2 launch processes tile units[b].tile of job units[b].job (job == kIdleJob: none); a
+// launch of one job may pass no table, and then workgroup b processes tile b.
+// Workgroups are dispatched round-robin over the GPU's 8 XCDs (b mod 8), each with its own L2. The host places the
+// candidates of one image tile in consecutive slots of one XCD, so the reference cache rows they all read come from
+// HBM once and from that XCD's L2 for the others.
+struct Unit {
+    uint32_t job, tile;
+};
+constexpr uint32_t kIdleJob = 0xFFFFFFFFu;
+constexpr int kXcds = 8;
 
 }  // namespace rcg

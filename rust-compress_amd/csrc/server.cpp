@@ -94,7 +94,7 @@ struct Request {
     double submit_s = 0, admit_s = 0;
     size_t footprint = 0;
     uint8_t* d_src = nullptr;
-    float* ref_planes[6] = {};
+    RefState ref{};
     void* d_ref = nullptr;
     bool ref_ready = false;
     Search search;
@@ -120,7 +120,7 @@ struct rcg_server {
 
     size_t footprint_of(const rcg_request& r) const {
         const PairPlan P = plan_pair(r.width, r.height, ctx->tune32);
-        size_t b = Arena::align(size_t(r.width) * r.height * r.channels) + ref_pyramid_bytes(P);
+        size_t b = Arena::align(size_t(r.width) * r.height * r.channels) + ref_state_bytes(P);
         Item it;
         std::memset(&it, 0, sizeof(it));
         it.w = r.width;
@@ -176,14 +176,12 @@ struct rcg_server {
         RCG_HIP(hipMalloc(reinterpret_cast<void**>(&r->d_src), n));
         RCG_HIP(hipMemcpyAsync(r->d_src, r->pixels.data(), n, hipMemcpyHostToDevice, ctx->stream));
         const PairPlan P = plan_pair(r->req.width, r->req.height, ctx->tune32);
-        const size_t pb = ref_pyramid_bytes(P);
-        if (pb) {
-            RCG_HIP(hipMalloc(&r->d_ref, pb));
-            Arena tmp;
-            tmp.base = static_cast<char*>(r->d_ref);
-            tmp.cap = pb;
-            take_ref_pyramid(tmp, P, r->ref_planes);
-        }
+        const size_t pb = ref_state_bytes(P);
+        RCG_HIP(hipMalloc(&r->d_ref, pb));
+        Arena tmp;
+        tmp.base = static_cast<char*>(r->d_ref);
+        tmp.cap = pb;
+        take_ref_state(tmp, P, r->ref);
         RCG_HIP(hipStreamSynchronize(ctx->stream));
         std::vector<uint8_t>().swap(r->pixels);
         return RCG_OK;
@@ -230,7 +228,7 @@ struct rcg_server {
                     it.subsampling[c] = r->req.subsampling;
                 }
                 it.target = r->req.target;
-                for (int s = 0; s < 6; s++) it.ref_planes[s] = r->ref_planes[s];
+                it.ref = r->ref;
                 it.ref_ready = r->ref_ready;
                 item_of[i] = int(items.size());
                 items.push_back(it);

@@ -2,22 +2,25 @@
 //
 // k_ssim_scale<T, NT> computes one scale of one or many image pairs. A workgroup of NT threads owns a tile of
 // OW = NT-12 output columns and TH output rows. Thread t owns column x0-6+t and walks down rows y0-5 .. y0+TH+4:
-//   1. loads the linear-light RGB of both images at its column (from RGB8 through the sRGB table, from the
+//   1. loads the linear-light RGB of the distorted image at its column (from RGB8 through the sRGB table, from the
 //      reconstructed JPEG sample planes with fused upsampling and colour conversion, or from the previous scale's
 //      f32 planes), rows above/below the image mirrored (ssimulacra2.rs:385-397);
-//   2. converts to XYB (ssimulacra2.rs:251-258) and keeps the last 11 rows in a register ring;
-//   3. blurs the 15 quantities (mu_a, mu_b, a^2, b^2, ab for X, Y, B) down its column from the ring, then exchanges
-//      one row of them through LDS and blurs across the row; columns outside the image hold zeros, which is the
-//      reference's horizontal border rule (ssimulacra2.rs:34-40, 278-290);
+//   2. converts to XYB (ssimulacra2.rs:251-258) and keeps the last 11 rows in a register ring, beside the ring of
+//      the reference image's XYB;
+//   3. blurs the per-channel quantities down its column from the rings, then exchanges one row of them through LDS
+//      and blurs across the row; columns outside the image hold zeros, which is the reference's horizontal border
+//      rule (ssimulacra2.rs:34-40, 278-290);
 //   4. evaluates the SSIM, artifact and detail maps and accumulates their sums and 4th-power sums;
 //   5. writes the 2x2 linear-light mean for the next scale (ssimulacra2.rs:225-247, summed in f64 in the
 //      reference's order, so the pyramid is bit-identical to the reference's f32 planes).
 // Each tile writes its 18 sums as f64; k_ssim_finalize reduces them in a fixed order and pools the scales.
 //
-// T = float is the precision form (f32 XYB shifted by per-channel constants, blurs and maps; f32 sums within a tile
-// column, f64 above that);
-// T = double follows the reference's f64 arithmetic and is used to settle candidates whose f32 score lies within
-// the guard band of a search target.
+// T = float is the precision form: f32 XYB shifted by per-channel constants, blurs and maps, f32 sums within a tile
+// column and f64 above that. Its reference side (XYB, blur and variance of the reference image) comes from a
+// per-image cache that k_ssim_ref writes once per scale, so a candidate costs one colour transform and three blurs
+// per channel instead of two and five.
+// T = double follows the reference's f64 arithmetic on both images and is used to settle candidates whose f32 score
+// lies within the guard band of a search target.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -37,13 +40,19 @@ __constant__ double c_weights[108];
 template <typename T>
 struct Xyb;
 
+// Multiply-add with one rounding. The blurs and the colour transform use it explicitly (the file is compiled without
+// FP contraction), so the reference-cache kernel and the candidate kernel round every shared quantity alike.
+__device__ __forceinline__ float mad(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return fma(a, b, c); }
+
 // Quantities blurred per channel and the SSIM map computed from them (core/src/ssimulacra2.rs:364-382, 424-433).
 //
 // f64 follows the reference: mu_a, mu_b, blur(a^2), blur(b^2), blur(ab) and
 //   d = 1 - (1 - (mu_a - mu_b)^2) * (2*sigma_ab + C2) / (sigma_aa + sigma_bb + C2).
-// f32 blurs four quantities, mu_a, mu_b, Q = blur(a^2 + b^2) and R = blur((a - b)^2). With dmu = mu_a - mu_b,
-// V = R - dmu^2 (= sigma_aa + sigma_bb - 2*sigma_ab, the local variance of a - b) and
-// D = Q - mu_a^2 - mu_b^2 + C2 (= sigma_aa + sigma_bb + C2), the same map is
+// f32 takes the reference image's side from its per-scale cache (a, mu_a and sigma_aa = blur(a^2) - mu_a^2, built
+// once per image by k_ssim_ref) and blurs three quantities of the candidate: mu_b, blur(b^2) and R = blur((a-b)^2).
+// With dmu = mu_a - mu_b, V = R - dmu^2 (= sigma_aa + sigma_bb - 2*sigma_ab, the local variance of a - b) and
+// D = sigma_aa + (blur(b^2) - mu_b^2) + C2 (= sigma_aa + sigma_bb + C2), the same map is
 //   d = (V + dmu^2 * (D - V)) / D,
 // since 2*sigma_ab + C2 = D - V. V is formed from differences of the two images, not by cancelling large blurred
 // products, so the f32 form keeps its precision where the images are close, and V = 0 exactly for identical ones.
@@ -52,6 +61,7 @@ struct Form;
 template <>
 struct Form<double> {
     static constexpr int NQ = 5;
+    static constexpr int MU_B = 1;  // index of mu_b among the blurred quantities
     static __device__ __forceinline__ void centre(double k0, double a0, double b0, double q[5]) {
         q[0] = k0 * a0;
         q[1] = k0 * b0;
@@ -60,11 +70,11 @@ struct Form<double> {
         q[4] = k0 * (a0 * b0);
     }
     static __device__ __forceinline__ void pair(double kt, double au, double ad, double bu, double bd, double q[5]) {
-        q[0] += kt * (au + ad);
-        q[1] += kt * (bu + bd);
-        q[2] += kt * (au * au + ad * ad);
-        q[3] += kt * (bu * bu + bd * bd);
-        q[4] += kt * (au * bu + ad * bd);
+        q[0] = mad(kt, au + ad, q[0]);
+        q[1] = mad(kt, bu + bd, q[1]);
+        q[2] = mad(kt, mad(au, au, ad * ad), q[2]);
+        q[3] = mad(kt, mad(bu, bu, bd * bd), q[3]);
+        q[4] = mad(kt, mad(au, bu, ad * bd), q[4]);
     }
     static __device__ __forceinline__ double ssim(const double v[5]) {
         const double m1 = v[0], m2 = v[1], s11 = v[2], s22 = v[3], s12 = v[4];
@@ -77,30 +87,26 @@ struct Form<double> {
 };
 template <>
 struct Form<float> {
-    static constexpr int NQ = 4;
-    static __device__ __forceinline__ void centre(float k0, float a0, float b0, float q[4]) {
-#pragma clang fp contract(fast)
-        const float d0 = a0 - b0;
-        q[0] = k0 * a0;
-        q[1] = k0 * b0;
-        q[2] = k0 * (a0 * a0 + b0 * b0);
-        q[3] = k0 * (d0 * d0);
+    static constexpr int NQ = 3;
+    static constexpr int MU_B = 0;
+    // The first ring holds e = (a - b)^2 (the reference side comes from the cache), the second b.
+    static __device__ __forceinline__ void centre(float k0, float e0, float b0, float q[3]) {
+        q[0] = k0 * b0;
+        q[1] = k0 * (b0 * b0);
+        q[2] = k0 * e0;
     }
-    static __device__ __forceinline__ void pair(float kt, float au, float ad, float bu, float bd, float q[4]) {
-#pragma clang fp contract(fast)
-        const float du = au - bu, dd = ad - bd;
-        q[0] += kt * (au + ad);
-        q[1] += kt * (bu + bd);
-        q[2] += kt * ((au * au + bu * bu) + (ad * ad + bd * bd));
-        q[3] += kt * (du * du + dd * dd);
+    static __device__ __forceinline__ void pair(float kt, float eu, float ed, float bu, float bd, float q[3]) {
+        q[0] = mad(kt, bu + bd, q[0]);
+        q[1] = mad(kt, mad(bu, bu, bd * bd), q[1]);
+        q[2] = mad(kt, eu + ed, q[2]);
     }
-    static __device__ __forceinline__ float ssim(const float v[4]) {
-#pragma clang fp contract(fast)
-        const float m1 = v[0], m2 = v[1], Q = v[2], R = v[3];
-        const float dmu = m1 - m2, dmu2 = dmu * dmu;
+    // v: the candidate's blurred quantities; ma, sa: mu_a and sigma_aa from the reference cache.
+    static __device__ __forceinline__ float ssim(const float v[3], float ma, float sa) {
+        const float m2 = v[0], B2 = v[1], R = v[2];
+        const float dmu = ma - m2, dmu2 = dmu * dmu;
         const float V = R - dmu2;
-        const float D = (Q - (m1 * m1 + m2 * m2)) + 0.0009f;
-        return fmaf(dmu2, D - V, V) * __builtin_amdgcn_rcpf(D);
+        const float D = (sa + mad(-m2, m2, B2)) + 0.0009f;
+        return mad(dmu2, D - V, V) * __builtin_amdgcn_rcpf(D);
     }
 };
 
@@ -113,7 +119,6 @@ struct Maps;
 template <>
 struct Maps<float> {
     static __device__ __forceinline__ float edge(float da, float db) {
-#pragma clang fp contract(fast)
         return (db - da) * __builtin_amdgcn_rcpf(1.0f + da);
     }
 };
@@ -134,16 +139,15 @@ struct Xyb<float> {
         return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
     }
     static __device__ __forceinline__ void eval(float r, float g, float b, float& X, float& Y, float& B) {
-#pragma clang fp contract(fast)
         const float M00 = 0.30f, M02 = 0.078f, M01 = float(1.0 - 0.078 - 0.30);
         const float M10 = 0.23f, M12 = 0.078f, M11 = float(1.0 - 0.078 - 0.23);
         const float M20 = float(0.2434226892454782), M21 = float(0.2047674442449682);
         const float M22 = float(1.0 - 0.2434226892454782 - 0.2047674442449682);
         const float BIAS = float(0.0037930732552754493);
         const float CB = float(0.15595420054924863);  // cbrt(BIAS), rounded to f32
-        const float m0 = cbrt_pos(fmaxf(M00 * r + M01 * g + M02 * b + BIAS, 0.0f)) - CB;
-        const float m1 = cbrt_pos(fmaxf(M10 * r + M11 * g + M12 * b + BIAS, 0.0f)) - CB;
-        const float m2 = cbrt_pos(fmaxf(M20 * r + M21 * g + M22 * b + BIAS, 0.0f)) - CB;
+        const float m0 = cbrt_pos(fmaxf(mad(M00, r, mad(M01, g, mad(M02, b, BIAS))), 0.0f)) - CB;
+        const float m1 = cbrt_pos(fmaxf(mad(M10, r, mad(M11, g, mad(M12, b, BIAS))), 0.0f)) - CB;
+        const float m2 = cbrt_pos(fmaxf(mad(M20, r, mad(M21, g, mad(M22, b, BIAS))), 0.0f)) - CB;
         const float x = 0.5f * (m0 - m1), y = 0.5f * (m0 + m1);
         X = x * 14.0f;      // (x*14 + 0.42) - 0.42
         Y = y - 0.5f;       // (y + 0.01) - 0.51
@@ -295,24 +299,49 @@ struct Loader<kSrcRecon> {
     }
 };
 
+// The reference image's per-scale cache written by k_ssim_ref: planes [9][h][w] holding the shifted XYB (0-2),
+// its blur mu_a (3-5) and sigma_aa = blur(a^2) - mu_a^2 (6-8). load() returns XYB, not linear RGB.
+template <>
+struct Loader<kSrcRefCache> {
+    const float* base;
+    const float* row;
+    const float* orow;
+    uint64_t n;
+    uint32_t w;
+    __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t h) {
+        base = S.planes;
+        w = w_;
+        n = uint64_t(w_) * h;
+    }
+    __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w; }
+    __device__ __forceinline__ void load(uint32_t x, const float*, float out[3]) const {
+        out[0] = gload(row + x);
+        out[1] = gload(row + n + x);
+        out[2] = gload(row + 2 * n + x);
+    }
+    // a, mu_a and sigma_aa at an output row.
+    __device__ __forceinline__ void begin_out_row(uint32_t r) { orow = base + uint64_t(r) * w; }
+    __device__ __forceinline__ void load_stats(uint32_t x, float a[3], float mu[3], float sig[3]) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a[c] = gload(orow + c * n + x);
+            mu[c] = gload(orow + (3 + c) * n + x);
+            sig[c] = gload(orow + (6 + c) * n + x);
+        }
+    }
+};
+
 // Exchange with the neighbouring lane of the pair (x ^ 1) through DPP (quad_perm [1,0,3,2]).
 __device__ __forceinline__ float swap_pair(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 
-__device__ __forceinline__ int find_job(const uint32_t* begins, int njobs, uint32_t bid) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (begins[mid] <= bid) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Per-workgroup constants and per-thread state of the scale kernel's row walk.
+// Per-workgroup constants and per-thread state of the scale kernel's row walk. The f32 kernel reads the reference
+// side from its cache (KA = kSrcRefCache); the f64 kernel computes both sides from pixels or pyramid planes.
 template <typename T, int NT, uint32_t KA, uint32_t KB>
 struct Walk {
+    static constexpr bool kCached = KA == kSrcRefCache;
+    static_assert(kCached == (sizeof(T) == 4), "the f32 kernel reads the reference cache; the f64 kernel does not");
     Loader<KA> la;
     Loader<KB> lb;
     float* a_next;
@@ -321,81 +350,101 @@ struct Walk {
     const T* taps;
     T (*s_q)[3 * Form<T>::NQ][NT];
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
-    bool in_img, is_out, want_a, want_b, want_next, ds_writer;
+    bool in_img, is_out, want_next, ds_writer;
     int yy;
-    T ra[11][3], rb[11][3];
+    T ra[11][3], rb[11][3];  // XYB rings of the reference (cached: (a - b)^2 instead) and of the distorted image
     T acc[18];
     float v0[3], v1[3];  // first row of the current 2x2 blocks: left and right column, this lane's image
 };
+
+// Next-scale 2x2 mean of one image row (core/src/ssimulacra2.rs:225-247). x0v/x1v hold the pair's left and right
+// columns; the first row of a block is kept in v0/v1 and the block is written on its second row (or on the last row
+// of an odd-height image), summed in f64 row by row, left to right, as the reference does.
+__device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[3], float v0[3], float v1[3],
+                                            bool first_row, bool emit, int gx, int yy, int w, int nw, int nh,
+                                            float* dst) {
+    if (first_row) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v0[c] = x0v[c];
+            v1[c] = x1v[c];
+        }
+    }
+    if (!emit || !dst) return;
+    const int gx0 = gx & ~1;
+    const bool two_cols = gx0 + 1 < w;
+    const int oy = yy >> 1, ox = gx0 >> 1;
+    const uint64_t n2 = uint64_t(nw) * nh, o = uint64_t(oy) * nw + ox;
+    const double inv = 1.0 / double((first_row ? 1 : 2) * (two_cols ? 2 : 1));
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double sum = double(v0[c]);
+        if (two_cols) sum += double(v1[c]);
+        if (!first_row) {
+            sum += double(x0v[c]);
+            if (two_cols) sum += double(x1v[c]);
+        }
+        gstore(dst + c * n2 + o, float(sum * inv));
+    }
+}
 
 // One input row of the walk, in ring slot S (compile-time, so the ring stays in registers). Returns false once the
 // walk is past its last row.
 template <int S, typename T, int NT, uint32_t KA, uint32_t KB>
 __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
 #pragma clang fp contract(fast)
+    constexpr bool kCached = Walk<T, NT, KA, KB>::kCached;
     const int yy = st.yy;
     if (yy > st.ylast) return false;
-    const int h = st.h, w = st.w;
+    const int h = st.h;
     const int r = yy < 0 ? -yy - 1 : (yy >= h ? 2 * h - 1 - yy : yy);
     float la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
     st.la.begin_row(uint32_t(r));
     st.lb.begin_row(uint32_t(r));
+    // Beyond the left/right edges the reference's samples are 0, i.e. -offset in the shifted domain, for both images
+    // (so their difference is 0).
+    T xa[3], xb[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) xa[c] = xb[c] = -Xyb<T>::off(c);
     if (st.in_img) {
         st.la.load(uint32_t(st.gx), st.lut, la);
         st.lb.load(uint32_t(st.gx), st.lut, lb);
-        Xyb<T>::eval(T(la[0]), T(la[1]), T(la[2]), st.ra[S][0], st.ra[S][1], st.ra[S][2]);
-        Xyb<T>::eval(T(lb[0]), T(lb[1]), T(lb[2]), st.rb[S][0], st.rb[S][1], st.rb[S][2]);
-    } else {
-        // Beyond the left/right edges the reference's samples are 0, i.e. -offset in the shifted domain.
+        Xyb<T>::eval(T(lb[0]), T(lb[1]), T(lb[2]), xb[0], xb[1], xb[2]);
+        if constexpr (kCached) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            st.ra[S][c] = -Xyb<T>::off(c);
-            st.rb[S][c] = -Xyb<T>::off(c);
+            for (int c = 0; c < 3; c++) xa[c] = la[c];
+        } else {
+            Xyb<T>::eval(T(la[0]), T(la[1]), T(la[2]), xa[0], xa[1], xa[2]);
         }
     }
-    // Next-scale 2x2 mean (uniform condition: the row is one of this tile's own rows). The two lanes of a column
-    // pair split the work: the even lane sums the reference's block, the odd lane the distorted one's, each after
-    // one DPP exchange per channel of the value its partner needs.
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        st.rb[S][c] = xb[c];
+        if constexpr (kCached) {
+            const T e = xa[c] - xb[c];
+            st.ra[S][c] = e * e;
+        } else {
+            st.ra[S][c] = xa[c];
+        }
+    }
+    // Next-scale 2x2 mean (uniform condition: the row is one of this tile's own rows). Uncached, the two lanes of a
+    // column pair split the work: the even lane sums the reference's block, the odd lane the distorted one's, each
+    // after one DPP exchange per channel of the value its partner needs. Cached, only the distorted image's level is
+    // written, by the even lane.
     if (st.want_next && yy >= st.y0 && yy < st.y_end) {
         const bool odd = (st.gx & 1) != 0;
         float x0v[3], x1v[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float recv = swap_pair(odd ? la[c] : lb[c]);
-            const float own = odd ? lb[c] : la[c];
+            const float recv = swap_pair(kCached ? lb[c] : (odd ? la[c] : lb[c]));
+            const float own = (kCached || odd) ? lb[c] : la[c];
             x0v[c] = odd ? recv : own;
             x1v[c] = odd ? own : recv;
         }
         const bool first_row = ((yy - st.y0) & 1) == 0;
-        if (first_row) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                st.v0[c] = x0v[c];
-                st.v1[c] = x1v[c];
-            }
-        }
-        const bool emit = !first_row || yy == h - 1;
-        if (emit && st.ds_writer) {
-            const int gx0 = st.gx & ~1;
-            const bool two_cols = gx0 + 1 < w;
-            const int oy = yy >> 1, ox = gx0 >> 1;
-            const uint64_t n2 = uint64_t(st.nw) * st.nh, o = uint64_t(oy) * st.nw + ox;
-            const double inv = 1.0 / double((first_row ? 1 : 2) * (two_cols ? 2 : 1));
-            float* dst = odd ? st.b_next : st.a_next;
-            if (dst) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    // core/src/ssimulacra2.rs:236-243: row by row, left to right, in f64.
-                    double sum = double(st.v0[c]);
-                    if (two_cols) sum += double(st.v1[c]);
-                    if (!first_row) {
-                        sum += double(x0v[c]);
-                        if (two_cols) sum += double(x1v[c]);
-                    }
-                    gstore(dst + c * n2 + o, float(sum * inv));
-                }
-            }
-        }
+        const bool emit = (!first_row || yy == h - 1) && st.ds_writer;
+        float* dst = odd ? st.b_next : (kCached ? st.b_next : st.a_next);
+        pyramid_row(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, dst);
     }
     if (yy >= st.y0 + 5) {
         // Vertical blur for output row yy-5: centre in slot (S+6)%11, distance t in (S+6-t)%11 and (S+6+t)%11.
@@ -418,6 +467,12 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
         }
         __syncthreads();
         if (st.is_out) {
+            float ac[3] = {0, 0, 0}, ma[3] = {0, 0, 0}, sa[3] = {0, 0, 0};
+            if constexpr (kCached) {
+                // The reference's a, mu_a and sigma_aa at this output pixel, loaded ahead of the horizontal blur.
+                st.la.begin_out_row(uint32_t(yy - 5));
+                st.la.load_stats(uint32_t(st.gx), ac, ma, sa);
+            }
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 T v[NQ];
@@ -426,13 +481,22 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
                     const T* row = &st.s_q[buf][c * NQ + i][tid];
                     T sum = k[0] * row[0];
 #pragma unroll
-                    for (int t = 1; t <= 5; t++) sum += k[t] * (row[-t] + row[t]);
+                    for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t] + row[t], sum);
                     v[i] = sum;
                 }
-                const T m1 = v[0], m2 = v[1];
-                T d = Form<T>::ssim(v);
+                T a0, m1, d;
+                if constexpr (kCached) {
+                    a0 = ac[c];
+                    m1 = ma[c];
+                    d = Form<T>::ssim(v, ma[c], sa[c]);
+                } else {
+                    a0 = st.ra[SC][c];
+                    m1 = v[0];
+                    d = Form<T>::ssim(v);
+                }
+                const T m2 = v[Form<T>::MU_B];
                 d = d > T(0) ? d : T(0);
-                const T d1 = Maps<T>::edge(fabs(st.ra[SC][c] - m1), fabs(st.rb[SC][c] - m2));
+                const T d1 = Maps<T>::edge(fabs(a0 - m1), fabs(st.rb[SC][c] - m2));
                 const T art = d1 > T(0) ? d1 : T(0);
                 const T det = -d1 > T(0) ? -d1 : T(0);
                 const T d2 = d * d, a2 = art * art, e2 = det * det;
@@ -450,20 +514,21 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
 }
 
 template <typename T, int NT, uint32_t KA, uint32_t KB>
-__global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ jobs,
-                                                    const uint32_t* __restrict__ tile_begins, int njobs,
-                                                    Taps<T> taps, const float* __restrict__ lut_g) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_ssim_scale(
+    const SsimJob* __restrict__ jobs, const Unit* __restrict__ units, Taps<T> taps, const float* __restrict__ lut_g) {
     constexpr int OW = NT - 12;
     constexpr int NWAVES = NT / 64;
     __shared__ T s_q[2][3 * Form<T>::NQ][NT];
     __shared__ float s_lut[256];
     __shared__ double s_red[NWAVES][18];
 
+    const Unit u = units ? units[blockIdx.x] : Unit{0u, blockIdx.x};  // no table: one job, tiles in order
+    if (u.job == kIdleJob) return;  // uniform over the workgroup, before any barrier
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += NT) s_lut[i] = lut_g[i];
 
-    const SsimJob& J = jobs[find_job(tile_begins, njobs, blockIdx.x)];
-    const uint32_t tile = blockIdx.x - J.tile_begin;
+    const SsimJob& J = jobs[u.job];
+    const uint32_t tile = u.tile;
     const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
     Walk<T, NT, KA, KB> st;
     st.la.init(J.a, J.w, J.h);
@@ -484,14 +549,13 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     st.is_out = tid >= 6 && tid < 6 + OW && st.gx < st.w;
     st.y_end = min(st.y0 + TH, st.h);
     st.ylast = st.y_end + 4;
-    st.want_a = st.a_next != nullptr;
-    st.want_b = st.b_next != nullptr;
-    st.want_next = st.want_a || st.want_b;
+    st.want_next = st.a_next != nullptr || st.b_next != nullptr;
     st.nw = (st.w + 1) >> 1;
     st.nh = (st.h + 1) >> 1;
-    // Both lanes of a column pair write (the even one the reference's plane, the odd one the distorted one's) when
-    // the pair's left column is one of this tile's output columns inside the image.
-    st.ds_writer = st.want_next && tid >= 6 && tid < 6 + OW && (st.gx & ~1) < st.w;
+    // A column pair's lanes write when the pair's left column is one of this tile's output columns inside the image:
+    // uncached both (the even one the reference's plane, the odd one the distorted one's), cached the even one.
+    st.ds_writer = st.want_next && tid >= 6 && tid < 6 + OW && (st.gx & ~1) < st.w &&
+                   (!Walk<T, NT, KA, KB>::kCached || (st.gx & 1) == 0);
     st.yy = st.y0 - 5;
 #pragma unroll
     for (int i = 0; i < 18; i++) st.acc[i] = T(0);
@@ -522,7 +586,144 @@ __global__ __launch_bounds__(NT) void k_ssim_scale(const SsimJob* __restrict__ j
     }
 }
 
+// ----------------------------------------------------------------------------------------------------------------
+// Reference cache. k_ssim_ref<NT, KA> walks one image at one scale with the same tiles and border rules as the scale
+// kernel and writes, for every pixel, the quantities of the reference side that do not depend on a candidate: the
+// shifted XYB a (planes 0-2), mu_a = blur(a) (3-5) and sigma_aa = blur(a^2) - mu_a^2 (6-8); it also writes the
+// image's next pyramid level. Every f32 candidate evaluation of the image then reads these 9 floats per pixel instead
+// of recomputing the reference's colour transform and two of its five blurs. The values at a pixel do not depend on
+// the tiling, so the cache is the same whichever tile policy built it.
 
+template <int NT, uint32_t KA>
+struct RefWalk {
+    Loader<KA> la;
+    float* a_next;
+    float* cache;
+    const float* lut;
+    const float* taps;
+    float (*s_q)[6][NT];
+    uint64_t n;
+    int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
+    bool in_img, is_out, ds_writer;
+    int yy;
+    float ra[11][3];
+    float v0[3], v1[3];
+};
+
+template <int S, int NT, uint32_t KA>
+__device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
+    const int yy = st.yy;
+    if (yy > st.ylast) return false;
+    const int h = st.h;
+    const int r = yy < 0 ? -yy - 1 : (yy >= h ? 2 * h - 1 - yy : yy);
+    float la[3] = {0, 0, 0};
+    st.la.begin_row(uint32_t(r));
+    if (st.in_img) {
+        st.la.load(uint32_t(st.gx), st.lut, la);
+        Xyb<float>::eval(la[0], la[1], la[2], st.ra[S][0], st.ra[S][1], st.ra[S][2]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) st.ra[S][c] = -Xyb<float>::off(c);
+    }
+    if (st.a_next && yy >= st.y0 && yy < st.y_end) {
+        // The even lane of a column pair sums the 2x2 block after one DPP exchange per channel.
+        float x0v[3], x1v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            x0v[c] = la[c];
+            x1v[c] = swap_pair(la[c]);
+        }
+        const bool first_row = ((yy - st.y0) & 1) == 0;
+        const bool emit = (!first_row || yy == h - 1) && st.ds_writer;
+        pyramid_row(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.a_next);
+    }
+    if (yy >= st.y0 + 5) {
+        constexpr int SC = (S + 6) % 11;
+        const int buf = (yy - 5) & 1;
+        const float* k = st.taps;
+        const int tid = st.tid;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float a0 = st.ra[SC][c];
+            float m = k[0] * a0, q = k[0] * (a0 * a0);
+#pragma unroll
+            for (int t = 1; t <= 5; t++) {
+                const float au = st.ra[(S + 6 - t + 11) % 11][c], ad = st.ra[(S + 6 + t) % 11][c];
+                m = mad(k[t], au + ad, m);
+                q = mad(k[t], mad(au, au, ad * ad), q);
+            }
+            st.s_q[buf][2 * c][tid] = m;
+            st.s_q[buf][2 * c + 1][tid] = q;
+        }
+        __syncthreads();
+        if (st.is_out) {
+            float* o = st.cache + uint64_t(yy - 5) * st.w + st.gx;
+            const uint64_t n = st.n;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float v[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const float* row = &st.s_q[buf][2 * c + i][tid];
+                    float sum = k[0] * row[0];
+#pragma unroll
+                    for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t] + row[t], sum);
+                    v[i] = sum;
+                }
+                gstore(o + c * n, st.ra[SC][c]);
+                gstore(o + (3 + c) * n, v[0]);
+                gstore(o + (6 + c) * n, mad(-v[0], v[0], v[1]));
+            }
+        }
+    }
+    st.yy = yy + 1;
+    return true;
+}
+
+template <int NT, uint32_t KA>
+__global__ __launch_bounds__(NT) void k_ssim_ref(const RefJob* __restrict__ jobs, const Unit* __restrict__ units,
+                                                 Taps<float> taps, const float* __restrict__ lut_g) {
+    constexpr int OW = NT - 12;
+    __shared__ float s_q[2][6][NT];
+    __shared__ float s_lut[256];
+    const Unit u = units ? units[blockIdx.x] : Unit{0u, blockIdx.x};  // no table: one job, tiles in order
+    if (u.job == kIdleJob) return;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 256; i += NT) s_lut[i] = lut_g[i];
+
+    const RefJob& J = jobs[u.job];
+    const uint32_t tile = u.tile;
+    const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+    RefWalk<NT, KA> st;
+    st.la.init(J.a, J.w, J.h);
+    st.a_next = J.a_next;
+    st.cache = J.cache;
+    st.lut = s_lut;
+    st.taps = taps.k;
+    st.s_q = s_q;
+    st.n = uint64_t(J.w) * J.h;
+    st.tid = tid;
+    st.w = int(J.w);
+    st.h = int(J.h);
+    st.x0 = int(tx) * OW;
+    st.y0 = int(ty) * int(J.th);
+    st.gx = st.x0 - 6 + tid;
+    st.in_img = st.gx >= 0 && st.gx < st.w;
+    st.is_out = tid >= 6 && tid < 6 + OW && st.gx < st.w;
+    st.y_end = min(st.y0 + int(J.th), st.h);
+    st.ylast = st.y_end + 4;
+    st.nw = (st.w + 1) >> 1;
+    st.nh = (st.h + 1) >> 1;
+    st.ds_writer = tid >= 6 && tid < 6 + OW && st.gx < st.w && (st.gx & 1) == 0;
+    st.yy = st.y0 - 5;
+#pragma unroll
+    for (int c = 0; c < 3; c++) st.v0[c] = st.v1[c] = 0.0f;
+    __syncthreads();
+    while (ref_row<0>(st) && ref_row<1>(st) && ref_row<2>(st) && ref_row<3>(st) && ref_row<4>(st) &&
+           ref_row<5>(st) && ref_row<6>(st) && ref_row<7>(st) && ref_row<8>(st) && ref_row<9>(st) &&
+           ref_row<10>(st)) {
+    }
+}
 
 // One 256-thread workgroup per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean and
 // 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
@@ -605,44 +806,86 @@ int upload_weights() {
 }
 
 template <typename T, int NT, uint32_t KA, uint32_t KB>
-static void launch_one(rcg_ctx* ctx, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t tiles,
-                       const Taps<T>& t) {
-    hipLaunchKernelGGL((k_ssim_scale<T, NT, KA, KB>), dim3(tiles), dim3(NT), 0, ctx->stream, d_jobs, d_begins, njobs, t,
+static void launch_one(rcg_ctx* ctx, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, const Taps<T>& t) {
+    hipLaunchKernelGGL((k_ssim_scale<T, NT, KA, KB>), dim3(nunits), dim3(NT), 0, ctx->stream, d_jobs, d_units, t,
                        ctx->d_lut);
 }
 
 template <typename T, int NT>
-static int launch_kinds(rcg_ctx* ctx, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t tiles,
-                        const Taps<T>& t, uint32_t ka, uint32_t kb) {
-    if (ka == kSrcBytes && kb == kSrcRecon)
-        launch_one<T, NT, kSrcBytes, kSrcRecon>(ctx, d_jobs, d_begins, njobs, tiles, t);
-    else if (ka == kSrcBytes && kb == kSrcBytes)
-        launch_one<T, NT, kSrcBytes, kSrcBytes>(ctx, d_jobs, d_begins, njobs, tiles, t);
-    else if (ka == kSrcPlanes && kb == kSrcPlanes)
-        launch_one<T, NT, kSrcPlanes, kSrcPlanes>(ctx, d_jobs, d_begins, njobs, tiles, t);
-    else {
-        set_error("internal: unsupported SSIMULACRA2 source combination");
-        return RCG_DEVICE;
+static int launch_kinds(rcg_ctx* ctx, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, const Taps<T>& t,
+                        uint32_t ka, uint32_t kb) {
+    if constexpr (sizeof(T) == 4) {
+        if (ka == kSrcRefCache && kb == kSrcRecon)
+            launch_one<T, NT, kSrcRefCache, kSrcRecon>(ctx, d_jobs, d_units, nunits, t);
+        else if (ka == kSrcRefCache && kb == kSrcBytes)
+            launch_one<T, NT, kSrcRefCache, kSrcBytes>(ctx, d_jobs, d_units, nunits, t);
+        else if (ka == kSrcRefCache && kb == kSrcPlanes)
+            launch_one<T, NT, kSrcRefCache, kSrcPlanes>(ctx, d_jobs, d_units, nunits, t);
+        else {
+            set_error("internal: unsupported SSIMULACRA2 source combination");
+            return RCG_DEVICE;
+        }
+    } else {
+        if (ka == kSrcBytes && kb == kSrcRecon)
+            launch_one<T, NT, kSrcBytes, kSrcRecon>(ctx, d_jobs, d_units, nunits, t);
+        else if (ka == kSrcBytes && kb == kSrcBytes)
+            launch_one<T, NT, kSrcBytes, kSrcBytes>(ctx, d_jobs, d_units, nunits, t);
+        else if (ka == kSrcPlanes && kb == kSrcPlanes)
+            launch_one<T, NT, kSrcPlanes, kSrcPlanes>(ctx, d_jobs, d_units, nunits, t);
+        else {
+            set_error("internal: unsupported SSIMULACRA2 source combination");
+            return RCG_DEVICE;
+        }
     }
     return RCG_OK;
 }
 
-int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const uint32_t* d_begins, int njobs,
-                      uint32_t total_tiles, int nt, uint32_t kind_a, uint32_t kind_b) {
-    if (total_tiles == 0) return RCG_OK;
+int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
+                      uint32_t kind_a, uint32_t kind_b) {
+    if (nunits == 0) return RCG_OK;
     ProfScope ps(ctx, "k_ssim_scale");
     int rc;
     if (precision == RCG_PREC_FP64) {
         Taps<double> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps64[i];
-        rc = launch_kinds<double, 128>(ctx, d_jobs, d_begins, njobs, total_tiles, t, kind_a, kind_b);
+        rc = launch_kinds<double, 128>(ctx, d_jobs, d_units, nunits, t, kind_a, kind_b);
     } else {
         Taps<float> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
-        rc = nt == 128 ? launch_kinds<float, 128>(ctx, d_jobs, d_begins, njobs, total_tiles, t, kind_a, kind_b)
-                       : launch_kinds<float, 256>(ctx, d_jobs, d_begins, njobs, total_tiles, t, kind_a, kind_b);
+        rc = nt == 128 ? launch_kinds<float, 128>(ctx, d_jobs, d_units, nunits, t, kind_a, kind_b)
+                       : launch_kinds<float, 256>(ctx, d_jobs, d_units, nunits, t, kind_a, kind_b);
     }
     if (rc != RCG_OK) return rc;
+    RCG_HIP(hipGetLastError());
+    return RCG_OK;
+}
+
+int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
+                    uint32_t kind_a) {
+    if (nunits == 0) return RCG_OK;
+    if (kind_a != kSrcBytes && kind_a != kSrcPlanes) {
+        set_error("internal: unsupported reference-cache source");
+        return RCG_DEVICE;
+    }
+    ProfScope ps(ctx, "k_ssim_ref");
+    Taps<float> t;
+    for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
+    const dim3 grid(nunits);
+    if (nt == 128) {
+        if (kind_a == kSrcBytes)
+            hipLaunchKernelGGL((k_ssim_ref<128, kSrcBytes>), grid, dim3(128), 0, ctx->stream, d_jobs, d_units, t,
+                               ctx->d_lut);
+        else
+            hipLaunchKernelGGL((k_ssim_ref<128, kSrcPlanes>), grid, dim3(128), 0, ctx->stream, d_jobs, d_units,
+                               t, ctx->d_lut);
+    } else {
+        if (kind_a == kSrcBytes)
+            hipLaunchKernelGGL((k_ssim_ref<256, kSrcBytes>), grid, dim3(256), 0, ctx->stream, d_jobs, d_units, t,
+                               ctx->d_lut);
+        else
+            hipLaunchKernelGGL((k_ssim_ref<256, kSrcPlanes>), grid, dim3(256), 0, ctx->stream, d_jobs, d_units,
+                               t, ctx->d_lut);
+    }
     RCG_HIP(hipGetLastError());
     return RCG_OK;
 }
