@@ -71,13 +71,14 @@ def main():
 
     rc = importlib.import_module("rust-compress_amd")
     ctx = rc.Context(0)
-    tune = ctx.autotune(8)  # tile heights per scale-height class, measured on this GPU
-    progress(f"autotuned tile policy: {tune.as_dict()}")
     W = H = args.size
     img = rc.synth_rgb8(2, W, H)
     d_img = ctx.upload(img)  # resident in HBM before the timed region
     ptr = d_img.ptr
     cands = [(q, s) for s in (rc.S420, rc.S444) for q in (50, 75, 90)]
+    # Tile heights per scale-height class, measured on this GPU for this workload's candidate count.
+    tune = ctx.autotune(len(cands))
+    progress(f"autotuned tile policy: {tune.as_dict()}")
     target = 70.0
 
     def step():
@@ -191,6 +192,7 @@ def bench_batch(rc, ctx, args):
     gen_s = time.perf_counter() - t0
     ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
     cands = [(q, rc.S420) for q in CANDS_B]
+    tune_b = ctx.autotune(len(cands))
     total_mp = sum(w * h for w, h in sizes) / 1e6
     ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)  # warm-up (sizes the workspace)
     ctx.synchronize()
@@ -200,13 +202,15 @@ def bench_batch(rc, ctx, args):
     ctx.synchronize()
     t1 = time.perf_counter()
     ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
+    ref_ms, _ = ctx.profile_get("k_ssim_ref")
     ctx.profile(False)
     batch_s = t1 - t0
     out = {"images": len(sizes), "megapixels": round(total_mp, 2), "candidates": list(CANDS_B),
            "seconds": round(batch_s, 4), "search_mps": round(total_mp / batch_s, 2),
            "eval_mps": round(total_mp * len(CANDS_B) / batch_s, 2),
            "hbm_frac": round(total_mp * 1e6 * (30.0 * len(CANDS_B) + 7.0) / batch_s / 1e9 / HBM_PEAK_GBS, 4),
-           "k_ssim_scale_ms": round(ssim_ms, 2), "k_ssim_scale_launches": ssim_n,
+           "k_ssim_scale_ms": round(ssim_ms, 2), "k_ssim_scale_launches": ssim_n, "k_ssim_ref_ms": round(ref_ms, 2),
+           "tile_policy": tune_b.as_dict(),
            "settled_fp64": int(sum(sum(r.settled[:len(CANDS_B)]) for r in res)),
            "chosen_equals_sweep": int(sum(r.chosen_quality == r.sweep_quality for r in res)),
            "generate_s": round(gen_s, 1)}

@@ -112,7 +112,6 @@ Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int s
 }
 
 PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
-    constexpr uint32_t kRefTiles = 2048;
     PairPlan P{};
     P.w = w;
     P.h = h;
@@ -131,10 +130,13 @@ PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
         P.tiles_x[s] = ssim_tiles_x(sw, int(P.nt[s]));
         P.tiles_y[s] = ssim_tiles_y(sh, int(P.th[s]));
         P.tiles[s] = P.tiles_x[s] * P.tiles_y[s];
-        // The reference-cache kernel's output does not depend on its tiling: one image's cache is built with shorter
-        // tiles, down to 16 rows, until the scale has kRefTiles workgroups (or the tiles are 16 rows high).
-        uint32_t tr = P.th[s];
-        while (tr > 16 && P.tiles_x[s] * ssim_tiles_y(sh, int(tr)) < kRefTiles) tr = std::max(16u, (tr / 2 + 1) & ~1u);
+        // The reference-cache kernel's output does not depend on its tiling: its tiles are as tall as one round of the
+        // GPU's workgroup slots allows for this scale (at least 16 rows), so one image keeps the GPU full without a
+        // partly idle last round.
+        const uint32_t slots = P.nt[s] == 128 ? T.ref_slots_narrow : T.ref_slots_wide;
+        const uint32_t rows_of_tiles = std::max(1u, slots / P.tiles_x[s]);
+        uint32_t tr = (sh + rows_of_tiles - 1) / rows_of_tiles;
+        tr = std::max(16u, (tr + 1) & ~1u);
         P.th_ref[s] = tr;
         P.tiles_ref[s] = P.tiles_x[s] * ssim_tiles_y(sh, int(tr));
         P.nscales++;
@@ -273,12 +275,21 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
     return st;
 }
 
+// The reference-cache launches (when the pair builds the reference state) go to the side stream, which the caller
+// has forked from the context's stream (fork_side()) when st.build_ref; work enqueued on the context's stream after
+// the fork runs beside them, and launch_staged_ssim() joins them.
+int launch_staged_refs(rcg_ctx* ctx, const StagedSsim& st) {
+    if (!st.build_ref) return RCG_OK;
+    for (uint32_t s = 0; s < st.P.nscales; s++)
+        RCG_TRY(launch_ssim_ref(ctx, st.ref_jobs[s], nullptr, st.P.tiles_ref[s], int(st.P.nt[s]),
+                                s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes), ctx->side));
+    return RCG_OK;
+}
+
 int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
     const bool f64 = st.precision == RCG_PREC_FP64;
+    if (st.build_ref) RCG_TRY(join_side(ctx));
     for (uint32_t s = 0; s < st.P.nscales; s++) {
-        if (st.build_ref)
-            RCG_TRY(launch_ssim_ref(ctx, st.ref_jobs[s], nullptr, st.P.tiles_ref[s], int(st.P.nt[s]),
-                                    s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes)));
         const uint32_t ka = !f64 ? uint32_t(kSrcRefCache) : (s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes));
         RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], nullptr, st.P.tiles[s], int(st.P.nt[s]), ka,
                                   s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes)));
@@ -326,8 +337,15 @@ EncJob make_enc_job(const Layout& L, const uint8_t* d_src) {
     return j;
 }
 
-int stage_and_launch_encode(rcg_ctx* ctx, Desc& D, const Layout& L, const uint8_t* d_src, int quality,
-                            int16_t* const d_coefs[3], uint8_t* const d_rec[3]) {
+// One encode job staged in D (not yet uploaded).
+struct StagedEncode {
+    const EncJob* job;
+    const uint32_t* begins;
+    uint32_t tiles;
+};
+
+StagedEncode stage_encode(Desc& D, const Layout& L, const uint8_t* d_src, int quality, int16_t* const d_coefs[3],
+                          uint8_t* const d_rec[3]) {
     EncCand c;
     std::memset(&c, 0, sizeof(c));
     uint8_t qy[64], qc[64];
@@ -341,10 +359,18 @@ int stage_and_launch_encode(rcg_ctx* ctx, Desc& D, const Layout& L, const uint8_
     j.ncand = 1;
     j.cand = D.put(&c, 1);
     const uint32_t zero = 0;
-    const uint32_t* db = D.put(&zero, 1);
-    const EncJob* dj = D.put(&j, 1);
+    StagedEncode se;
+    se.begins = D.put(&zero, 1);
+    se.job = D.put(&j, 1);
+    se.tiles = j.tiles_x * j.tiles_y;
+    return se;
+}
+
+int stage_and_launch_encode(rcg_ctx* ctx, Desc& D, const Layout& L, const uint8_t* d_src, int quality,
+                            int16_t* const d_coefs[3], uint8_t* const d_rec[3]) {
+    const StagedEncode se = stage_encode(D, L, d_src, quality, d_coefs, d_rec);
     RCG_TRY(D.upload(ctx->stream));
-    return launch_encode(ctx, dj, db, 1, j.tiles_x * j.tiles_y);
+    return launch_encode(ctx, se.job, se.begins, 1, se.tiles);
 }
 
 size_t coef_bytes(const Layout& L) {
@@ -430,6 +456,8 @@ int ssim_pair_dev(rcg_ctx* ctx, Call& call, const uint8_t* d_a, const uint8_t* d
     SsimBuffers B = take_ssim(ctx->arena, P, precision);
     StagedSsim st = stage_ssim(call.D, P, bytes_src(d_a, channels), bytes_src(d_b, channels), B, false, precision);
     RCG_TRY(call.D.upload(ctx->stream));
+    if (st.build_ref) RCG_TRY(fork_side(ctx));
+    RCG_TRY(launch_staged_refs(ctx, st));
     RCG_TRY(launch_staged_ssim(ctx, st));
     double stats[6 * 18];
     double sc = 0.0;
@@ -474,9 +502,12 @@ int rcg_create(int device, rcg_ctx** out) {
     if (!c) return RCG_OOM;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
-        set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
-        delete c;
+        set_error(std::string("stream/event creation: ") + hipGetErrorString(e));
+        rcg_destroy(c);
         return RCG_DEVICE;
     }
     float lut[256];
@@ -491,7 +522,14 @@ int rcg_create(int device, rcg_ctx** out) {
     make_taps(c->taps64);
     for (int i = 0; i < 6; i++) c->taps32[i] = float(c->taps64[i]);
     c->tune64.fixed_nt = 128;
-    int rc = upload_weights();
+    int rc = ref_kernel_slots(device, &c->tune32.ref_slots_wide, &c->tune32.ref_slots_narrow);
+    if (rc != RCG_OK) {
+        rcg_destroy(c);
+        return rc;
+    }
+    c->tune64.ref_slots_wide = c->tune32.ref_slots_wide;
+    c->tune64.ref_slots_narrow = c->tune32.ref_slots_narrow;
+    rc = upload_weights();
     if (rc != RCG_OK) {
         rcg_destroy(c);
         return rc;
@@ -504,9 +542,13 @@ void rcg_destroy(rcg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->arena.base) (void)hipFree(ctx->arena.base);
     if (ctx->pinned.base) (void)hipHostFree(ctx->pinned.base);
     if (ctx->d_lut) (void)hipFree(ctx->d_lut);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -748,11 +790,15 @@ int score_candidate(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_
                     uint8_t* const rec[3], const SsimBuffers& B32, const SsimBuffers& B64, bool& a32_ready,
                     bool& a64_ready, double target, double& score, bool& settled) {
     call.D.off = 0;
-    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, nullptr, rec));
+    const StagedEncode se = stage_encode(call.D, L, d_src, quality, nullptr, rec);
     const PairPlan P = plan_pair(L.w, L.h, ctx->tune32);
     StagedSsim st = stage_ssim(call.D, P, bytes_src(d_src, L.channels), recon_src(L, rec), B32, a32_ready,
                                RCG_PREC_FP32);
     RCG_TRY(call.D.upload(ctx->stream));
+    // The first evaluation builds the reference state on the side stream, beside the encode.
+    if (st.build_ref) RCG_TRY(fork_side(ctx));
+    RCG_TRY(launch_encode(ctx, se.job, se.begins, 1, se.tiles));
+    RCG_TRY(launch_staged_refs(ctx, st));
     RCG_TRY(launch_staged_ssim(ctx, st));
     double s = 0.0;
     RCG_HIP(hipMemcpyAsync(&s, B32.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1007,7 +1053,7 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
             const Unit* du = D.put(units.data(), units.size());
             const RefJob* drj = D.put(&rj, 1);
             RCG_TRY(D.upload(ctx->stream));
-            RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes));
+            RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes, ctx->stream));
             const uint32_t nu = uint32_t(units.size());
             RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
             const int reps = S >= 2048 ? 3 : 10;

@@ -5,7 +5,7 @@
 //   k_encode        one launch; per image one job per distinct sampling layout, whose candidates share the
 //                   colour conversion and FDCT and differ only in quantisation + IDCT;
 //   k_ssim_ref      per scale, one launch over the images whose reference state is not built yet: the reference
-//                   image's cache and next pyramid level;
+//                   image's cache and next pyramid level; these run on a second stream, beside k_encode;
 //   k_ssim_scale    per scale, one launch covering every (image, candidate) pair that has that scale, reading the
 //                   reference side from the cache;
 //   k_ssim_finalize one launch pooling every pair.
@@ -52,17 +52,33 @@ void Prof::collect() {
     pending.clear();
 }
 
-ProfScope::ProfScope(rcg_ctx* c, const char* n) : ctx(c), name(n) {
+ProfScope::ProfScope(rcg_ctx* c, const char* n, hipStream_t s) : ctx(c), name(n), stream(s ? s : c->stream) {
     if (!ctx->prof.on) return;
     hipEvent_t a = ctx->prof.get();
     b = ctx->prof.get();
-    if (a) (void)hipEventRecord(a, ctx->stream);
+    if (a) (void)hipEventRecord(a, stream);
     ctx->prof.pending.push_back(Prof::Pending{name, a, b});
 }
 
 ProfScope::~ProfScope() {
-    if (ctx->prof.on && b) (void)hipEventRecord(b, ctx->stream);
+    if (ctx->prof.on && b) (void)hipEventRecord(b, stream);
 }
+
+namespace rcg {
+
+int fork_side(rcg_ctx* ctx) {
+    RCG_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    RCG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    return RCG_OK;
+}
+
+int join_side(rcg_ctx* ctx) {
+    RCG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    RCG_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    return RCG_OK;
+}
+
+}  // namespace rcg
 
 extern "C" int rcg_profile(rcg_ctx* ctx, int enable) {
     if (!ctx) return RCG_INVALID;
@@ -202,7 +218,8 @@ struct StagedLaunch {
     uint32_t ka, kb;
 };
 struct StagedPairs {
-    std::vector<StagedLaunch> scales;
+    std::vector<StagedLaunch> refs;    // reference-cache launches, scale by scale
+    std::vector<StagedLaunch> scales;  // candidate launches, scale by scale
     const FinalDesc* fin = nullptr;
     int npairs = 0;
     int precision = RCG_PREC_FP32;
@@ -246,7 +263,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     g.nt = nt;
                     g.ka = s == 0 ? uint32_t(kSrcBytes) : uint32_t(kSrcPlanes);
                     g.kb = 0;
-                    out.scales.push_back(g);
+                    out.refs.push_back(g);
                 }
             }
             std::vector<SsimJob> sj;
@@ -308,14 +325,19 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
     return out;
 }
 
+// The reference-cache launches go to the side stream, which the caller has forked from the context's stream; the
+// caller enqueues work that does not need them (the encode) on the context's stream and join_side()s before
+// launch_pairs().
+int launch_refs_side(rcg_ctx* ctx, const StagedPairs& sp) {
+    for (const StagedLaunch& g : sp.refs)
+        RCG_TRY(launch_ssim_ref(ctx, static_cast<const RefJob*>(g.jobs), g.units, g.nunits, g.nt, g.ka, ctx->side));
+    return RCG_OK;
+}
+
 int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp) {
-    for (const StagedLaunch& g : sp.scales) {
-        if (g.ref)
-            RCG_TRY(launch_ssim_ref(ctx, static_cast<const RefJob*>(g.jobs), g.units, g.nunits, g.nt, g.ka));
-        else
-            RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
-                                      g.ka, g.kb));
-    }
+    for (const StagedLaunch& g : sp.scales)
+        RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
+                                  g.ka, g.kb));
     return launch_finalize(ctx, sp.fin, sp.npairs);
 }
 
@@ -443,7 +465,13 @@ int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, Score
 }
 
 int enqueue_items(rcg_ctx* ctx, const ScoreState& S) {
+    // The reference caches are built on the side stream beside the encode; the encode is enqueued first so that it
+    // starts while the host is still enqueueing the reference launches.
+    const bool refs = !S.fp32.refs.empty();
+    if (refs) RCG_TRY(fork_side(ctx));
     if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
+    if (refs) RCG_TRY(launch_refs_side(ctx, S.fp32));
+    if (refs) RCG_TRY(join_side(ctx));
     if (!S.pairs.empty()) {
         RCG_TRY(launch_pairs(ctx, S.fp32));
         RCG_HIP(hipMemcpyAsync(S.h_scores, S.d_s32, S.pairs.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -80,6 +80,9 @@ struct SsimTune {
     int th[8] = {64, 32, 16, 8, 8, 8, 8, 8};
     uint32_t th_min_h[8] = {2048, 512, 128, 0, 0, 0, 0, 0};  // descending; the last class must have 0
     int fixed_nt = 0;  // FP64 kernel: one thread count for every scale
+    // Workgroups of the reference-cache kernel the GPU holds at once (256 and 128 threads); its tiles are sized so
+    // that one image fills them in one round.
+    uint32_t ref_slots_wide = 1792, ref_slots_narrow = 3584;
 };
 
 }  // namespace rcg
@@ -87,6 +90,10 @@ struct SsimTune {
 struct rcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second stream (default priority) for the reference-cache launches, which run beside the encode launch: forked
+    // from and joined back into `stream` with the two events.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::mutex mu;
     rcg::Arena arena;
     rcg::Pinned pinned;
@@ -112,12 +119,13 @@ Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int s
 void quant_tables(int quality, uint8_t luma[64], uint8_t chroma[64]);
 void fill_cand_tables(const uint8_t qy[64], const uint8_t qc[64], EncCand& c);
 
-// Brackets one kernel launch with profiling events when profiling is on.
+// Brackets one kernel launch with profiling events when profiling is on (on `stream`, by default the context's).
 struct ProfScope {
     rcg_ctx* ctx;
     const char* name;
+    hipStream_t stream;
     hipEvent_t b = nullptr;
-    ProfScope(rcg_ctx* c, const char* n);
+    ProfScope(rcg_ctx* c, const char* n, hipStream_t s = nullptr);
     ~ProfScope();
 };
 
@@ -135,7 +143,11 @@ int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int r
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
                       uint32_t kind_a, uint32_t kind_b);
 int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
-                    uint32_t kind_a);
+                    uint32_t kind_a, hipStream_t stream);
+// Fork the side stream from the context's stream (everything enqueued there so far is visible to it), and join it
+// back (the context's stream waits for everything enqueued on the side stream so far).
+int fork_side(rcg_ctx* ctx);
+int join_side(rcg_ctx* ctx);
 
 // Jobs job0 .. job0+njobs-1 of a launch that share one tiling of `tiles` tiles (the candidates of one image at one
 // scale, or a single job).
@@ -156,6 +168,8 @@ struct FinalDesc {
 };
 int launch_finalize(rcg_ctx* ctx, const FinalDesc* d_jobs, int n);
 int upload_weights();
+// Resident workgroups of the reference-cache kernel on `device` (256- and 128-thread instances).
+int ref_kernel_slots(int device, uint32_t* wide, uint32_t* narrow);
 
 // Tiles of the SSIMULACRA2 scale kernel for a w x h scale.
 inline uint32_t ssim_tiles_x(uint32_t w, int nt) { return (w + (nt - 12) - 1) / (nt - 12); }

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "rcg_host.h"
 #include "rcg_jobs.h"
 #include "rcg_pixel.h"
@@ -860,30 +862,41 @@ int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const 
     return RCG_OK;
 }
 
+int ref_kernel_slots(int device, uint32_t* wide, uint32_t* narrow) {
+    hipDeviceProp_t prop;
+    RCG_HIP(hipGetDeviceProperties(&prop, device));
+    int b256 = 0, b128 = 0;
+    RCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b256, k_ssim_ref<256, kSrcBytes>, 256, 0));
+    RCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b128, k_ssim_ref<128, kSrcBytes>, 128, 0));
+    *wide = uint32_t(std::max(1, b256 * prop.multiProcessorCount));
+    *narrow = uint32_t(std::max(1, b128 * prop.multiProcessorCount));
+    return RCG_OK;
+}
+
 int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
-                    uint32_t kind_a) {
+                    uint32_t kind_a, hipStream_t stream) {
     if (nunits == 0) return RCG_OK;
     if (kind_a != kSrcBytes && kind_a != kSrcPlanes) {
         set_error("internal: unsupported reference-cache source");
         return RCG_DEVICE;
     }
-    ProfScope ps(ctx, "k_ssim_ref");
+    ProfScope ps(ctx, "k_ssim_ref", stream);
     Taps<float> t;
     for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
     const dim3 grid(nunits);
     if (nt == 128) {
         if (kind_a == kSrcBytes)
-            hipLaunchKernelGGL((k_ssim_ref<128, kSrcBytes>), grid, dim3(128), 0, ctx->stream, d_jobs, d_units, t,
+            hipLaunchKernelGGL((k_ssim_ref<128, kSrcBytes>), grid, dim3(128), 0, stream, d_jobs, d_units, t,
                                ctx->d_lut);
         else
-            hipLaunchKernelGGL((k_ssim_ref<128, kSrcPlanes>), grid, dim3(128), 0, ctx->stream, d_jobs, d_units,
+            hipLaunchKernelGGL((k_ssim_ref<128, kSrcPlanes>), grid, dim3(128), 0, stream, d_jobs, d_units,
                                t, ctx->d_lut);
     } else {
         if (kind_a == kSrcBytes)
-            hipLaunchKernelGGL((k_ssim_ref<256, kSrcBytes>), grid, dim3(256), 0, ctx->stream, d_jobs, d_units, t,
+            hipLaunchKernelGGL((k_ssim_ref<256, kSrcBytes>), grid, dim3(256), 0, stream, d_jobs, d_units, t,
                                ctx->d_lut);
         else
-            hipLaunchKernelGGL((k_ssim_ref<256, kSrcPlanes>), grid, dim3(256), 0, ctx->stream, d_jobs, d_units,
+            hipLaunchKernelGGL((k_ssim_ref<256, kSrcPlanes>), grid, dim3(256), 0, stream, d_jobs, d_units,
                                t, ctx->d_lut);
     }
     RCG_HIP(hipGetLastError());
