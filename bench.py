@@ -38,6 +38,69 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def _pmc_run(counter, bench_args, outdir, name):
+    """One rocprofv3 counter pass over a child run of this script; returns [(kernel, dispatch, KiB)] in dispatch
+    order, or None if the pass failed."""
+    import shutil
+    import sqlite3
+    import subprocess
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    d = os.path.join(outdir, name)
+    cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "-d", d, "-o", name, "--",
+           sys.executable, os.path.abspath(__file__)] + bench_args
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=ROOT)
+    dbs = []
+    for root, _, files in os.walk(d):
+        dbs += [os.path.join(root, f) for f in files if f.endswith("_results.db")]
+    if not dbs:
+        progress(f"{counter} pass failed (exit {r.returncode})")
+        return None
+    con = sqlite3.connect(dbs[0])
+    rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                       "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+    con.close()
+    return [(k.split("(")[0], i, v) for i, k, v in rows]
+
+
+def measure_traffic(size):
+    """HBM bytes per launch of the dominant kernel (k_ssim_scale) in one headline step, from FETCH_SIZE and
+    WRITE_SIZE (one rocprofv3 pass each over a one-step child run), each scaled by what the counter reads for a
+    512 MiB device-to-device copy (the same counter's calibration factor). Runs before this process touches the GPU:
+    the passes are child processes."""
+    import tempfile
+    out = tempfile.mkdtemp(prefix="rcg_traffic_")
+    step = ["--steps", "1", "--warmup", "0", "--size", str(size), "--no-cpu", "--no-plain", "--no-batch",
+            "--no-serve", "--no-traffic"]
+    copy = ["--copy-calibration", "512"]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        progress(f"traffic: {counter} passes")
+        cal = _pmc_run(counter, copy, out, counter.lower() + "_copy")
+        run = _pmc_run(counter, step, out, counter.lower() + "_step")
+        if not cal or not run:
+            return None
+        # Calibration: the copy's largest dispatch moved 512 MiB (read for FETCH_SIZE, written for WRITE_SIZE).
+        kib = max(v for _, _, v in cal)
+        factor = (512 * 1024) / kib if kib > 0 else None
+        # The last headline step: the dispatches after the last reference-cache launch from RGB8 bytes (the autotuner
+        # launches it only from planes).
+        last_ref = max((i for i, (k, _, _) in enumerate(run) if k.startswith("void rcg::k_ssim_ref<") and
+                        k.endswith("0u>")), default=None)
+        if last_ref is None or factor is None:
+            return None
+        vals = [v for k, _, v in run[last_ref:] if "k_ssim_scale<float" in k]
+        if not vals:
+            return None
+        res[counter] = {"kib_per_launch": sum(vals) / len(vals), "factor": factor, "launches": len(vals)}
+    fetch = res["FETCH_SIZE"]["kib_per_launch"] * res["FETCH_SIZE"]["factor"] * 1024
+    write = res["WRITE_SIZE"]["kib_per_launch"] * res["WRITE_SIZE"]["factor"] * 1024
+    return {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_factor": round(res["FETCH_SIZE"]["factor"], 3), "write_factor": round(res["WRITE_SIZE"]["factor"], 3),
+            "launches": res["FETCH_SIZE"]["launches"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +115,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (HBM traffic counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that give roofline.traffic")
     args = ap.parse_args()
     if args.copy_calibration:
         # One device-to-device copy of a known size (the traffic counters of a plain copy kernel).
@@ -69,6 +134,7 @@ def main():
               "(images would shard across GPUs with no collective; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
+    traffic = None if args.no_traffic else measure_traffic(args.size)  # child processes, before any GPU use here
     rc = importlib.import_module("rust-compress_amd")
     ctx = rc.Context(0)
     W = H = args.size
@@ -125,7 +191,10 @@ def main():
                                "trellis off, encode+reconstruct+SSIMULACRA2 per candidate",
                    "candidates": evals, "megapixels": mp},
         "roofline": {"bound": "hbm", "kernel": "k_ssim_scale", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["bytes_per_launch"]) if traffic else None,
+                     "traffic_detail": traffic,
+                     "algorithmic_bytes_per_launch": round(SSIM_BYTES_PER_PX * W * H * evals / max(1, ssim_n / args.steps)),
                      "avg_launch_ms": round(ssim_ms / max(1, ssim_n), 4), "launches": ssim_n,
                      "bytes_per_px_per_candidate": SSIM_BYTES_PER_PX},
         "kernels_ms_per_step": {"k_ssim_scale": round(ssim_ms / args.steps, 3),
@@ -138,6 +207,17 @@ def main():
     }
     eval_bytes = sum(EVAL_BYTES_PER_PX[1 if s == rc.S444 else 3] for _, s in cands) * W * H
     out["eval_hbm_frac"] = round(eval_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+    # The same step from a host buffer (rcg_score_batch copies the source over PCIe on every call): the rate a caller
+    # that hands over host memory sees. Not `value`, which starts with the source resident in HBM.
+    ctx.score_batch([img], cands, target)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.score_batch([img], cands, target)
+    ctx.synchronize()
+    host_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    out["pcie_inclusive"] = {"ms_per_step": round(host_ms, 3), "value": round(mp * evals / (host_ms / 1000.0), 2),
+                             "unit": "MP/s"}
 
     if not args.no_plain:
         ctx.find_quality_dev(ptr, W, H, 3, target, 100, rc.S420)
