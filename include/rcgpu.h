@@ -270,6 +270,25 @@ int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_ms, int* lau
  * rounded and clamped. Deterministic for (seed, width, height) on any thread count. */
 int rcg_synth_rgb8(uint64_t seed, uint32_t width, uint32_t height, uint8_t* out);
 
+/* ------------------------------------------------------------------------------------------------------------
+ * Entropy coding on the host (no device, no context): the stage after the search, which serialises the chosen
+ * coefficients. Byte-identical to the reference's writers. ci points to HOST coefficient buffers; 1-4 components.
+ * Descriptive metadata (EXIF, XMP, ICC) is not carried by rcg_coef_image and is not written.
+ */
+typedef enum {
+    RCG_WRITE_BASELINE = 0,     /* rc_jpeg::encode::write_jpeg (jpeg/src/encode.rs:595): baseline, optimised tables */
+    RCG_WRITE_PROGRESSIVE = 1,  /* rc_jpeg::progressive::write_progressive (jpeg/src/progressive.rs:386) */
+    RCG_WRITE_BEST = 2          /* rc_jpeg::encode::write_best (jpeg/src/encode.rs:543): the smaller estimate */
+} rcg_write_mode;
+
+/* *out receives a buffer of *len bytes allocated by the library; release it with rcg_free. */
+int rcg_write_jpeg(const rcg_coef_image* ci, int mode, uint8_t** out, size_t* len);
+void rcg_free(void* p);
+
+/* Replaces rc_core::util::huffman::build_lengths (core/src/util/huffman.rs:10): length-limited Huffman code
+ * lengths of n symbol frequencies (0 for unused symbols; a single used symbol gets length 1). */
+int rcg_huffman_lengths(const uint32_t* freqs, size_t n, int max_len, uint8_t* lengths);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,6 +150,11 @@ def lib():
         L.rcg_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
         L.rcg_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
         L.rcg_autotune.argtypes = [P, I, ctypes.POINTER(Tune)]
+        L.rcg_write_jpeg.argtypes = [ctypes.POINTER(_CoefImage), I, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+                                     ctypes.POINTER(ctypes.c_size_t)]
+        L.rcg_free.argtypes = [P]
+        L.rcg_free.restype = None
+        L.rcg_huffman_lengths.argtypes = [P, ctypes.c_size_t, I, P]
         L.rcg_profile.argtypes = [P, I]
         L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
         _LIB = L
@@ -546,6 +551,32 @@ def default_context():
 
 def build_coefs(img, quality, subsampling=S420):
     return default_context().build_coefs(img, quality, subsampling)
+
+
+WRITE_BASELINE, WRITE_PROGRESSIVE, WRITE_BEST = 0, 1, 2
+
+
+def write_jpeg(ci, mode=WRITE_BASELINE):
+    """Entropy-code a host CoefImage into JPEG bytes on the CPU (no device): WRITE_BASELINE is
+    rc_jpeg::encode::write_jpeg, WRITE_PROGRESSIVE rc_jpeg::progressive::write_progressive, WRITE_BEST
+    rc_jpeg::encode::write_best."""
+    keep = []
+    s = _coef_image_struct(ci, keep, device=False)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    _check(lib().rcg_write_jpeg(ctypes.byref(s), mode, ctypes.byref(out), ctypes.byref(n)))
+    try:
+        return ctypes.string_at(out, n.value)
+    finally:
+        lib().rcg_free(out)
+
+
+def huffman_lengths(freqs, max_len):
+    """rc_core::util::huffman::build_lengths: length-limited Huffman code lengths of the symbol frequencies."""
+    f = np.ascontiguousarray(freqs, dtype=np.uint32)
+    out = np.zeros(len(f), dtype=np.uint8)
+    _check(lib().rcg_huffman_lengths(f.ctypes.data, len(f), max_len, out.ctypes.data))
+    return out
 
 
 def reconstruct(ci):

@@ -204,13 +204,13 @@ inline Result<CoefImage> build_coefs(Device& dev, const Image& src, const Encode
     return ci;
 }
 
-// jpeg/src/decode.rs:334
-inline Result<Image> reconstruct(Device& dev, const CoefImage& ci) {
+// The C ABI's view of a CoefImage (borrows its buffers).
+inline Result<rcg_coef_image> to_c(const CoefImage& ci) {
     rcg_coef_image c{};
     c.width = uint32_t(ci.width);
     c.height = uint32_t(ci.height);
     c.ncomp = uint32_t(ci.comps.size());
-    if (c.ncomp > 4) return Error{Error::Invalid, "too many components"};
+    if (c.ncomp < 1 || c.ncomp > 4) return Error{Error::Invalid, "1 to 4 components"};
     for (size_t i = 0; i < ci.comps.size(); i++) {
         const CoefComponent& k = ci.comps[i];
         if (k.blocks_w == 0) return Error{Error::Invalid, "empty component"};
@@ -224,6 +224,34 @@ inline Result<Image> reconstruct(Device& dev, const CoefImage& ci) {
     }
     for (int s = 0; s < 4; s++) c.qtables[s] = ci.qtables[s] ? ci.qtables[s]->data() : nullptr;
     c.adobe_transform = ci.adobe_transform ? int(*ci.adobe_transform) : -1;
+    return c;
+}
+
+// Entropy coding on the host (no device): jpeg/src/encode.rs:595 write_jpeg, jpeg/src/progressive.rs:386
+// write_progressive, jpeg/src/encode.rs:543 write_best. The reference's functions cannot fail; these return an
+// error only for a coefficient image the reference could not have built (a missing table, a short block grid).
+inline Result<std::vector<uint8_t>> write_with(const CoefImage& ci, int mode) {
+    Result<rcg_coef_image> c = to_c(ci);
+    if (c.is_err()) return c.error();
+    uint8_t* out = nullptr;
+    size_t len = 0;
+    const int rc = rcg_write_jpeg(&c.value(), mode, &out, &len);
+    if (rc != RCG_OK) return Error::from_status(rc);
+    std::vector<uint8_t> bytes(out, out + len);
+    rcg_free(out);
+    return bytes;
+}
+inline Result<std::vector<uint8_t>> write_jpeg(const CoefImage& ci) { return write_with(ci, RCG_WRITE_BASELINE); }
+inline Result<std::vector<uint8_t>> write_progressive(const CoefImage& ci) {
+    return write_with(ci, RCG_WRITE_PROGRESSIVE);
+}
+inline Result<std::vector<uint8_t>> write_best(const CoefImage& ci) { return write_with(ci, RCG_WRITE_BEST); }
+
+// jpeg/src/decode.rs:334
+inline Result<Image> reconstruct(Device& dev, const CoefImage& ci) {
+    Result<rcg_coef_image> cc = to_c(ci);
+    if (cc.is_err()) return cc.error();
+    const rcg_coef_image& c = cc.value();
     Image img;
     img.width = c.width;
     img.height = c.height;

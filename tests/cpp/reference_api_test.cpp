@@ -127,6 +127,23 @@ int main() {
         CHECK(build_coefs(dev, textured(16, 16), t).is_err(), "trellis is CPU-only");
         std::printf("ok search_rejects_images_under_8x8\n");
     }
+    {  // the search's winner, written: the same file as writing build_coefs at the chosen quality (the CLI's
+       // encode_jpeg_pixels, cli/src/pipeline.rs:225-247), and write_best picks one of the two encodings
+        Image ref = textured(160, 96);
+        Found f = find_jpeg_quality(dev, ref, 75.0, opts(100, Subsampling::S420)).value();
+        const std::vector<uint8_t> a = write_jpeg(f.coefs).value();
+        const std::vector<uint8_t> b = write_jpeg(build_coefs(dev, ref, opts(f.quality, Subsampling::S420)).value()).value();
+        CHECK(a == b, "winner file differs");
+        CHECK(a.size() > 4 && a[0] == 0xFF && a[1] == 0xD8 && a[a.size() - 2] == 0xFF && a[a.size() - 1] == 0xD9,
+              "SOI/EOI");
+        const std::vector<uint8_t> p = write_progressive(f.coefs).value();
+        const std::vector<uint8_t> best = write_best(f.coefs).value();
+        CHECK(best == a || best == p, "write_best is one of the two");
+        CoefImage broken = f.coefs;
+        broken.qtables[0].reset();
+        CHECK(write_jpeg(broken).is_err(), "missing table");
+        std::printf("ok search_winner_writes_the_same_file\n");
+    }
     std::printf("all reference-API tests passed\n");
     return 0;
 }

@@ -286,3 +286,22 @@ def test_device_entry_points_match_host_ones(ctx, rc):
     f1 = ctx.find_quality_dev(d.ptr, 333, 250, 3, 70.0)
     f2 = ctx.find_quality(img, 70.0)
     assert (f1.quality, f1.score, f1.steps) == (f2.quality, f2.score, f2.steps)
+
+
+def test_written_files_from_device_coefficients_are_byte_identical(ctx, rc):
+    # The stage after the search: the device's coefficients, entropy-coded on the host, give the same file as the
+    # oracle's coefficients (the coefficients are bit-exact), for baseline and progressive output; and the file
+    # decodes (libjpeg) to the device reconstruction within an independent decoder's IDCT rounding.
+    import io
+    for (w, h, q, subs) in [(333, 250, 80, O.S420), (97, 61, 92, O.S444), (640, 480, 55, O.S422)]:
+        img = rc.synth_rgb8(77 + w, w, h)
+        dev = ctx.build_coefs(img, q, subs)
+        ora = O.build_coefs(img, q, subs)
+        as_rc = rc.CoefImage(ora.width, ora.height,
+                             [rc.CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, c.coefs) for c in ora.comps],
+                             list(ora.qtables) + [None] * (4 - len(ora.qtables)))
+        for mode in (rc.WRITE_BASELINE, rc.WRITE_PROGRESSIVE, rc.WRITE_BEST):
+            assert rc.write_jpeg(dev, mode) == rc.write_jpeg(as_rc, mode), (w, h, mode)
+        dec = np.asarray(Image.open(io.BytesIO(rc.write_jpeg(dev, rc.WRITE_BASELINE)))).astype(np.int32)
+        d = np.abs(dec - ctx.reconstruct(dev).astype(np.int32))
+        assert d.max() <= 3, (w, h, d.max())
