@@ -271,6 +271,27 @@ int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_ms, int* lau
 int rcg_synth_rgb8(uint64_t seed, uint32_t width, uint32_t height, uint8_t* out);
 
 /* ------------------------------------------------------------------------------------------------------------
+ * Trellis hand-off (jpeg/src/encode.rs:527-537, jpeg/src/rdo.rs). The reference's default encode re-quantises
+ * every block with a rate-distortion trellis that runs on the CPU. The device supplies what that pass needs, so the
+ * host trellis skips its own FDCT:
+ *   rcg_build_scaled  the plain coefficients (as rcg_build_coefs) and, per block, the 64 f32 values
+ *                     a[k] = dct[f]*div[f] of quantize_plane's trellis branch (encode.rs:206-211), zigzag order,
+ *                     same block layout as the coefficients. Bit-exact with the reference.
+ *   rcg_trellis       the trellis pass on the host: code lengths measured on the plain coefficients
+ *                     (ac_code_lengths, encode.rs:222-227), then per block the DC rounded and clamped and
+ *                     Rdo::quantize_ac (rdo.rs:52-113) with TRELLIS_KAPPA. plain: host coefficient image with the
+ *                     tables in slots 0 (luma) and 1 (chroma); scaled[c]: that component's values; out[c] receives
+ *                     the trellis coefficients (may alias nothing in plain). Bit-exact with the reference.
+ *   rcg_rdo_quantize_ac  rc_jpeg::rdo::Rdo::new(table, kappa, sizes).quantize_ac(a, out) for one block (positions
+ *                     1..63 of out are written).
+ */
+int rcg_build_scaled(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                     int quality, int subsampling, int16_t* coefs[3], float* scaled[3], uint8_t qtables[128]);
+int rcg_trellis(const rcg_coef_image* plain, const float* const scaled[4], int16_t* out[4]);
+int rcg_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t sizes[256], const float a[64],
+                        int16_t out[64]);
+
+/* ------------------------------------------------------------------------------------------------------------
  * Entropy coding on the host (no device, no context): the stage after the search, which serialises the chosen
  * coefficients. Byte-identical to the reference's writers. ci points to HOST coefficient buffers; 1-4 components.
  * Descriptive metadata (EXIF, XMP, ICC) is not carried by rcg_coef_image and is not written.

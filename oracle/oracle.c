@@ -22,6 +22,9 @@
  *                       round-trip PSNR floors; 4:4:4/odd-chroma colour decode is pinned only by restatement.
  *   orc_ssimulacra2   - the 54 py-ssimulacra2 golden pairs (fixtures/ssim2/expected.csv) at 1e-4, identity = 100.
  *   orc_find_quality  - bisection semantics against brute-force search over monotone score tables.
+ *   orc_build_lengths - Kraft completeness, the length limit, single-symbol and textbook-Huffman optimality.
+ *   orc_rdo_quantize_ac / orc_trellis - the two unit tests of jpeg/src/rdo.rs and brute-force optimality of the
+ *                       trellis choice over every candidate assignment of small blocks.
  */
 #include <math.h>
 #include <stdint.h>
@@ -176,8 +179,10 @@ static int ZZ_T(int k) {
     return (n % 8) * 8 + n / 8;
 }
 
-/* jpeg/src/encode.rs:185-219 (plain branch, no trellis). */
-static void quantize_plane(const uint8_t* plane, size_t stride, int bw, int bh, const float div[64], int16_t* out) {
+/* jpeg/src/encode.rs:185-219. out receives the plain coefficients; scaled (may be NULL) the f32 values
+ * a[k] = flat[f] * div[f] that the trellis branch (encode.rs:206-211) hands to Rdo::quantize_ac. */
+static void quantize_plane(const uint8_t* plane, size_t stride, int bw, int bh, const float div[64], int16_t* out,
+                           float* scaled) {
     int zzt[64];
     for (int k = 0; k < 64; k++) zzt[k] = ZZ_T(k);
 #pragma omp parallel for schedule(dynamic, 4)
@@ -195,6 +200,10 @@ static void quantize_plane(const uint8_t* plane, size_t stride, int bw, int bh, 
                 int f = zzt[k];
                 int32_t r = round_f32(flat[f] * div[f]);
                 dst[k] = (int16_t)(r < -2047 ? -2047 : (r > 2047 ? 2047 : r));
+            }
+            if (scaled) {
+                float* sd = scaled + ((size_t)by * bw + bx) * 64;
+                for (int k = 0; k < 64; k++) sd[k] = flat[zzt[k]] * div[zzt[k]];
             }
         }
     }
@@ -246,8 +255,27 @@ int orc_build_coefs(const uint8_t* pixels, int w, int h, int channels, int quali
 
 /* The same with explicit natural-order tables (quality only picks Auto subsampling); lets the tests feed the
  * tables that a fixture file carries. */
+static int build_impl(const uint8_t* pixels, int w, int h, int channels, int quality, int subsampling,
+                      const uint8_t* qy, const uint8_t* qc, int16_t* out_y, int16_t* out_cb, int16_t* out_cr,
+                      float* sc_y, float* sc_cb, float* sc_cr);
+
 int orc_build_coefs_tables(const uint8_t* pixels, int w, int h, int channels, int quality, int subsampling,
                            const uint8_t* qy, const uint8_t* qc, int16_t* out_y, int16_t* out_cb, int16_t* out_cr) {
+    return build_impl(pixels, w, h, channels, quality, subsampling, qy, qc, out_y, out_cb, out_cr, NULL, NULL, NULL);
+}
+
+/* As orc_build_coefs, also writing the trellis hand-off values (64 f32 per block, same layout). */
+int orc_build_scaled(const uint8_t* pixels, int w, int h, int channels, int quality, int subsampling, int16_t* out_y,
+                     int16_t* out_cb, int16_t* out_cr, float* sc_y, float* sc_cb, float* sc_cr) {
+    uint8_t qy[64], qc[64];
+    orc_scale_table(quality, qy);
+    orc_scale_table(quality, qc);
+    return build_impl(pixels, w, h, channels, quality, subsampling, qy, qc, out_y, out_cb, out_cr, sc_y, sc_cb, sc_cr);
+}
+
+static int build_impl(const uint8_t* pixels, int w, int h, int channels, int quality, int subsampling,
+                      const uint8_t* qy, const uint8_t* qc, int16_t* out_y, int16_t* out_cb, int16_t* out_cr,
+                      float* sc_y, float* sc_cb, float* sc_cr) {
     if (w < 1 || h < 1 || (channels != 1 && channels != 3)) return 1;
     int hmax, vmax;
     orc_sampling(channels, quality, subsampling, &hmax, &vmax);
@@ -263,7 +291,7 @@ int orc_build_coefs_tables(const uint8_t* pixels, int w, int h, int channels, in
             int sr = r < h - 1 ? r : h - 1;
             for (int x = 0; x < pw; x++) yp[(size_t)r * pw + x] = pixels[(size_t)sr * w + (x < w - 1 ? x : w - 1)];
         }
-        quantize_plane(yp, pw, pw / 8, ph / 8, dy, out_y);
+        quantize_plane(yp, pw, pw / 8, ph / 8, dy, out_y, sc_y);
         free(yp);
         return 0;
     }
@@ -304,9 +332,9 @@ int orc_build_coefs_tables(const uint8_t* pixels, int w, int h, int channels, in
                 }
         }
     }
-    quantize_plane(yp, pw, pw / 8, ph / 8, dy, out_y);
-    quantize_plane(cb, cw, cw / 8, ch / 8, dc, out_cb);
-    quantize_plane(cr, cw, cw / 8, ch / 8, dc, out_cr);
+    quantize_plane(yp, pw, pw / 8, ph / 8, dy, out_y, sc_y);
+    quantize_plane(cb, cw, cw / 8, ch / 8, dc, out_cb, sc_cb);
+    quantize_plane(cr, cw, cw / 8, ch / 8, dc, out_cr, sc_cr);
     if (cb != cbp) { free(cb); free(cr); }
     free(yp);
     free(cbp);
@@ -449,6 +477,216 @@ int orc_reconstruct(int w, int h, int ncomp, const int* comp_h, const int* comp_
 #define MIN_SIDE 8
 #define RADIUS 5
 #define STRIPE 96
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Trellis quantisation (jpeg/src/rdo.rs, jpeg/src/encode.rs:222-265,527-537) and the Huffman code lengths it takes
+ * its rate model from (core/src/util/huffman.rs:10-88). */
+
+/* core/src/util/huffman.rs:10-88: length-limited Huffman code lengths. */
+typedef struct { uint32_t f; int i; } FreqSym;
+static int cmp_freqsym(const void* pa, const void* pb) {
+    const FreqSym* a = (const FreqSym*)pa;
+    const FreqSym* b = (const FreqSym*)pb;
+    if (a->f != b->f) return a->f < b->f ? -1 : 1;
+    return a->i > b->i ? -1 : (a->i < b->i ? 1 : 0); /* higher symbol first among equal frequencies */
+}
+
+void orc_build_lengths(const uint32_t* freqs, int nsym, int max_len, uint8_t* lengths) {
+    memset(lengths, 0, (size_t)nsym);
+    FreqSym* syms = (FreqSym*)malloc(sizeof(FreqSym) * (size_t)(nsym > 0 ? nsym : 1));
+    int n = 0;
+    for (int i = 0; i < nsym; i++)
+        if (freqs[i] > 0) { syms[n].f = freqs[i]; syms[n].i = i; n++; }
+    if (n == 0) { free(syms); return; }
+    if (n == 1) { lengths[syms[0].i] = 1; free(syms); return; }
+    qsort(syms, (size_t)n, sizeof(FreqSym), cmp_freqsym);
+    uint64_t* weight = (uint64_t*)calloc((size_t)(2 * n - 1), sizeof(uint64_t));
+    int* parent = (int*)calloc((size_t)(2 * n - 1), sizeof(int));
+    uint32_t* depth = (uint32_t*)calloc((size_t)(2 * n - 1), sizeof(uint32_t));
+    for (int i = 0; i < n; i++) weight[i] = syms[i].f;
+    int leaf = 0, node = n, next = n;
+    while (next < 2 * n - 1) {
+        int pair[2];
+        for (int t = 0; t < 2; t++) {
+            if (leaf < n && (node >= next || weight[leaf] <= weight[node])) pair[t] = leaf++;
+            else pair[t] = node++;
+        }
+        weight[next] = weight[pair[0]] + weight[pair[1]];
+        parent[pair[0]] = next;
+        parent[pair[1]] = next;
+        next++;
+    }
+    for (int i = 2 * n - 3; i >= 0; i--) depth[i] = depth[parent[i]] + 1;
+    uint32_t count[65] = {0};
+    for (int i = 0; i < n; i++) count[depth[i] < (uint32_t)max_len ? depth[i] : (uint32_t)max_len]++;
+    uint64_t total = 0;
+    for (int l = 1; l <= max_len; l++) total += (uint64_t)count[l] << (max_len - l);
+    while (total > ((uint64_t)1 << max_len)) {
+        count[max_len]--;
+        for (int l = max_len - 1; l >= 1; l--)
+            if (count[l] > 0) { count[l]--; count[l + 1] += 2; break; }
+        total--;
+    }
+    int idx = 0;
+    for (int l = max_len; l >= 1; l--)
+        for (uint32_t k = 0; k < count[l]; k++) lengths[syms[idx++].i] = (uint8_t)l;
+    free(weight); free(parent); free(depth); free(syms);
+}
+
+/* jpeg/src/encode.rs:240-265: code sizes of an optimal table (a dummy symbol reserves the all-ones code). */
+static void optimal_sizes(const uint32_t freq[256], uint8_t size[256]) {
+    uint32_t f[257];
+    uint8_t lens[257];
+    memcpy(f, freq, sizeof(uint32_t) * 256);
+    f[256] = 1;
+    orc_build_lengths(f, 257, 16, lens);
+    int any = 0;
+    for (int s = 0; s < 256; s++) { size[s] = lens[s]; any |= lens[s] > 0; }
+    if (!any) size[0] = 1;
+}
+
+/* jpeg/src/encode.rs:337-370,376-396: AC symbol counts of the plain coefficients, per table class. */
+static void ac_counts(int ncomp, const int16_t* const* coefs, const int* comp_h, const int* comp_v,
+                      const int* blocks_w, int mcus_x, int mcus_y, uint32_t ac[2][256]) {
+    memset(ac, 0, sizeof(uint32_t) * 512);
+    for (int my = 0; my < mcus_y; my++)
+        for (int mx = 0; mx < mcus_x; mx++)
+            for (int c = 0; c < ncomp; c++)
+                for (int by = 0; by < comp_v[c]; by++)
+                    for (int bx = 0; bx < comp_h[c]; bx++) {
+                        const size_t b = ((size_t)my * comp_v[c] + by) * blocks_w[c] + (size_t)mx * comp_h[c] + bx;
+                        const int16_t* blk = coefs[c] + b * 64;
+                        const int tbl = c < 1 ? c : 1;
+                        uint32_t run = 0;
+                        for (int k = 1; k < 64; k++) {
+                            if (blk[k] == 0) { run++; continue; }
+                            while (run >= 16) { ac[tbl][0xF0]++; run -= 16; }
+                            int32_t v = blk[k];
+                            uint32_t a = (uint32_t)(v < 0 ? -v : v), cat = 0;
+                            while (a) { cat++; a >>= 1; }
+                            ac[tbl][(run << 4) | cat]++;
+                            run = 0;
+                        }
+                        if (run > 0) ac[tbl][0]++;
+                    }
+}
+
+typedef struct { float q[64], lambda, w[64], len[256]; } Rdo;
+
+static int cmp_float(const void* a, const void* b) {
+    float x = *(const float*)a, y = *(const float*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* jpeg/src/rdo.rs:30-48 (FREQ_WEIGHT_GAMMA = 1). */
+static void rdo_new(Rdo* r, const uint8_t table[64], float kappa, const uint8_t sizes[256]) {
+    for (int k = 0; k < 64; k++) r->q[k] = (float)table[ZIGZAG[k]];
+    float sorted[63];
+    memcpy(sorted, r->q + 1, sizeof(sorted));
+    qsort(sorted, 63, sizeof(float), cmp_float);
+    const float median = sorted[31];
+    for (int s = 0; s < 256; s++) r->len[s] = sizes[s] == 0 ? 18.0f : (float)sizes[s];
+    r->w[0] = 1.0f;
+    for (int k = 1; k < 64; k++) r->w[k] = powf(median / r->q[k], 1.0f);
+    r->lambda = kappa * median * median;
+}
+
+/* jpeg/src/rdo.rs:50-55 */
+static float rdo_rate(const Rdo* r, int run, uint32_t v) {
+    uint32_t size = 0, t = v;
+    while (t) { size++; t >>= 1; }
+    int zrl = run / 16;
+    return (float)zrl * r->len[0xF0] + r->len[((run % 16) << 4) | (int)size] + (float)size;
+}
+
+/* jpeg/src/rdo.rs:57-113 */
+void orc_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t sizes[256], const float a[64],
+                         int16_t out[64]);
+static void rdo_quantize_ac(const Rdo* r, const float a[64], int16_t out[64]) {
+    typedef struct { int pos; uint32_t v; float cost; int prev; } Node;
+    Node nodes[126];
+    int nn = 0;
+    float zd[64];
+    zd[0] = 0.0f;
+    for (int k = 1; k < 64; k++) {
+        float e = a[k] * r->q[k];
+        zd[k] = zd[k - 1] + e * e * r->w[k];
+    }
+    for (int i = 1; i < 64; i++) {
+        float m = fabsf(a[i]);
+        uint32_t v0 = (uint32_t)(m + 0.5f);
+        if (v0 == 0) continue;
+        uint32_t lo = v0 > 1 ? v0 - 1 : v0;
+        for (uint32_t v = lo; v <= v0; v++) {
+            float e = (m - (float)v) * r->q[i];
+            float dnz = e * e * r->w[i];
+            float best = zd[i - 1] + r->lambda * rdo_rate(r, i - 1, v) + dnz;
+            int prev = -1;
+            for (int ni = 0; ni < nn; ni++) {
+                if (nodes[ni].pos >= i) continue;
+                float c = nodes[ni].cost + (zd[i - 1] - zd[nodes[ni].pos]) +
+                          r->lambda * rdo_rate(r, i - nodes[ni].pos - 1, v) + dnz;
+                if (c < best) { best = c; prev = ni; }
+            }
+            nodes[nn].pos = i; nodes[nn].v = v; nodes[nn].cost = best; nodes[nn].prev = prev;
+            nn++;
+        }
+    }
+    float eob = r->lambda * r->len[0];
+    float best_cost = zd[63] + eob;
+    int best_end = -1;
+    for (int ni = 0; ni < nn; ni++) {
+        float c = nodes[ni].cost + (zd[63] - zd[nodes[ni].pos]) + (nodes[ni].pos < 63 ? eob : 0.0f);
+        if (c < best_cost) { best_cost = c; best_end = ni; }
+    }
+    for (int k = 1; k < 64; k++) out[k] = 0;
+    for (int cur = best_end; cur != -1; cur = nodes[cur].prev) {
+        int16_t v = (int16_t)nodes[cur].v;
+        out[nodes[cur].pos] = a[nodes[cur].pos] < 0.0f ? (int16_t)-v : v;
+    }
+}
+
+void orc_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t sizes[256], const float a[64],
+                         int16_t out[64]) {
+    Rdo r;
+    rdo_new(&r, table, kappa, sizes);
+    rdo_quantize_ac(&r, a, out);
+}
+
+/* jpeg/src/encode.rs:527-537 with TRELLIS_KAPPA (encode.rs:42): code lengths of the plain pass, then every block's
+ * DC rounded and clamped and its AC re-quantised. Components: 1 (gray) or 3 (Y, Cb, Cr); class = min(c, 1).
+ * width/height give the MCU grid the code-length statistics walk. */
+int orc_trellis(int width, int height, int ncomp, const int* comp_h, const int* comp_v, const int* blocks_w,
+                const int* blocks_h, const int16_t* const* plain, const float* const* scaled, const uint8_t* qy,
+                const uint8_t* qc, int16_t* const* out) {
+    int hmax = 1, vmax = 1;
+    for (int c = 0; c < ncomp; c++) {
+        if (comp_h[c] > hmax) hmax = comp_h[c];
+        if (comp_v[c] > vmax) vmax = comp_v[c];
+    }
+    const int mcus_x = (width + 8 * hmax - 1) / (8 * hmax), mcus_y = (height + 8 * vmax - 1) / (8 * vmax);
+    uint32_t ac[2][256];
+    ac_counts(ncomp, plain, comp_h, comp_v, blocks_w, mcus_x, mcus_y, ac);
+    uint8_t sizes[2][256];
+    optimal_sizes(ac[0], sizes[0]);
+    optimal_sizes(ac[1], sizes[1]);
+    Rdo r[2];
+    rdo_new(&r[0], qy, 0.02f, sizes[0]);
+    rdo_new(&r[1], qc, 0.02f, sizes[1]);
+    for (int c = 0; c < ncomp; c++) {
+        const Rdo* rr = &r[c < 1 ? c : 1];
+        const long nb = (long)blocks_w[c] * blocks_h[c];
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long b = 0; b < nb; b++) {
+            const float* a = scaled[c] + b * 64;
+            int16_t* o = out[c] + b * 64;
+            int32_t dc = round_f32(a[0]);
+            o[0] = (int16_t)(dc < -2047 ? -2047 : (dc > 2047 ? 2047 : dc));
+            rdo_quantize_ac(rr, a, o);
+        }
+    }
+    return 0;
+}
+
 static const double C2 = 0.0009;
 static const double SIGMA = 1.5;
 static const double M02 = 0.078, M00 = 0.30, M12 = 0.078, M10 = 0.23;

@@ -50,6 +50,12 @@ def lib():
         L.orc_idct_block.argtypes = [P]
         L.orc_round_f32.argtypes = [ctypes.c_float]
         L.orc_round_f32.restype = ctypes.c_int32
+        L.orc_build_scaled.argtypes = [P, I, I, I, I, I, P, P, P, P, P, P]
+        L.orc_build_lengths.argtypes = [P, I, I, P]
+        L.orc_build_lengths.restype = None
+        L.orc_rdo_quantize_ac.argtypes = [P, ctypes.c_float, P, P, P]
+        L.orc_rdo_quantize_ac.restype = None
+        L.orc_trellis.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P]
         L.orc_ssim2_pool.argtypes = [P, I]
         L.orc_ssim2_pool.restype = D
         _LIB = L
@@ -126,6 +132,74 @@ def build_coefs(img, quality, subsampling=S420, tables=None):
     comps = [CoefComponent(1, hmax, vmax, 0, ybw, y), CoefComponent(2, 1, 1, 1, cbw, cb),
              CoefComponent(3, 1, 1, 1, cbw, cr)]
     return CoefImage(w, h, comps, [qt[:64].copy(), qt[64:].copy(), None, None])
+
+
+def build_scaled(img, quality, subsampling=S420):
+    """build_coefs plus the trellis hand-off values: (CoefImage, [float32 array per component]) where each array
+    holds 64 values a[k] = dct[f]*div[f] per block (jpeg/src/encode.rs:206-211), in the coefficients' layout."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape[:2]
+    ch = _channels(img)
+    hmax, vmax = sampling(ch, quality, subsampling)
+    pw = -(-w // (8 * hmax)) * 8 * hmax
+    ph = -(-h // (8 * vmax)) * 8 * vmax
+    ybw, ybh = pw // 8, ph // 8
+    cbw, cbh = pw // hmax // 8, ph // vmax // 8
+    y = np.zeros(ybw * ybh * 64, np.int16)
+    cb = np.zeros(cbw * cbh * 64 if ch == 3 else 64, np.int16)
+    cr = np.zeros_like(cb)
+    sy = np.zeros(len(y), np.float32)
+    scb = np.zeros(len(cb), np.float32)
+    scr = np.zeros(len(cr), np.float32)
+    rc = lib().orc_build_scaled(_ptr(img), w, h, ch, quality, subsampling, _ptr(y), _ptr(cb), _ptr(cr), _ptr(sy),
+                                _ptr(scb), _ptr(scr))
+    if rc != 0:
+        raise ValueError(f"orc_build_scaled failed: {rc}")
+    qy = scale_table(quality)
+    if ch == 1:
+        return CoefImage(w, h, [CoefComponent(1, 1, 1, 0, ybw, y)], [qy, None, None, None]), [sy]
+    comps = [CoefComponent(1, hmax, vmax, 0, ybw, y), CoefComponent(2, 1, 1, 1, cbw, cb),
+             CoefComponent(3, 1, 1, 1, cbw, cr)]
+    return CoefImage(w, h, comps, [qy, scale_table(quality), None, None]), [sy, scb, scr]
+
+
+def build_lengths(freqs, max_len):
+    """core/src/util/huffman.rs:10-88"""
+    f = np.ascontiguousarray(freqs, dtype=np.uint32)
+    out = np.zeros(len(f), np.uint8)
+    lib().orc_build_lengths(_ptr(f), len(f), max_len, _ptr(out))
+    return out
+
+
+def rdo_quantize_ac(table, kappa, sizes, a):
+    """jpeg/src/rdo.rs: Rdo::new(table, kappa, sizes).quantize_ac(a, out); returns out (positions 1..63 set)."""
+    t = np.ascontiguousarray(table, dtype=np.uint8)
+    sz = np.ascontiguousarray(sizes, dtype=np.uint8)
+    av = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.zeros(64, np.int16)
+    lib().orc_rdo_quantize_ac(_ptr(t), ctypes.c_float(kappa), _ptr(sz), _ptr(av), _ptr(out))
+    return out
+
+
+def trellis(ci, scaled):
+    """The trellis pass of build_coefs (jpeg/src/encode.rs:527-537) over the plain coefficients of `ci` and their
+    hand-off values; returns the trellis coefficients per component."""
+    n = len(ci.comps)
+    ch = np.array([c.h for c in ci.comps], np.int32)
+    cv = np.array([c.v for c in ci.comps], np.int32)
+    bw = np.array([c.blocks_w for c in ci.comps], np.int32)
+    bh = np.array([len(c.coefs) // 64 // c.blocks_w for c in ci.comps], np.int32)
+    plain = [np.ascontiguousarray(c.coefs, dtype=np.int16) for c in ci.comps]
+    sc = [np.ascontiguousarray(x, dtype=np.float32) for x in scaled]
+    outs = [np.zeros_like(p) for p in plain]
+    pp = (ctypes.c_void_p * n)(*[x.ctypes.data for x in plain])
+    sp = (ctypes.c_void_p * n)(*[x.ctypes.data for x in sc])
+    op = (ctypes.c_void_p * n)(*[x.ctypes.data for x in outs])
+    qy = np.ascontiguousarray(ci.qtables[0], dtype=np.uint8)
+    qc = np.ascontiguousarray(ci.qtables[1] if ci.qtables[1] is not None else ci.qtables[0], dtype=np.uint8)
+    lib().orc_trellis(ci.width, ci.height, n, _ptr(ch), _ptr(cv), _ptr(bw), _ptr(bh), ctypes.cast(pp, ctypes.c_void_p),
+                      ctypes.cast(sp, ctypes.c_void_p), _ptr(qy), _ptr(qc), ctypes.cast(op, ctypes.c_void_p))
+    return outs
 
 
 def reconstruct(ci):

@@ -155,6 +155,9 @@ def lib():
         L.rcg_free.argtypes = [P]
         L.rcg_free.restype = None
         L.rcg_huffman_lengths.argtypes = [P, ctypes.c_size_t, I, P]
+        L.rcg_build_scaled.argtypes = [P, P, U, U, U, I, I, P, P, P]
+        L.rcg_trellis.argtypes = [ctypes.POINTER(_CoefImage), P, P]
+        L.rcg_rdo_quantize_ac.argtypes = [P, ctypes.c_float, P, P, P]
         L.rcg_profile.argtypes = [P, I]
         L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
         _LIB = L
@@ -270,6 +273,29 @@ class Context:
                  CoefComponent(2, 1, 1, 1, L.blocks_w[1], bufs[1]),
                  CoefComponent(3, 1, 1, 1, L.blocks_w[2], bufs[2])]
         return CoefImage(w, h, comps, [qt[:64].copy(), qt[64:].copy(), None, None])
+
+    def build_scaled(self, img, quality, subsampling=S420):
+        """build_coefs plus the trellis hand-off values (rcg_build_scaled): (CoefImage, [float32 per component])."""
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape[:2]
+        ch = _channels(img)
+        L = coef_layout(w, h, ch, quality, subsampling)
+        bufs = [np.empty(L.blocks_w[c] * L.blocks_h[c] * 64, np.int16) for c in range(L.ncomp)]
+        scs = [np.empty(L.blocks_w[c] * L.blocks_h[c] * 64, np.float32) for c in range(L.ncomp)]
+        ptrs = (ctypes.c_void_p * 3)(*([b.ctypes.data for b in bufs] + [None] * (3 - len(bufs))))
+        sptrs = (ctypes.c_void_p * 3)(*([b.ctypes.data for b in scs] + [None] * (3 - len(scs))))
+        qt = np.zeros(128, np.uint8)
+        _check(lib().rcg_build_scaled(self._h, _ptr(img), w, h, ch, quality, subsampling,
+                                      ctypes.cast(ptrs, ctypes.c_void_p), ctypes.cast(sptrs, ctypes.c_void_p),
+                                      _ptr(qt)))
+        if ch == 1:
+            ci = CoefImage(w, h, [CoefComponent(1, 1, 1, 0, L.blocks_w[0], bufs[0])], [qt[:64].copy(), None, None, None])
+        else:
+            ci = CoefImage(w, h, [CoefComponent(1, L.hmax, L.vmax, 0, L.blocks_w[0], bufs[0]),
+                                  CoefComponent(2, 1, 1, 1, L.blocks_w[1], bufs[1]),
+                                  CoefComponent(3, 1, 1, 1, L.blocks_w[2], bufs[2])],
+                           [qt[:64].copy(), qt[64:].copy(), None, None])
+        return ci, scs
 
     def reconstruct(self, ci):
         keep = []
@@ -444,7 +470,7 @@ def _coef_image_struct(ci, keep, device):
             qa = np.ascontiguousarray(q, dtype=np.uint8)
             keep.append(qa)
             s.qtables[slot] = qa.ctypes.data
-    s.adobe_transform = ci.adobe_transform
+    s.adobe_transform = getattr(ci, "adobe_transform", -1)
     return s
 
 
@@ -569,6 +595,31 @@ def write_jpeg(ci, mode=WRITE_BASELINE):
         return ctypes.string_at(out, n.value)
     finally:
         lib().rcg_free(out)
+
+
+def trellis(ci, scaled):
+    """The host trellis pass (rcg_trellis) over a plain CoefImage and its hand-off values: a CoefImage with the
+    trellis coefficients (jpeg/src/encode.rs:527-537)."""
+    keep = []
+    s = _coef_image_struct(ci, keep, device=False)
+    sc = [np.ascontiguousarray(x, dtype=np.float32) for x in scaled]
+    outs = [np.zeros(len(c.coefs), np.int16) for c in ci.comps]
+    sp = (ctypes.c_void_p * 4)(*([x.ctypes.data for x in sc] + [None] * (4 - len(sc))))
+    op = (ctypes.c_void_p * 4)(*([x.ctypes.data for x in outs] + [None] * (4 - len(outs))))
+    _check(lib().rcg_trellis(ctypes.byref(s), ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(op, ctypes.c_void_p)))
+    comps = [CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, o) for c, o in zip(ci.comps, outs)]
+    return CoefImage(ci.width, ci.height, comps, list(ci.qtables), getattr(ci, "adobe_transform", -1))
+
+
+def rdo_quantize_ac(table, kappa, sizes, a):
+    """rc_jpeg::rdo::Rdo::new(table, kappa, sizes).quantize_ac(a, out) (rcg_rdo_quantize_ac); returns out."""
+    t = np.ascontiguousarray(table, dtype=np.uint8)
+    sz = np.ascontiguousarray(sizes, dtype=np.uint8)
+    av = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.zeros(64, np.int16)
+    _check(lib().rcg_rdo_quantize_ac(t.ctypes.data, ctypes.c_float(kappa), sz.ctypes.data, av.ctypes.data,
+                                     out.ctypes.data))
+    return out
 
 
 def huffman_lengths(freqs, max_len):

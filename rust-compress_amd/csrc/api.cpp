@@ -345,7 +345,7 @@ struct StagedEncode {
 };
 
 StagedEncode stage_encode(Desc& D, const Layout& L, const uint8_t* d_src, int quality, int16_t* const d_coefs[3],
-                          uint8_t* const d_rec[3]) {
+                          uint8_t* const d_rec[3], float* const d_scaled[3] = nullptr) {
     EncCand c;
     std::memset(&c, 0, sizeof(c));
     uint8_t qy[64], qc[64];
@@ -354,6 +354,7 @@ StagedEncode stage_encode(Desc& D, const Layout& L, const uint8_t* d_src, int qu
     for (int k = 0; k < 3; k++) {
         c.coef[k] = d_coefs ? d_coefs[k] : nullptr;
         c.rec[k] = d_rec ? d_rec[k] : nullptr;
+        c.scaled[k] = d_scaled ? d_scaled[k] : nullptr;
     }
     EncJob j = make_enc_job(L, d_src);
     j.ncand = 1;
@@ -367,8 +368,8 @@ StagedEncode stage_encode(Desc& D, const Layout& L, const uint8_t* d_src, int qu
 }
 
 int stage_and_launch_encode(rcg_ctx* ctx, Desc& D, const Layout& L, const uint8_t* d_src, int quality,
-                            int16_t* const d_coefs[3], uint8_t* const d_rec[3]) {
-    const StagedEncode se = stage_encode(D, L, d_src, quality, d_coefs, d_rec);
+                            int16_t* const d_coefs[3], uint8_t* const d_rec[3], float* const d_scaled[3] = nullptr) {
+    const StagedEncode se = stage_encode(D, L, d_src, quality, d_coefs, d_rec, d_scaled);
     RCG_TRY(D.upload(ctx->stream));
     return launch_encode(ctx, se.job, se.begins, 1, se.tiles);
 }
@@ -613,6 +614,38 @@ int rcg_build_coefs(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_
     for (uint32_t c = 0; c < L.ncomp; c++)
         RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
                                ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    if (qtables) quant_tables(quality, qtables, qtables + 64);
+    return RCG_OK;
+}
+
+int rcg_build_scaled(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
+                     int quality, int subsampling, int16_t* coefs[3], float* scaled[3], uint8_t qtables[128]) {
+    if (!ctx || !pixels || !coefs || !scaled) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    const Layout L = make_layout(width, height, channels, quality, subsampling);
+    for (uint32_t c = 0; c < L.ncomp; c++)
+        if (!coefs[c] || !scaled[c]) return RCG_INVALID;
+    const size_t src_bytes = size_t(width) * height * channels;
+    size_t sc_bytes = 0;
+    for (uint32_t c = 0; c < L.ncomp; c++) sc_bytes += Arena::align(L.coef_count(c) * sizeof(float));
+    Call call(ctx);
+    RCG_TRY(call.begin(Arena::align(src_bytes) + coef_bytes(L) + sc_bytes));
+    uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+    int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    float* d_sc[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < L.ncomp; c++) {
+        d_co[c] = static_cast<int16_t*>(ctx->arena.take(L.coef_count(c) * sizeof(int16_t)));
+        d_sc[c] = static_cast<float*>(ctx->arena.take(L.coef_count(c) * sizeof(float)));
+    }
+    RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, d_co, nullptr, d_sc));
+    for (uint32_t c = 0; c < L.ncomp; c++) {
+        RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        RCG_HIP(hipMemcpyAsync(scaled[c], d_sc[c], L.coef_count(c) * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    }
     RCG_HIP(hipStreamSynchronize(ctx->stream));
     if (qtables) quant_tables(quality, qtables, qtables + 64);
     return RCG_OK;

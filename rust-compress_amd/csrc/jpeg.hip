@@ -172,6 +172,21 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
 #pragma unroll
             for (int k = 0; k < 8; k++) dst[k] = make_uint4(pk[4 * k], pk[4 * k + 1], pk[4 * k + 2], pk[4 * k + 3]);
         }
+        float* sp = K.scaled[comp];
+        if (sp) {
+            // The trellis branch of quantize_plane (encode.rs:206-211): the same f32 products, unrounded.
+            float4* dst = reinterpret_cast<float4*>(sp + blk_index * 64);
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                float a[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int n = zigzag(4 * k + j), f = (n % 8) * 8 + n / 8;
+                    a[j] = __fmul_rn(blk[f], div[f]);
+                }
+                dst[k] = make_float4(a[0], a[1], a[2], a[3]);
+            }
+        }
         uint8_t* rp = K.rec[comp];
         if (rp) {
             const float* mult = K.mult[cls];

@@ -18,6 +18,7 @@ constexpr int kEncThreads = 192;   // threads per encode workgroup: one per 8x8 
 struct EncCand {
     int16_t* coef[3];     // zigzag coefficients per component, or null when not wanted
     uint8_t* rec[3];      // reconstructed (IDCT) sample planes per component, or null
+    float* scaled[3];     // trellis hand-off: the f32 values a[k] = dct[f]*div[f] before rounding, zigzag order, or null
     float div[2][64];     // reciprocal divisors, transposed [u*8+v] layout (jpeg/src/encode.rs:175-183)
     float mult[2][64];    // dequantisation multipliers, natural [v*8+u] layout (jpeg/src/decode.rs:1317-1322)
 };

@@ -7,11 +7,13 @@
 // tie-breaking, the same segment order and the same bit packing. This is the stage after the device search: it
 // serialises the coefficients the search chose (rcg_find_quality's coefs, or rcg_build_coefs') into a file.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
+#include <system_error>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -719,6 +721,140 @@ int view(const rcg_coef_image* ci, Coefs& out) {
     return RCG_OK;
 }
 
+// ------------------------------------------------------------------------------------------------------ trellis
+
+// Round to nearest, ties to even, through the magic-constant add of jpeg/src/dct.rs:126-129 (|x| < 2^22).
+inline int32_t round_f32(float x) {
+    const float y = x + 12582912.0f;
+    int32_t bits;
+    std::memcpy(&bits, &y, 4);
+    return bits - 0x4B400000;
+}
+
+// Rate-distortion quantiser of one component class (jpeg/src/rdo.rs:15-114). Every f32 expression keeps the
+// reference's operation order (this file is compiled without FP contraction), so its choices are the reference's.
+struct Rdo {
+    float q[64];       // quantisation step per zigzag position
+    float lambda;
+    float w[64];       // distortion weight per zigzag position
+    float len[256];    // code length of each AC symbol; 18 for a symbol the first pass never used
+
+    Rdo(const uint8_t table[64], float kappa, const uint8_t sizes[256]) {
+        for (int k = 0; k < 64; k++) q[k] = float(table[kZigzag[k]]);
+        float sorted[63];
+        std::memcpy(sorted, q + 1, sizeof(sorted));
+        std::sort(sorted, sorted + 63);
+        const float median = sorted[63 / 2];
+        for (int s = 0; s < 256; s++) len[s] = sizes[s] == 0 ? 18.0f : float(sizes[s]);
+        w[0] = 1.0f;
+        // (median / q)^gamma with gamma = 1 (rdo.rs:12): the quotient itself.
+        for (int k = 1; k < 64; k++) w[k] = median / q[k];
+        lambda = kappa * median * median;
+    }
+
+    float rate(size_t run, uint32_t v) const {
+        const uint32_t size = bit_length(v);
+        const size_t zrl = run / 16;
+        return float(zrl) * len[0xF0] + len[((run % 16) << 4) | size] + float(size);
+    }
+
+    // Positions 1..63 of a (values in quantisation steps); DC is the caller's (rdo.rs:52-113).
+    void quantize_ac(const float a[64], int16_t out[64]) const {
+        struct Node {
+            size_t pos;
+            uint32_t v;
+            float cost;
+            size_t prev;
+        };
+        const size_t START = SIZE_MAX;
+        float zd[64];
+        zd[0] = 0.0f;
+        for (int k = 1; k < 64; k++) {
+            const float e = a[k] * q[k];
+            zd[k] = zd[k - 1] + e * e * w[k];
+        }
+        Node nodes[126];
+        size_t nn = 0;
+        for (size_t i = 1; i < 64; i++) {
+            const float m = std::fabs(a[i]);
+            const uint32_t v0 = uint32_t(m + 0.5f);
+            if (v0 == 0) continue;
+            const uint32_t lo = v0 > 1 ? v0 - 1 : v0;
+            for (uint32_t v = lo; v <= v0; v++) {
+                const float e = (m - float(v)) * q[i];
+                const float dnz = e * e * w[i];
+                float best = zd[i - 1] + lambda * rate(i - 1, v) + dnz;
+                size_t prev = START;
+                for (size_t ni = 0; ni < nn; ni++) {
+                    const Node& n = nodes[ni];
+                    if (n.pos >= i) continue;
+                    const float c = n.cost + (zd[i - 1] - zd[n.pos]) + lambda * rate(i - n.pos - 1, v) + dnz;
+                    if (c < best) {
+                        best = c;
+                        prev = ni;
+                    }
+                }
+                nodes[nn++] = Node{i, v, best, prev};
+            }
+        }
+        const float eob = lambda * len[0];
+        float best_cost = zd[63] + eob;
+        size_t best_end = START;
+        for (size_t ni = 0; ni < nn; ni++) {
+            const Node& n = nodes[ni];
+            const float c = n.cost + (zd[63] - zd[n.pos]) + (n.pos < 63 ? eob : 0.0f);
+            if (c < best_cost) {
+                best_cost = c;
+                best_end = ni;
+            }
+        }
+        for (int k = 1; k < 64; k++) out[k] = 0;
+        for (size_t cur = best_end; cur != START; cur = nodes[cur].prev) {
+            const Node& n = nodes[cur];
+            const int16_t v = int16_t(n.v);
+            out[n.pos] = a[n.pos] < 0.0f ? int16_t(-v) : v;
+        }
+    }
+};
+
+constexpr float kTrellisKappa = 0.02f;  // TRELLIS_KAPPA, jpeg/src/encode.rs:42
+
+// AC code lengths of the plainly quantised coefficients, luma and chroma (encode.rs:222-227): the rate model.
+void ac_code_lengths(const Coefs& ci, uint8_t sizes[2][256]) {
+    Stats st;
+    scan_all(ci, st, false);
+    for (int t = 0; t < 2; t++) {
+        const HuffTable tab = optimal_table(st.ac[t]);
+        std::memcpy(sizes[t], tab.size, 256);
+    }
+}
+
+// The trellis pass of build_coefs (encode.rs:527-537) over the scaled values of the plain pass.
+void trellis(const Coefs& plain, const float* const scaled[4], int16_t* const out[4]) {
+    uint8_t sizes[2][256];
+    ac_code_lengths(plain, sizes);
+    const uint8_t* tabs[2] = {plain.qtables[0], plain.qtables[1] ? plain.qtables[1] : plain.qtables[0]};
+    const Rdo rdo[2] = {Rdo(tabs[0], kTrellisKappa, sizes[0]), Rdo(tabs[1], kTrellisKappa, sizes[1])};
+    for (size_t c = 0; c < plain.comps.size(); c++) {
+        const Comp& k = plain.comps[c];
+        const Rdo& r = rdo[std::min<int>(k.tq, 1)];
+        const size_t nblocks = k.blocks_w * k.blocks_h;
+        const size_t nth = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), nblocks / 256 + 1));
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nth; t++)
+            pool.emplace_back([&, t] {
+                for (size_t b = t; b < nblocks; b += nth) {
+                    const float* a = scaled[c] + b * 64;
+                    int16_t* o = out[c] + b * 64;
+                    const int32_t dc = round_f32(a[0]);
+                    o[0] = int16_t(dc < -2047 ? -2047 : (dc > 2047 ? 2047 : dc));
+                    r.quantize_ac(a, o);
+                }
+            });
+        for (std::thread& th : pool) th.join();
+    }
+}
+
 int hand_over(const Bytes& b, uint8_t** out, size_t* len) {
     uint8_t* p = static_cast<uint8_t*>(std::malloc(b.size()));
     if (!p) return RCG_OOM;
@@ -768,5 +904,34 @@ extern "C" int rcg_huffman_lengths(const uint32_t* freqs, size_t n, int max_len,
     } catch (const std::bad_alloc&) {
         return RCG_OOM;
     }
+    return RCG_OK;
+}
+
+extern "C" int rcg_trellis(const rcg_coef_image* plain, const float* const scaled[4], int16_t* out[4]) {
+    if (!plain || !scaled || !out) return RCG_INVALID;
+    Coefs c;
+    RCG_TRY(view(plain, c));
+    if (c.comps.size() > 3 || !c.qtables[0]) {
+        rcg::set_error("trellis: 1 or 3 components with a luma table in slot 0");
+        return RCG_INVALID;
+    }
+    for (size_t k = 0; k < c.comps.size(); k++)
+        if (!scaled[k] || !out[k]) return RCG_INVALID;
+    try {
+        trellis(c, scaled, out);
+    } catch (const std::bad_alloc&) {
+        return RCG_OOM;
+    } catch (const std::system_error&) {
+        rcg::set_error("trellis: could not start host threads");
+        return RCG_DEVICE;
+    }
+    return RCG_OK;
+}
+
+extern "C" int rcg_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t sizes[256], const float a[64],
+                                   int16_t out[64]) {
+    if (!table || !sizes || !a || !out) return RCG_INVALID;
+    const Rdo r(table, kappa, sizes);
+    r.quantize_ac(a, out);
     return RCG_OK;
 }

@@ -167,10 +167,14 @@ inline int subsampling_code(Subsampling s) {
                                                                                                        : RCG_S420;
 }
 
+// The C ABI's view of a CoefImage (borrows its buffers).
+inline Result<rcg_coef_image> to_c(const CoefImage& ci);
+
 // jpeg/src/encode.rs:426. `src` must be Gray or Rgb at 8 bits (callers flatten first, as encode.rs:416-421 does).
+// With opts.trellis (the reference's default) and quality < 100, the device computes the plain coefficients and the
+// hand-off values and the host runs the reference's trellis pass over them (encode.rs:527-537).
 inline Result<CoefImage> build_coefs(Device& dev, const Image& src, const EncodeOptions& opts) {
-    if (opts.trellis && opts.quality < 100)
-        return Error{Error::Unsupported, "trellis quantisation runs on the CPU; set trellis = false"};
+    const bool trellis = opts.trellis && opts.quality < 100;
     if (src.depth != 8 || (src.color != ColorType::Gray && src.color != ColorType::Rgb))
         return Error{Error::Unsupported, "build_coefs takes flattened 8-bit Gray or Rgb pixels"};
     const uint32_t ch = uint32_t(channels(src.color));
@@ -193,18 +197,43 @@ inline Result<CoefImage> build_coefs(Device& dev, const Image& src, const Encode
     }
     for (uint32_t c = 0; c < L.ncomp; c++) ptrs[c] = ci.comps[c].coefs.data();
     uint8_t qt[128];
-    rc = rcg_build_coefs(dev.raw(), src.data.data(), src.width, src.height, ch, opts.quality,
-                         subsampling_code(opts.subsampling), ptrs, qt);
+    std::vector<std::vector<float>> scaled(trellis ? L.ncomp : 0);
+    if (trellis) {
+        float* sp[3] = {nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            scaled[c].resize(ci.comps[c].coefs.size());
+            sp[c] = scaled[c].data();
+        }
+        rc = rcg_build_scaled(dev.raw(), src.data.data(), src.width, src.height, ch, opts.quality,
+                              subsampling_code(opts.subsampling), ptrs, sp, qt);
+    } else {
+        rc = rcg_build_coefs(dev.raw(), src.data.data(), src.width, src.height, ch, opts.quality,
+                             subsampling_code(opts.subsampling), ptrs, qt);
+    }
     if (rc != RCG_OK) return Error::from_status(rc);
     std::array<uint8_t, 64> a, b;
     std::copy(qt, qt + 64, a.begin());
     std::copy(qt + 64, qt + 128, b.begin());
     ci.qtables[0] = a;
     if (L.ncomp == 3) ci.qtables[1] = b;
+    if (trellis) {
+        // The trellis pass reads the plain coefficients (its rate model) and the hand-off values.
+        CoefImage out = ci;
+        Result<rcg_coef_image> plain = to_c(ci);
+        if (plain.is_err()) return plain.error();
+        const float* sp[4] = {nullptr, nullptr, nullptr, nullptr};
+        int16_t* op[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            sp[c] = scaled[c].data();
+            op[c] = out.comps[c].coefs.data();
+        }
+        rc = rcg_trellis(&plain.value(), sp, op);
+        if (rc != RCG_OK) return Error::from_status(rc);
+        return out;
+    }
     return ci;
 }
 
-// The C ABI's view of a CoefImage (borrows its buffers).
 inline Result<rcg_coef_image> to_c(const CoefImage& ci) {
     rcg_coef_image c{};
     c.width = uint32_t(ci.width);

@@ -122,10 +122,31 @@ int main() {
         Image tiny = textured(7, 7);
         Result<Found> r = find_jpeg_quality(dev, tiny, 70.0, opts(100, Subsampling::S420));
         CHECK(r.is_err() && r.error().kind == Error::Unsupported, "tiny image");
+        std::printf("ok search_rejects_images_under_8x8\n");
+    }
+    {  // trellis on (the reference's default): the device hands the unrounded values to the host trellis. DC stays the
+       // plain DC, every AC is zero or one of its two candidates, and the file shrinks (jpeg/src/rdo.rs:1-7)
+        Image img = textured(200, 136);
         EncodeOptions t = opts(80, Subsampling::S420);
         t.trellis = true;
-        CHECK(build_coefs(dev, textured(16, 16), t).is_err(), "trellis is CPU-only");
-        std::printf("ok search_rejects_images_under_8x8\n");
+        CoefImage tr = build_coefs(dev, img, t).value();
+        CoefImage pl = build_coefs(dev, img, opts(80, Subsampling::S420)).value();
+        size_t changed = 0;
+        for (size_t c = 0; c < tr.comps.size(); c++)
+            for (size_t i = 0; i < tr.comps[c].coefs.size(); i++) {
+                const int a = tr.comps[c].coefs[i], p = pl.comps[c].coefs[i];
+                if (i % 64 == 0) CHECK(a == p, "DC changed");
+                // plain rounds half to even, the trellis's candidates come from round-half-up: allow one step
+                const bool ok = a == 0 || (p == 0 ? std::abs(a) == 1 : ((a > 0) == (p > 0) && std::abs(std::abs(a) - std::abs(p)) <= 1));
+                CHECK(ok, "candidate set: %d vs %d", a, p);
+                changed += a != p;
+            }
+        CHECK(changed > 0, "trellis changed nothing");
+        CHECK(write_jpeg(tr).value().size() < write_jpeg(pl).value().size(), "trellis file not smaller");
+        t.quality = 100;  // the reference skips the trellis at q100 (encode.rs:527)
+        CHECK(build_coefs(dev, img, t).value().comps[0].coefs == build_coefs(dev, img, opts(100, Subsampling::S420)).value().comps[0].coefs,
+              "q100 is plain");
+        std::printf("ok trellis_hand_off\n");
     }
     {  // the search's winner, written: the same file as writing build_coefs at the chosen quality (the CLI's
        // encode_jpeg_pixels, cli/src/pipeline.rs:225-247), and write_best picks one of the two encodings

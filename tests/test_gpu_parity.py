@@ -305,3 +305,19 @@ def test_written_files_from_device_coefficients_are_byte_identical(ctx, rc):
         dec = np.asarray(Image.open(io.BytesIO(rc.write_jpeg(dev, rc.WRITE_BASELINE)))).astype(np.int32)
         d = np.abs(dec - ctx.reconstruct(dev).astype(np.int32))
         assert d.max() <= 3, (w, h, d.max())
+
+
+@pytest.mark.parametrize("w,h,q,subs", [(333, 250, 80, O.S420), (97, 61, 92, O.S444), (640, 480, 55, O.S422),
+                                        (64, 48, 30, O.S420)])
+def test_trellis_hand_off_is_bit_exact(ctx, rc, w, h, q, subs):
+    # The device's hand-off values (a[k] = dct[f]*div[f], unrounded) equal the oracle's bit for bit, so the host
+    # trellis over them gives the reference's trellis coefficients.
+    img = rc.synth_rgb8(500 + w, w, h)
+    dev, dsc = ctx.build_scaled(img, q, subs)
+    ora, osc = O.build_scaled(img, q, subs)
+    for a, b, x, y in zip(dev.comps, ora.comps, dsc, osc):
+        np.testing.assert_array_equal(a.coefs, b.coefs)
+        np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))
+    got = rc.trellis(dev, dsc)
+    for g, want in zip(got.comps, O.trellis(ora, osc)):
+        np.testing.assert_array_equal(g.coefs, want)
