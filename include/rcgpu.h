@@ -292,6 +292,30 @@ int rcg_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t size
                         int16_t out[64]);
 
 /* ------------------------------------------------------------------------------------------------------------
+ * DCT-domain requantisation (JPEG -> JPEG). */
+typedef struct {
+    int16_t* coefs[4];          /* caller buffers, one per component, as large as the source component's */
+    uint8_t qtables[4][64];     /* the new tables by slot, natural order */
+    int present[4];             /* slot holds a table */
+    uint8_t tq[4];              /* each component's new slot */
+} rcg_requantized;
+
+/* Replaces rc_jpeg::encode::CoefImage::requantize(quality, trellis) (jpeg/src/encode.rs:111): every component moves
+ * to the quality's table, never finer than its old one (so q100 changes nothing), by round_f32(c * old/new);
+ * with trellis and a changed table the blocks are re-quantised by the trellis from c * old/new. Host; 1-4
+ * components. Bit-exact with the reference. */
+int rcg_requantize(const rcg_coef_image* ci, int quality, int trellis, rcg_requantized* out);
+
+/* The JPEG -> JPEG target search of cli/src/pipeline.rs:369-386: reference = reconstruct(ci), make(q) =
+ * requantize(ci, q, trellis), metric = SSIMULACRA2, bisection as rcg_find_quality. The source coefficients are
+ * uploaded once; the reference image and its SSIMULACRA2 state are built once; each step requantises and
+ * reconstructs on the device (the requantisation fused into the IDCT) or, with trellis, requantises on the host.
+ * winner (may be NULL) receives requantize(ci, chosen quality, trellis). 1- or 3-component images whose chroma
+ * planes share one geometry with sampling ratios of 1 or 2; otherwise RCG_UNSUPPORTED. */
+int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, double target, int max_quality, int trellis,
+                             rcg_found* out, rcg_requantized* winner);
+
+/* ------------------------------------------------------------------------------------------------------------
  * Entropy coding on the host (no device, no context): the stage after the search, which serialises the chosen
  * coefficients. Byte-identical to the reference's writers. ci points to HOST coefficient buffers; 1-4 components.
  * Descriptive metadata (EXIF, XMP, ICC) is not carried by rcg_coef_image and is not written.

@@ -687,6 +687,73 @@ int orc_trellis(int width, int height, int ncomp, const int* comp_h, const int* 
     return 0;
 }
 
+/* jpeg/src/encode.rs:107-171 -- CoefImage::requantize(quality, trellis). Components c = 0..ncomp-1 with table slot
+ * tq[c] (qtabs[slot] natural order or NULL = all ones), sampling comp_h/comp_v, grid blocks_w x blocks_h; adobe is
+ * the Adobe transform (-1 = none). out[c] receives the coefficients, new_tabs[4][64] the tables by slot,
+ * present[4] which slots hold one, new_tq[c] each component's slot. */
+int orc_requantize(int width, int height, int ncomp, const int* tq, const int* comp_h, const int* comp_v,
+                   const int* blocks_w, const int* blocks_h, const int16_t* const* coefs, const uint8_t* const* qtabs,
+                   int adobe, int quality, int trellis, int16_t* const* out, uint8_t* new_tabs, int* present,
+                   int* new_tq) {
+    uint8_t std_tab[64];
+    orc_scale_table(quality, std_tab); /* luma_table and chroma_table share the base (quality.rs:37-43) */
+    float ratio[4][64];
+    uint8_t tabs[4][64];
+    int changed = 0;
+    memset(present, 0, sizeof(int) * 4);
+    memset(new_tabs, 0, 4 * 64);
+    for (int c = 0; c < ncomp; c++) {
+        uint8_t old[64];
+        if (qtabs[tq[c]]) memcpy(old, qtabs[tq[c]], 64);
+        else memset(old, 1, 64);
+        for (int n = 0; n < 64; n++) {
+            tabs[c][n] = std_tab[n] > old[n] ? std_tab[n] : old[n];
+            if (tabs[c][n] != old[n]) changed = 1;
+        }
+        for (int k = 0; k < 64; k++) ratio[c][k] = (float)old[ZIGZAG[k]] / (float)tabs[c][ZIGZAG[k]];
+        const long n = (long)blocks_w[c] * blocks_h[c] * 64;
+        for (long i = 0; i < n; i++) {
+            int32_t r = round_f32((float)coefs[c][i] * ratio[c][i % 64]);
+            out[c][i] = (int16_t)(r < -2047 ? -2047 : (r > 2047 ? 2047 : r));
+        }
+        int slot = -1;
+        for (int s2 = 0; s2 < 4 && slot < 0; s2++)
+            if (present[s2] && memcmp(new_tabs + s2 * 64, tabs[c], 64) == 0) slot = s2;
+        if (slot < 0) {
+            for (int s2 = 0; s2 < 4 && slot < 0; s2++)
+                if (!present[s2]) slot = s2;
+            present[slot] = 1;
+            memcpy(new_tabs + slot * 64, tabs[c], 64);
+        }
+        new_tq[c] = slot;
+    }
+    (void)adobe;
+    if (!trellis || !changed) return 0;
+    int hmax = 1, vmax = 1;
+    for (int c = 0; c < ncomp; c++) {
+        if (comp_h[c] > hmax) hmax = comp_h[c];
+        if (comp_v[c] > vmax) vmax = comp_v[c];
+    }
+    uint32_t ac[2][256];
+    ac_counts(ncomp, (const int16_t* const*)out, comp_h, comp_v, blocks_w, (width + 8 * hmax - 1) / (8 * hmax),
+              (height + 8 * vmax - 1) / (8 * vmax), ac);
+    uint8_t sizes[2][256];
+    optimal_sizes(ac[0], sizes[0]);
+    optimal_sizes(ac[1], sizes[1]);
+    for (int c = 0; c < ncomp; c++) {
+        Rdo r;
+        rdo_new(&r, tabs[c], 0.02f, sizes[c < 1 ? c : 1]);
+        const long nb = (long)blocks_w[c] * blocks_h[c];
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long b = 0; b < nb; b++) {
+            float a[64];
+            for (int k = 0; k < 64; k++) a[k] = (float)coefs[c][b * 64 + k] * ratio[c][k];
+            rdo_quantize_ac(&r, a, out[c] + b * 64);
+        }
+    }
+    return 0;
+}
+
 static const double C2 = 0.0009;
 static const double SIGMA = 1.5;
 static const double M02 = 0.078, M00 = 0.30, M12 = 0.078, M10 = 0.23;

@@ -56,6 +56,7 @@ def lib():
         L.orc_rdo_quantize_ac.argtypes = [P, ctypes.c_float, P, P, P]
         L.orc_rdo_quantize_ac.restype = None
         L.orc_trellis.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P]
+        L.orc_requantize.argtypes = [I, I, I, P, P, P, P, P, P, P, I, I, I, P, P, P, P]
         L.orc_ssim2_pool.argtypes = [P, I]
         L.orc_ssim2_pool.restype = D
         _LIB = L
@@ -200,6 +201,31 @@ def trellis(ci, scaled):
     lib().orc_trellis(ci.width, ci.height, n, _ptr(ch), _ptr(cv), _ptr(bw), _ptr(bh), ctypes.cast(pp, ctypes.c_void_p),
                       ctypes.cast(sp, ctypes.c_void_p), _ptr(qy), _ptr(qc), ctypes.cast(op, ctypes.c_void_p))
     return outs
+
+
+def requantize(ci, quality, trellis=False, adobe=-1):
+    """jpeg/src/encode.rs:107-171: CoefImage::requantize(quality, trellis); returns a new CoefImage."""
+    n = len(ci.comps)
+    i32 = lambda v: np.array(v, np.int32)
+    tq = i32([c.tq for c in ci.comps])
+    ch, cv = i32([c.h for c in ci.comps]), i32([c.v for c in ci.comps])
+    bw = i32([c.blocks_w for c in ci.comps])
+    bh = i32([len(c.coefs) // 64 // c.blocks_w for c in ci.comps])
+    src = [np.ascontiguousarray(c.coefs, dtype=np.int16) for c in ci.comps]
+    outs = [np.zeros_like(x) for x in src]
+    qts = [None if q is None else np.ascontiguousarray(q, dtype=np.uint8) for q in list(ci.qtables) + [None] * 4][:4]
+    sp = (ctypes.c_void_p * n)(*[x.ctypes.data for x in src])
+    op = (ctypes.c_void_p * n)(*[x.ctypes.data for x in outs])
+    qp = (ctypes.c_void_p * 4)(*[None if q is None else q.ctypes.data for q in qts])
+    tabs = np.zeros((4, 64), np.uint8)
+    present = np.zeros(4, np.int32)
+    new_tq = np.zeros(n, np.int32)
+    lib().orc_requantize(ci.width, ci.height, n, _ptr(tq), _ptr(ch), _ptr(cv), _ptr(bw), _ptr(bh),
+                         ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(qp, ctypes.c_void_p), adobe, quality,
+                         1 if trellis else 0, ctypes.cast(op, ctypes.c_void_p), _ptr(tabs), _ptr(present),
+                         _ptr(new_tq))
+    comps = [CoefComponent(c.id, c.h, c.v, int(t), c.blocks_w, o) for c, t, o in zip(ci.comps, new_tq, outs)]
+    return CoefImage(ci.width, ci.height, comps, [tabs[s].copy() if present[s] else None for s in range(4)])
 
 
 def reconstruct(ci):

@@ -57,6 +57,11 @@ class _Found(ctypes.Structure):
 MAX_CAND = 16
 
 
+class _Requantized(ctypes.Structure):
+    _fields_ = [("coefs", ctypes.c_void_p * 4), ("qtables", (ctypes.c_uint8 * 64) * 4), ("present", ctypes.c_int * 4),
+                ("tq", ctypes.c_uint8 * 4)]
+
+
 class BatchJob(ctypes.Structure):
     """rcg_batch_job: one image and its candidate (quality, subsampling) list."""
     _fields_ = [("pixels", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
@@ -158,6 +163,9 @@ def lib():
         L.rcg_build_scaled.argtypes = [P, P, U, U, U, I, I, P, P, P]
         L.rcg_trellis.argtypes = [ctypes.POINTER(_CoefImage), P, P]
         L.rcg_rdo_quantize_ac.argtypes = [P, ctypes.c_float, P, P, P]
+        L.rcg_requantize.argtypes = [ctypes.POINTER(_CoefImage), I, I, ctypes.POINTER(_Requantized)]
+        L.rcg_requant_find_quality.argtypes = [P, ctypes.POINTER(_CoefImage), D, I, I, ctypes.POINTER(_Found),
+                                               ctypes.POINTER(_Requantized)]
         L.rcg_profile.argtypes = [P, I]
         L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
         _LIB = L
@@ -296,6 +304,16 @@ class Context:
                                   CoefComponent(3, 1, 1, 1, L.blocks_w[2], bufs[2])],
                            [qt[:64].copy(), qt[64:].copy(), None, None])
         return ci, scs
+
+    def requant_find_quality(self, ci, target, max_quality=100, trellis=False):
+        """The JPEG -> JPEG target search (rcg_requant_find_quality): Found with the winner's CoefImage."""
+        keep = []
+        s = _coef_image_struct(ci, keep, device=False)
+        r, outs = _requantized(ci)
+        f = _Found()
+        _check(lib().rcg_requant_find_quality(self._h, ctypes.byref(s), target, max_quality, 1 if trellis else 0,
+                                              ctypes.byref(f), ctypes.byref(r)))
+        return Found(f.quality, f.score, _from_requantized(ci, r, outs), f.steps)
 
     def reconstruct(self, ci):
         keep = []
@@ -609,6 +627,29 @@ def trellis(ci, scaled):
     _check(lib().rcg_trellis(ctypes.byref(s), ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(op, ctypes.c_void_p)))
     comps = [CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, o) for c, o in zip(ci.comps, outs)]
     return CoefImage(ci.width, ci.height, comps, list(ci.qtables), getattr(ci, "adobe_transform", -1))
+
+
+def _requantized(ci):
+    r = _Requantized()
+    outs = [np.zeros(len(c.coefs), np.int16) for c in ci.comps]
+    for i, o in enumerate(outs):
+        r.coefs[i] = o.ctypes.data
+    return r, outs
+
+
+def _from_requantized(ci, r, outs):
+    comps = [CoefComponent(c.id, c.h, c.v, int(r.tq[i]), c.blocks_w, o) for i, (c, o) in enumerate(zip(ci.comps, outs))]
+    tabs = [np.frombuffer(bytes(r.qtables[s]), np.uint8).copy() if r.present[s] else None for s in range(4)]
+    return CoefImage(ci.width, ci.height, comps, tabs, getattr(ci, "adobe_transform", -1))
+
+
+def requantize(ci, quality, trellis=False):
+    """rc_jpeg::encode::CoefImage::requantize(quality, trellis) on the host (rcg_requantize): a new CoefImage."""
+    keep = []
+    s = _coef_image_struct(ci, keep, device=False)
+    r, outs = _requantized(ci)
+    _check(lib().rcg_requantize(ctypes.byref(s), quality, 1 if trellis else 0, ctypes.byref(r)))
+    return _from_requantized(ci, r, outs)
 
 
 def rdo_quantize_ac(table, kappa, sizes, a):

@@ -700,7 +700,7 @@ void mult_table(const uint8_t q[64], float mult[64]) {
 // IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device.
 int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[3], uint8_t* d_out,
                           uint32_t hmax, uint32_t vmax, int rgb_direct) {
-    IdctPlane planes[3];
+    IdctPlane planes[3] = {};
     PlaneView views[3];
     for (uint32_t c = 0; c < ci->ncomp; c++) {
         const rcg_coef_component& k = ci->comps[c];
@@ -949,6 +949,196 @@ int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, 
 }
 
 }  // extern "C"
+
+// ================================================================================================================
+// JPEG -> JPEG: the DCT-domain requantisation search (cli/src/pipeline.rs:369-386)
+
+namespace {
+
+// One requantised candidate scored against the decoded source: its sample planes (requantised on the device inside
+// the IDCT, or uploaded after the host trellis), then SSIMULACRA2 with the FP64 re-score inside the guard band.
+int score_requant(rcg_ctx* ctx, Call& call, const rcg_coef_image* ci, const Layout& L, const int16_t* const d_src[3],
+                  const uint8_t* d_ref, uint32_t ref_ch, uint8_t* const planes[3], int16_t* const d_tr[3],
+                  const std::vector<std::vector<int16_t>>& host_src, int quality, bool trellis, const SsimBuffers& B32,
+                  const SsimBuffers& B64, bool& a32_ready, bool& a64_ready, double target, double& score) {
+    RequantPlan P;
+    RCG_TRY(requant_plan(ci, quality, P));
+    IdctPlane ip[3] = {};
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const rcg_coef_component& k = ci->comps[c];
+        ip[c].blocks_w = k.blocks_w;
+        ip[c].blocks_h = k.blocks_h;
+        ip[c].plane = planes[c];
+        mult_table(P.table[c], ip[c].mult);
+    }
+    const bool host_trellis = trellis && P.changed;
+    if (host_trellis) {
+        // The trellis runs on the host from the source coefficients (encode.rs:146-169); its output is uploaded.
+        std::vector<std::vector<int16_t>> rq(ci->ncomp);
+        rcg_requantized out;
+        std::memset(&out, 0, sizeof(out));
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            rq[c].resize(host_src[c].size());
+            out.coefs[c] = rq[c].data();
+        }
+        RCG_TRY(rcg_requantize(ci, quality, 1, &out));
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            RCG_HIP(hipMemcpyAsync(d_tr[c], rq[c].data(), rq[c].size() * sizeof(int16_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            ip[c].coefs = d_tr[c];
+        }
+        RCG_HIP(hipStreamSynchronize(ctx->stream));  // rq leaves scope
+    } else {
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            ip[c].coefs = d_src[c];
+            ip[c].requant = 1;
+            std::memcpy(ip[c].ratio, P.ratio[c], sizeof(ip[c].ratio));
+        }
+    }
+    uint8_t* rec[3] = {planes[0], planes[1], planes[2]};
+    const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
+    call.D.off = 0;
+    StagedSsim st = stage_ssim(call.D, P32, bytes_src(d_ref, ref_ch), recon_src(L, rec), B32, a32_ready,
+                               RCG_PREC_FP32);
+    RCG_TRY(call.D.upload(ctx->stream));
+    if (st.build_ref) RCG_TRY(fork_side(ctx));  // the reference state beside the IDCT, on the first step
+    RCG_TRY(launch_idct(ctx, ip, int(ci->ncomp)));
+    RCG_TRY(launch_staged_refs(ctx, st));
+    RCG_TRY(launch_staged_ssim(ctx, st));
+    double s = 0.0;
+    RCG_HIP(hipMemcpyAsync(&s, B32.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    a32_ready = true;
+    if (std::isfinite(s) && std::fabs(s - target) < kGuard) {
+        call.D.off = 0;
+        const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
+        StagedSsim s64 = stage_ssim(call.D, P64, bytes_src(d_ref, ref_ch), recon_src(L, rec), B64, a64_ready,
+                                    RCG_PREC_FP64);
+        RCG_TRY(call.D.upload(ctx->stream));
+        RCG_TRY(launch_staged_ssim(ctx, s64));
+        RCG_HIP(hipMemcpyAsync(&s, B64.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        a64_ready = true;
+    }
+    score = std::isfinite(s) ? s : -DBL_MAX;
+    return RCG_OK;
+}
+
+}  // namespace
+
+extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, double target, int max_quality,
+                                        int trellis, rcg_found* out, rcg_requantized* winner) {
+    if (!ctx || !ci || !out) return RCG_INVALID;
+    uint32_t hmax, vmax;
+    int rgb_direct;
+    RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
+    // The scorer reads the reconstruction through the fused upsampling loader: luma at full resolution, chroma
+    // planes of one geometry, ratios 1 or 2, YCbCr colour.
+    const rcg_coef_component* k = ci->comps;
+    bool ok = !rgb_direct && k[0].h == hmax && k[0].v == vmax && hmax <= 2 && vmax <= 2;
+    if (ci->ncomp == 3)
+        ok = ok && k[1].h == 1 && k[1].v == 1 && k[2].h == 1 && k[2].v == 1 && k[1].blocks_w == k[2].blocks_w &&
+             k[1].blocks_h == k[2].blocks_h;
+    if (!ok) {
+        set_error("requantisation search: this sampling layout is not supported");
+        return RCG_UNSUPPORTED;
+    }
+    if (ci->width < 8 || ci->height < 8) {
+        set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
+        return RCG_UNSUPPORTED;
+    }
+    Layout L;
+    std::memset(&L, 0, sizeof(L));
+    L.w = ci->width;
+    L.h = ci->height;
+    L.channels = ci->ncomp == 1 ? 1 : 3;
+    L.hmax = hmax;
+    L.vmax = vmax;
+    L.ncomp = ci->ncomp;
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        L.bw[c] = ci->comps[c].blocks_w;
+        L.bh[c] = ci->comps[c].blocks_h;
+    }
+    const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
+    const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
+    const size_t ref_bytes = size_t(L.w) * L.h * L.channels;
+    size_t need = Arena::align(ref_bytes) + ssim_bytes(P32, RCG_PREC_FP32) + ssim_bytes(P64, RCG_PREC_FP64);
+    std::vector<std::vector<int16_t>> host_src(ci->ncomp);
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+        need += 2 * Arena::align(nb * 64 * sizeof(int16_t)) + 2 * Arena::align(nb * 64);
+        if (trellis) host_src[c].assign(ci->comps[c].coefs, ci->comps[c].coefs + nb * 64);
+    }
+    Call call(ctx);
+    RCG_TRY(call.begin(need));
+    int16_t* d_src[3] = {nullptr, nullptr, nullptr};
+    int16_t* d_tr[3] = {nullptr, nullptr, nullptr};
+    uint8_t* planes[3] = {nullptr, nullptr, nullptr};
+    uint8_t* src_planes[3] = {nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+        d_src[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
+        d_tr[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
+        planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
+        src_planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
+        RCG_HIP(hipMemcpyAsync(d_src[c], ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+    }
+    uint8_t* d_ref = static_cast<uint8_t*>(ctx->arena.take(ref_bytes));
+    const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
+    const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
+    // reference = reconstruct(&ci) (pipeline.rs:373), decoded once on the device.
+    {
+        IdctPlane ip[3] = {};
+        PlaneView views[3];
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            const rcg_coef_component& kc = ci->comps[c];
+            ip[c].coefs = d_src[c];
+            ip[c].blocks_w = kc.blocks_w;
+            ip[c].blocks_h = kc.blocks_h;
+            ip[c].plane = src_planes[c];
+            mult_table(ci->qtables[kc.tq], ip[c].mult);
+            views[c].p = src_planes[c];
+            views[c].stride = kc.blocks_w * 8;
+            views[c].cw = (ci->width * kc.h + hmax - 1) / hmax;
+            views[c].ch = (ci->height * kc.v + vmax - 1) / vmax;
+            views[c].fx = hmax / kc.h;
+            views[c].fy = vmax / kc.v;
+        }
+        RCG_TRY(launch_idct(ctx, ip, int(ci->ncomp)));
+        RCG_TRY(launch_upsample_color(ctx, views, int(ci->ncomp), 0, ci->width, ci->height, d_ref));
+    }
+    bool a32 = false, a64 = false;
+    // jpeg/src/target.rs:37-55
+    int lo = 1, hi = std::max(1, std::min(100, max_quality));
+    int steps = 0;
+    double s = 0.0;
+    RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, hi, trellis != 0, B32,
+                          B64, a32, a64, target, s));
+    steps++;
+    int best_q = hi;
+    double best_s = s;
+    if (s >= target) {
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, mid,
+                                  trellis != 0, B32, B64, a32, a64, target, s));
+            steps++;
+            if (s >= target) {
+                hi = mid;
+                best_q = mid;
+                best_s = s;
+            } else {
+                lo = mid + 1;
+            }
+        }
+    }
+    out->quality = best_q;
+    out->score = best_s;
+    out->steps = steps;
+    if (winner) RCG_TRY(rcg_requantize(ci, best_q, trellis, winner));
+    return RCG_OK;
+}
 
 // ================================================================================================================
 // Tile policy and autotuning

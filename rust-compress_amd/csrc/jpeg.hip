@@ -220,6 +220,9 @@ struct IdctComp {
     uint8_t* plane;          // stride blocks_w*8
     uint32_t blocks_w, blocks_h;
     float mult[64];          // natural order
+    int requant;             // requantise first (jpeg/src/encode.rs:130-136)
+    float ratio[64];         // old/new step per zigzag position
+    int16_t* requant_out;    // the requantised coefficients, or null
 };
 struct IdctArgs {
     IdctComp c[4];
@@ -240,6 +243,22 @@ __global__ __launch_bounds__(256) void k_idct(IdctArgs args) {
         pk[4 * k + 1] = v.y;
         pk[4 * k + 2] = v.z;
         pk[4 * k + 3] = v.w;
+    }
+    if (C.requant) {
+        // DCT-domain requantisation: round_f32(c * old/new), clamped (encode.rs:130-136).
+#pragma unroll
+        for (int k = 0; k < 64; k++) {
+            const int16_t q = int16_t((pk[k / 2] >> (16 * (k & 1))) & 0xFFFF);
+            int32_t r = round_f32(__fmul_rn(float(q), C.ratio[k]));
+            r = r < -2047 ? -2047 : (r > 2047 ? 2047 : r);
+            const uint32_t sh = 16 * (k & 1);
+            pk[k / 2] = (pk[k / 2] & ~(0xFFFFu << sh)) | ((uint32_t(r) & 0xFFFFu) << sh);
+        }
+        if (C.requant_out) {
+            uint4* dst = reinterpret_cast<uint4*>(C.requant_out + uint64_t(b) * 64);
+#pragma unroll
+            for (int k = 0; k < 8; k++) dst[k] = make_uint4(pk[4 * k], pk[4 * k + 1], pk[4 * k + 2], pk[4 * k + 3]);
+        }
     }
     float blk[64];
 #pragma unroll
@@ -321,6 +340,9 @@ int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n) {
         a.c[i].blocks_w = planes[i].blocks_w;
         a.c[i].blocks_h = planes[i].blocks_h;
         std::memcpy(a.c[i].mult, planes[i].mult, sizeof(a.c[i].mult));
+        a.c[i].requant = planes[i].requant;
+        std::memcpy(a.c[i].ratio, planes[i].ratio, sizeof(a.c[i].ratio));
+        a.c[i].requant_out = planes[i].requant_out;
         maxb = std::max(maxb, planes[i].blocks_w * planes[i].blocks_h);
     }
     if (maxb == 0) return RCG_OK;

@@ -136,6 +136,12 @@ struct IdctPlane {
     uint8_t* plane;
     uint32_t blocks_w, blocks_h;
     float mult[64];
+    // Requantisation fused ahead of the dequantisation (when requant): each coefficient becomes
+    // clamp(round_f32(c * ratio[k]), +-2047) (k: zigzag position), is written to requant_out if not null, and is
+    // dequantised with mult (the new table's).
+    int requant;
+    float ratio[64];
+    int16_t* requant_out;
 };
 int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
@@ -170,6 +176,20 @@ int launch_finalize(rcg_ctx* ctx, const FinalDesc* d_jobs, int n);
 int upload_weights();
 // Resident workgroups of the reference-cache kernel on `device` (256- and 128-thread instances).
 int ref_kernel_slots(int device, uint32_t* wide, uint32_t* narrow);
+
+// DCT-domain requantisation of a coefficient image towards the tables of `quality` (jpeg/src/encode.rs:107-145):
+// per component the new table (the quality's, never finer than the old one), the old/new ratio per zigzag position
+// and the new table slot (components with equal tables share one).
+struct RequantPlan {
+    uint32_t ncomp;
+    uint8_t table[4][64];   // natural order, per component
+    float ratio[4][64];     // zigzag order, per component
+    uint8_t tq[4];          // new slot per component
+    uint8_t slots[4][64];   // new tables by slot
+    int present[4];
+    bool changed;           // some component's table changed
+};
+int requant_plan(const rcg_coef_image* ci, int quality, RequantPlan& P);
 
 // Tiles of the SSIMULACRA2 scale kernel for a w x h scale.
 inline uint32_t ssim_tiles_x(uint32_t w, int nt) { return (w + (nt - 12) - 1) / (nt - 12); }

@@ -1,7 +1,10 @@
-// JPEG entropy coding of a coefficient image on the host: baseline with optimised Huffman tables
-// (jpeg/src/encode.rs:240-413,568-673), progressive with a planned scan script (jpeg/src/progressive.rs), and the
-// smaller of the two (write_best, encode.rs:543-565). Canonical length-limited Huffman code lengths follow
-// core/src/util/huffman.rs:10-88.
+// The host side of the JPEG path, after and around the device search:
+//   - entropy coding of a coefficient image: baseline with optimised Huffman tables (jpeg/src/encode.rs:240-413,
+//     568-673), progressive with a planned scan script (jpeg/src/progressive.rs), and the smaller of the two
+//     (write_best, encode.rs:543-565); canonical length-limited Huffman code lengths (core/src/util/huffman.rs:10-88);
+//   - the trellis quantiser (jpeg/src/rdo.rs) and the trellis pass of build_coefs (encode.rs:527-537) over the
+//     device's hand-off values;
+//   - DCT-domain requantisation (CoefImage::requantize, encode.rs:107-171).
 //
 // The output is byte-identical to the reference's: the same symbol statistics, the same code-length construction and
 // tie-breaking, the same segment order and the same bit packing. This is the stage after the device search: it
@@ -855,6 +858,44 @@ void trellis(const Coefs& plain, const float* const scaled[4], int16_t* const ou
     }
 }
 
+// CoefImage::requantize (encode.rs:111-171) on the host: the plain requantisation of every block and, with trellis
+// and a changed table, the trellis re-quantisation of every block from its exact source values a = c * ratio.
+void requantize_host(const Coefs& src, const rcg::RequantPlan& P, bool use_trellis, int16_t* const out[4]) {
+    for (size_t c = 0; c < src.comps.size(); c++) {
+        const Comp& k = src.comps[c];
+        const size_t n = k.blocks_w * k.blocks_h * 64;
+        for (size_t i = 0; i < n; i++) {
+            const int32_t r = round_f32(float(k.coefs[i]) * P.ratio[c][i % 64]);
+            out[c][i] = int16_t(r < -2047 ? -2047 : (r > 2047 ? 2047 : r));
+        }
+    }
+    if (!use_trellis || !P.changed) return;
+    // The rate model: AC code lengths of the plainly requantised image, per table class min(c, 1).
+    Coefs plain = src;
+    for (size_t c = 0; c < plain.comps.size(); c++) {
+        plain.comps[c].coefs = out[c];
+        plain.comps[c].tq = P.tq[c];
+    }
+    uint8_t sizes[2][256];
+    ac_code_lengths(plain, sizes);
+    for (size_t c = 0; c < src.comps.size(); c++) {
+        const Rdo rdo(P.table[c], kTrellisKappa, sizes[std::min<size_t>(c, 1)]);
+        const Comp& k = src.comps[c];
+        const size_t nblocks = k.blocks_w * k.blocks_h;
+        const size_t nth = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), nblocks / 256 + 1));
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nth; t++)
+            pool.emplace_back([&, t, c] {
+                for (size_t b = t; b < nblocks; b += nth) {
+                    float a[64];
+                    for (int i = 0; i < 64; i++) a[i] = float(k.coefs[b * 64 + i]) * P.ratio[c][i];
+                    rdo.quantize_ac(a, out[c] + b * 64);
+                }
+            });
+        for (std::thread& th : pool) th.join();
+    }
+}
+
 int hand_over(const Bytes& b, uint8_t** out, size_t* len) {
     uint8_t* p = static_cast<uint8_t*>(std::malloc(b.size()));
     if (!p) return RCG_OOM;
@@ -933,5 +974,86 @@ extern "C" int rcg_rdo_quantize_ac(const uint8_t table[64], float kappa, const u
     if (!table || !sizes || !a || !out) return RCG_INVALID;
     const Rdo r(table, kappa, sizes);
     r.quantize_ac(a, out);
+    return RCG_OK;
+}
+
+namespace rcg {
+
+int requant_plan(const rcg_coef_image* ci, int quality, RequantPlan& P) {
+    std::memset(&P, 0, sizeof(P));
+    if (!ci || ci->ncomp < 1 || ci->ncomp > 4 || quality < 1 || quality > 100) {
+        set_error("requantize: 1-4 components and a quality of 1..100");
+        return RCG_INVALID;
+    }
+    P.ncomp = ci->ncomp;
+    uint8_t std_tab[2][64];
+    quant_tables(quality, std_tab[0], std_tab[1]);
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        const rcg_coef_component& k = ci->comps[c];
+        if (k.tq > 3) {
+            set_error("requantize: table slot out of range");
+            return RCG_INVALID;
+        }
+        // Table class (encode.rs:96-104): in a 4-component file only YCCK's colour-difference channels are chroma.
+        const int cls = ci->ncomp == 4 ? int(ci->adobe_transform == 2 && (c == 1 || c == 2)) : int(std::min(c, 1u));
+        uint8_t old[64];
+        if (ci->qtables[k.tq])
+            std::memcpy(old, ci->qtables[k.tq], 64);
+        else
+            std::memset(old, 1, 64);
+        bool differs = false;
+        for (int n = 0; n < 64; n++) {
+            P.table[c][n] = std::max(std_tab[cls][n], old[n]);
+            differs |= P.table[c][n] != old[n];
+        }
+        P.changed |= differs;
+        for (int z = 0; z < 64; z++) P.ratio[c][z] = float(old[kZigzag[z]]) / float(P.table[c][kZigzag[z]]);
+        // Components with the same table share a slot; a different table takes the next free one.
+        int slot = -1;
+        for (int s = 0; s < 4 && slot < 0; s++)
+            if (P.present[s] && std::memcmp(P.slots[s], P.table[c], 64) == 0) slot = s;
+        if (slot < 0) {
+            for (int s = 0; s < 4 && slot < 0; s++)
+                if (!P.present[s]) slot = s;
+            P.present[slot] = 1;
+            std::memcpy(P.slots[slot], P.table[c], 64);
+        }
+        P.tq[c] = uint8_t(slot);
+    }
+    return RCG_OK;
+}
+
+}  // namespace rcg
+
+extern "C" int rcg_requantize(const rcg_coef_image* ci, int quality, int trellis_on, rcg_requantized* out) {
+    if (!ci || !out) return RCG_INVALID;
+    rcg::RequantPlan P;
+    RCG_TRY(rcg::requant_plan(ci, quality, P));
+    // The coefficient grid and sampling checks of the writer's view, without requiring the tables it will replace.
+    rcg_coef_image probe = *ci;
+    uint8_t ones[64];
+    std::memset(ones, 1, 64);
+    for (int s = 0; s < 4; s++)
+        if (!probe.qtables[s]) probe.qtables[s] = ones;
+    Coefs c;
+    RCG_TRY(view(&probe, c));
+    int16_t* outs[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (uint32_t k = 0; k < ci->ncomp; k++) {
+        if (!out->coefs[k]) return RCG_INVALID;
+        outs[k] = out->coefs[k];
+    }
+    try {
+        requantize_host(c, P, trellis_on != 0, outs);
+    } catch (const std::bad_alloc&) {
+        return RCG_OOM;
+    } catch (const std::system_error&) {
+        rcg::set_error("requantize: could not start host threads");
+        return RCG_DEVICE;
+    }
+    for (int s = 0; s < 4; s++) {
+        out->present[s] = P.present[s];
+        std::memcpy(out->qtables[s], P.slots[s], 64);
+    }
+    for (uint32_t k = 0; k < 4; k++) out->tq[k] = k < ci->ncomp ? P.tq[k] : 0;
     return RCG_OK;
 }

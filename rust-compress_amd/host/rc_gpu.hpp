@@ -276,6 +276,67 @@ inline Result<std::vector<uint8_t>> write_progressive(const CoefImage& ci) {
 }
 inline Result<std::vector<uint8_t>> write_best(const CoefImage& ci) { return write_with(ci, RCG_WRITE_BEST); }
 
+// jpeg/src/encode.rs:111: CoefImage::requantize(quality, trellis), in place (host).
+inline Result<bool> requantize(CoefImage& ci, uint8_t quality, bool trellis) {
+    Result<rcg_coef_image> c = to_c(ci);
+    if (c.is_err()) return c.error();
+    std::vector<std::vector<int16_t>> outs(ci.comps.size());
+    rcg_requantized r{};
+    for (size_t i = 0; i < ci.comps.size(); i++) {
+        outs[i].resize(ci.comps[i].coefs.size());
+        r.coefs[i] = outs[i].data();
+    }
+    const int rc = rcg_requantize(&c.value(), quality, trellis ? 1 : 0, &r);
+    if (rc != RCG_OK) return Error::from_status(rc);
+    for (size_t i = 0; i < ci.comps.size(); i++) {
+        ci.comps[i].coefs = std::move(outs[i]);
+        ci.comps[i].tq = r.tq[i];
+    }
+    for (int s = 0; s < 4; s++) {
+        if (r.present[s]) {
+            std::array<uint8_t, 64> t;
+            std::copy(r.qtables[s], r.qtables[s] + 64, t.begin());
+            ci.qtables[s] = t;
+        } else {
+            ci.qtables[s].reset();
+        }
+    }
+    return true;
+}
+
+// The JPEG -> JPEG target search of cli/src/pipeline.rs:369-386 on the device (rcg_requant_find_quality).
+inline Result<Found> find_jpeg_quality_requant(Device& dev, const CoefImage& ci, double target, bool trellis) {
+    Result<rcg_coef_image> c = to_c(ci);
+    if (c.is_err()) return c.error();
+    std::vector<std::vector<int16_t>> outs(ci.comps.size());
+    rcg_requantized r{};
+    for (size_t i = 0; i < ci.comps.size(); i++) {
+        outs[i].resize(ci.comps[i].coefs.size());
+        r.coefs[i] = outs[i].data();
+    }
+    rcg_found f{};
+    const int rc = rcg_requant_find_quality(dev.raw(), &c.value(), target, 100, trellis ? 1 : 0, &f, &r);
+    if (rc != RCG_OK) return Error::from_status(rc);
+    Found out;
+    out.quality = uint8_t(f.quality);
+    out.score = f.score;
+    out.coefs = ci;
+    for (size_t i = 0; i < ci.comps.size(); i++) {
+        out.coefs.comps[i].coefs = std::move(outs[i]);
+        out.coefs.comps[i].tq = r.tq[i];
+    }
+    for (int s = 0; s < 4; s++) {
+        if (r.present[s]) {
+            std::array<uint8_t, 64> t;
+            std::copy(r.qtables[s], r.qtables[s] + 64, t.begin());
+            out.coefs.qtables[s] = t;
+        } else {
+            out.coefs.qtables[s].reset();
+        }
+    }
+    return out;
+}
+
 // jpeg/src/decode.rs:334
 inline Result<Image> reconstruct(Device& dev, const CoefImage& ci) {
     Result<rcg_coef_image> cc = to_c(ci);

@@ -148,6 +148,26 @@ int main() {
               "q100 is plain");
         std::printf("ok trellis_hand_off\n");
     }
+    {  // JPEG -> JPEG (cli/src/pipeline.rs:369-386): requantising a q92 file towards a target; q100 is lossless
+       // (cli/tests/cmyk.rs:38-54) and the device search's winner equals requantize at the chosen quality
+        Image img = textured(176, 120);
+        CoefImage src = build_coefs(dev, img, opts(92, Subsampling::S420)).value();
+        CoefImage same = src;
+        CHECK(requantize(same, 100, true).is_ok(), "requantize q100");
+        CHECK(same.comps[0].coefs == src.comps[0].coefs && same.qtables[0] == src.qtables[0], "q100 lossless");
+        for (bool trellis : {false, true}) {
+            Found f = find_jpeg_quality_requant(dev, src, 70.0, trellis).value();
+            CHECK(f.score >= 70.0 || f.quality == 100, "requant target: %f", f.score);
+            CoefImage w = src;
+            requantize(w, f.quality, trellis).value();
+            CHECK(w.comps[0].coefs == f.coefs.comps[0].coefs && w.comps[1].coefs == f.coefs.comps[1].coefs,
+                  "winner");
+            const Image decoded = reconstruct(dev, src).value();
+            const double s = ssimulacra2(dev, decoded, reconstruct(dev, f.coefs).value()).value();
+            CHECK(std::fabs(s - f.score) < 0.05, "score %f vs %f", s, f.score);
+        }
+        std::printf("ok requantisation_search\n");
+    }
     {  // the search's winner, written: the same file as writing build_coefs at the chosen quality (the CLI's
        // encode_jpeg_pixels, cli/src/pipeline.rs:225-247), and write_best picks one of the two encodings
         Image ref = textured(160, 96);

@@ -2,7 +2,7 @@
 the rust-compress tree at /root/reference). Only data files are copied: the SSIMULACRA2 golden pairs and their
 expected scores (fixtures/ssim2/, checked by cli/tests/ssimulacra2.rs:10-38) and three hierarchical JPEGs whose
 first frame is write_jpeg(build_coefs(...)) at q85 with trellis off (jpeg/src/hierarchical.rs:209-225,
-jpeg/examples/make_hier.rs:50-96)."""
+jpeg/examples/make_hier.rs:50-96), plus the baseline JPEG files named below."""
 import os
 import shutil
 import sys
@@ -18,4 +18,7 @@ for name in ("hier_dct_huffman.jpg", "hier_gray_dct_then_lossless.jpg", "hier_gr
 # Pictures of jpeg/tests/jpeg.rs:26-59 (PSNR floors) and baseline files of jpeg/tests/jpeg.rs:121-130 (67x45: odd
 # chroma sizes), decoded independently by libjpeg through Pillow in the tests.
 for name in ("rgb8.png", "gray8.png", "rgb8_420.jpg", "rgb8_444.jpg", "gray8.jpg"):
+    shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, name))
+# Four-component baseline files of cli/tests/cmyk.rs:38-54 (requantisation at q100 changes no coefficient).
+for name in ("cmyk.jpg", "ycck.jpg", "ycck_444.jpg"):
     shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, name))

@@ -321,3 +321,61 @@ def test_trellis_hand_off_is_bit_exact(ctx, rc, w, h, q, subs):
     got = rc.trellis(dev, dsc)
     for g, want in zip(got.comps, O.trellis(ora, osc)):
         np.testing.assert_array_equal(g.coefs, want)
+
+
+def _oracle_requant_search(src, target, trellis):
+    # cli/src/pipeline.rs:369-386 with the oracle: reference = reconstruct(ci), make(q) = requantize(ci, q, trellis),
+    # metric = f64 SSIMULACRA2, bisection of jpeg/src/target.rs:37-55.
+    ref = O.reconstruct(src)
+    cache = {}
+
+    def score(q):
+        if q not in cache:
+            cache[q] = O.ssimulacra2(ref, O.reconstruct(O.requantize(src, q, trellis)))
+        return cache[q]
+
+    lo, hi = 1, 100
+    best = hi
+    if score(hi) >= target:
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            if score(mid) >= target:
+                hi = mid
+                best = mid
+            else:
+                lo = mid + 1
+    return best, cache
+
+
+@pytest.mark.parametrize("w,h,q0,subs,gray,target,trellis", [
+    (96, 72, 92, O.S420, False, 75.0, False), (96, 72, 92, O.S420, False, 60.0, True),
+    (130, 66, 95, O.S444, False, 80.0, False), (64, 48, 90, O.S444, True, 70.0, True),
+    (200, 120, 85, O.S422, False, 65.0, False)])
+def test_requant_search_chooses_the_oracles_quality(ctx, rc, w, h, q0, subs, gray, target, trellis):
+    img = rc.synth_rgb8(900 + w, w, h)
+    if gray:
+        img = np.ascontiguousarray(img[:, :, 2])
+    ci = O.build_coefs(img, q0, subs)
+    src = rc.CoefImage(ci.width, ci.height, [rc.CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, c.coefs)
+                                             for c in ci.comps], list(ci.qtables) + [None] * (4 - len(ci.qtables)))
+    found = ctx.requant_find_quality(src, target, 100, trellis)
+    want, scores = _oracle_requant_search(src, target, trellis)
+    assert found.quality == want, (found.quality, want, sorted(scores.items()))
+    assert abs(found.score - scores[want]) <= TOL32
+    ref = O.requantize(src, want, trellis)
+    for a, b in zip(found.coefs.comps, ref.comps):
+        np.testing.assert_array_equal(a.coefs, b.coefs)
+
+
+def test_device_requantisation_matches_the_host(ctx, rc):
+    # The requantisation fused into the device IDCT reconstructs exactly what the host requantisation's coefficients
+    # reconstruct to (checked through the search's winner: its score at q equals a fresh score of the host result).
+    img = rc.synth_rgb8(41, 160, 96)
+    ci = O.build_coefs(img, 93, O.S420)
+    src = rc.CoefImage(ci.width, ci.height, [rc.CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, c.coefs)
+                                             for c in ci.comps], list(ci.qtables) + [None] * 2)
+    found = ctx.requant_find_quality(src, 70.0, 100, False)
+    host = rc.requantize(src, found.quality)
+    np.testing.assert_array_equal(ctx.reconstruct(host), O.reconstruct(host))
+    s = ctx.ssimulacra2(O.reconstruct(src), ctx.reconstruct(host), rc.FP32)
+    assert abs(s - found.score) <= TOL32
