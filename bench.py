@@ -272,7 +272,8 @@ def bench_batch(rc, ctx, args):
     gen_s = time.perf_counter() - t0
     ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
     cands = [(q, rc.S420) for q in CANDS_B]
-    tune_b = ctx.autotune(len(cands))
+    # A batch launch holds many pairs of every size: tune for many pairs per launch, not for one image's candidates.
+    tune_b = ctx.autotune(64)
     total_mp = sum(w * h for w, h in sizes) / 1e6
     ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)  # warm-up (sizes the workspace)
     ctx.synchronize()
@@ -308,6 +309,11 @@ def bench_batch(rc, ctx, args):
     out["graph_search_mps"] = round(total_mp / graph_s, 2)
     out["graph_matches_eager"] = int(all(g.chosen_quality == r.chosen_quality for g, r in zip(gres, res)))
     if not args.no_plain:
+        # The plain search scores one pair per launch: it gets the policy tuned for that, not the batch's.
+        out["plain_tile_policy"] = ctx.autotune(1).as_dict()
+        for p in ptrs[:4]:
+            ctx.find_quality_dev(p[0], p[1], p[2], 3, 70.0, 100, rc.S420)  # warm-up under the new policy
+        ctx.synchronize()
         t0 = time.perf_counter()
         steps = 0
         for p in ptrs:

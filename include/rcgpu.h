@@ -195,8 +195,11 @@ void rcg_batch_plan_destroy(rcg_plan* plan);
  * class i with h >= min_h[i] (classes in descending min_h, the last with min_h = 0), and 128-thread workgroups when
  * its width is at most narrow_w (else wide_nt = 256). The policy depends only on an image's own size, so a score
  * never depends on what else shares a launch; different policies change scores only in the last bits.
- * rcg_autotune times the scale kernel for th in {8,16,32,64,128} on `candidates` synthetic pairs at each size
- * class (4096 down to 64 rows) and installs the fastest per class; it returns the installed policy.
+ * rcg_autotune times the scale kernel for th in {2,4,8,16,24,32,48,64,96,128,160,208,256} at each size class
+ * (4096 down to 64 rows) and installs the fastest per class; it returns the installed policy. `jobs` (1..64) is how
+ * many same-size pairs share a launch in the workload the policy is for: the candidates of one image for a single
+ * search or sweep, more for a batch of many images (their launches hold many pairs of every size, so short tiles,
+ * which only shorten the latency of a lone small image, cost halo work instead).
  */
 typedef struct {
     int wide_nt;
@@ -207,7 +210,7 @@ typedef struct {
 } rcg_tune;
 int rcg_get_tune(rcg_ctx* ctx, rcg_tune* out);
 int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune);
-int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out);
+int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out);
 
 /*
  * Serving loop with continuous batching. A worker thread owns one context. Requests (an image and a

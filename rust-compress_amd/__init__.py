@@ -427,9 +427,10 @@ class Context:
     def set_tune(self, tune):
         _check(lib().rcg_set_tune(self._h, ctypes.byref(tune)))
 
-    def autotune(self, candidates=8):
+    def autotune(self, jobs=8):
+        """Measure and install the tile policy for launches of `jobs` same-size pairs (rcg_autotune)."""
         t = Tune()
-        _check(lib().rcg_autotune(self._h, candidates, ctypes.byref(t)))
+        _check(lib().rcg_autotune(self._h, jobs, ctypes.byref(t)))
         return t
 
     # ------------------------------------------------------------------ profiling

@@ -131,12 +131,12 @@ PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
         P.tiles_y[s] = ssim_tiles_y(sh, int(P.th[s]));
         P.tiles[s] = P.tiles_x[s] * P.tiles_y[s];
         // The reference-cache kernel's output does not depend on its tiling: its tiles are as tall as one round of the
-        // GPU's workgroup slots allows for this scale (at least 16 rows), so one image keeps the GPU full without a
-        // partly idle last round.
+        // GPU's workgroup slots allows for this scale (at least 4 rows), so one image keeps the GPU full without a
+        // partly idle last round, and small scales walk few rows.
         const uint32_t slots = P.nt[s] == 128 ? T.ref_slots_narrow : T.ref_slots_wide;
         const uint32_t rows_of_tiles = std::max(1u, slots / P.tiles_x[s]);
         uint32_t tr = (sh + rows_of_tiles - 1) / rows_of_tiles;
-        tr = std::max(16u, (tr + 1) & ~1u);
+        tr = std::max(4u, (tr + 1) & ~1u);
         P.th_ref[s] = tr;
         P.tiles_ref[s] = P.tiles_x[s] * ssim_tiles_y(sh, int(tr));
         P.nscales++;
@@ -1194,27 +1194,29 @@ extern "C" int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune) {
     return RCG_OK;
 }
 
-// Times k_ssim_scale (f32, the reference cache and f32 planes in, no pyramid out) on `candidates` pairs of S x S
-// synthetic planes for every tile height, for S = 4096 ... 64, and installs the fastest tile height of each size
+// Times k_ssim_scale (f32, the reference cache and f32 planes in, no pyramid out) on `jobs` pairs of S x S synthetic
+// planes in one launch for every tile height, for S = 4096 ... 64, and installs the fastest tile height of each size
 // class.
-extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
+extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
+    const int candidates = jobs;
     if (!ctx || candidates < 1 || candidates > 64) return RCG_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     RCG_HIP(hipSetDevice(ctx->device));
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
     // Tile heights: powers of two and the steps between them (a launch's last round of workgroups is the less idle
-    // the closer its tile count comes to a multiple of the GPU's workgroup slots).
-    const int ths[] = {8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256};
+    // the closer its tile count comes to a multiple of the GPU's workgroup slots). Small scales are latency-bound
+    // (one workgroup walks TH + 10 rows one after another), so very short tiles are tried too.
+    const int ths[] = {2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256};
     const int nsz = 7;
     const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
     ctx->arena.reset();
-    RCG_TRY(ctx->arena.ensure(5 * Arena::align(plane_bytes) + (size_t(4) << 20) + (size_t(1) << 20)));
-    RCG_TRY(ctx->pinned.ensure(size_t(1) << 20));
+    RCG_TRY(ctx->arena.ensure(5 * Arena::align(plane_bytes) + (size_t(32) << 20) + (size_t(4) << 20)));
+    RCG_TRY(ctx->pinned.ensure(size_t(4) << 20));
     float* a = static_cast<float*>(ctx->arena.take(plane_bytes));
     float* b = static_cast<float*>(ctx->arena.take(plane_bytes));
     float* cache = static_cast<float*>(ctx->arena.take(3 * plane_bytes));
-    double* partials = static_cast<double*>(ctx->arena.take(size_t(4) << 20));
-    char* ddesc = static_cast<char*>(ctx->arena.take(size_t(1) << 20));
+    double* partials = static_cast<double*>(ctx->arena.take(size_t(32) << 20));
+    char* ddesc = static_cast<char*>(ctx->arena.take(size_t(4) << 20));
     // Synthetic linear-light planes: a smooth pattern and a perturbed copy (the timing does not depend on values).
     {
         std::vector<float> h(size_t(4096) * 4096 * 3);
@@ -1242,7 +1244,7 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
             if (uint32_t(th) > S) continue;
             const int nt = S <= T.narrow_w ? 128 : T.wide_nt;
             const uint32_t tx = ssim_tiles_x(S, nt), ty = ssim_tiles_y(S, th);
-            if (size_t(tx) * ty * 18 * 8 * candidates > (size_t(4) << 20)) continue;
+            if (size_t(tx) * ty * 18 * 8 * candidates > (size_t(32) << 20)) continue;
             // The reference cache of the S x S plane `a` (the same for every tile height; rebuilt per size).
             RefJob rj;
             std::memset(&rj, 0, sizeof(rj));
@@ -1271,7 +1273,7 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
             Desc D;
             D.host = ctx->pinned.base;
             D.dev = ddesc;
-            D.cap = size_t(1) << 20;
+            D.cap = size_t(4) << 20;
             const SsimJob* dj = D.put(jobs.data(), jobs.size());
             const Unit* du = D.put(units.data(), units.size());
             const RefJob* drj = D.put(&rj, 1);
@@ -1279,7 +1281,8 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int candidates, rcg_tune* out) {
             RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes, ctx->stream));
             const uint32_t nu = uint32_t(units.size());
             RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
-            const int reps = S >= 2048 ? 3 : 10;
+            const size_t work = size_t(S) * S * size_t(candidates);
+            const int reps = work >= (size_t(64) << 20) ? 2 : (work >= (size_t(16) << 20) ? 3 : 10);
             RCG_HIP(hipEventRecord(e0, ctx->stream));
             for (int r = 0; r < reps; r++)
                 RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));

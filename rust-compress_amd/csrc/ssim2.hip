@@ -377,7 +377,8 @@ __device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[
     const bool two_cols = gx0 + 1 < w;
     const int oy = yy >> 1, ox = gx0 >> 1;
     const uint64_t n2 = uint64_t(nw) * nh, o = uint64_t(oy) * nw + ox;
-    const double inv = 1.0 / double((first_row ? 1 : 2) * (two_cols ? 2 : 1));
+    // Dividing by the pixel count (1, 2 or 4), as the reference does, is the exact multiplication by its inverse.
+    const double inv = first_row ? (two_cols ? 0.5 : 1.0) : (two_cols ? 0.25 : 0.5);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         double sum = double(v0[c]);
@@ -727,26 +728,26 @@ __global__ __launch_bounds__(NT) void k_ssim_ref(const RefJob* __restrict__ jobs
     }
 }
 
-// One 256-thread workgroup per image pair: fixed-order f64 reduction of the tile sums of every scale, the mean and
-// 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
-constexpr int kFinThreads = 256;
+// One workgroup per image pair, one wave per scale: fixed-order f64 reduction of each scale's tile sums, then the
+// mean and 4-norm of each map (ssimulacra2.rs:457-465), the weighted pooling and the final mapping (:491-511).
+constexpr int kFinThreads = 384;
 
 __global__ __launch_bounds__(kFinThreads) void k_ssim_finalize(const FinalDesc* __restrict__ fj) {
     const FinalDesc& F = fj[blockIdx.x];
-    __shared__ double s_wave[kFinThreads / 64][18];
     __shared__ double s_sum[6][18];
     __shared__ double s_stat[6][18];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, sc = tid >> 6;
     const uint32_t ns = F.nscales;
-    // Thread t sums tiles t, t+256, ... of a scale (each tile's 18 sums are one contiguous 144-byte record); a
-    // fixed butterfly per wave and a fixed-order sum over the waves combine the threads.
-    for (uint32_t sc = 0; sc < ns; sc++) {
+    // Lane l of wave s sums tiles l, l+64, ... of scale s (each tile's 18 sums are one contiguous 144-byte record);
+    // a fixed butterfly combines the lanes.
+    if (uint32_t(sc) < ns) {
         const double* p = F.partials[sc];
         const uint32_t nt = F.ntiles[sc];
         double acc[18];
 #pragma unroll
         for (int i = 0; i < 18; i++) acc[i] = 0.0;
-        for (uint32_t t = tid; t < nt; t += kFinThreads) {
+#pragma unroll 2
+        for (uint32_t t = lane; t < nt; t += 64) {
             const double2* rec = reinterpret_cast<const double2*>(p + uint64_t(t) * 18);
 #pragma unroll
             for (int i = 0; i < 9; i++) {
@@ -760,16 +761,10 @@ __global__ __launch_bounds__(kFinThreads) void k_ssim_finalize(const FinalDesc* 
             double v = acc[i];
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) s_wave[wave][i] = v;
+            if (lane == 0) s_sum[sc][i] = v;
         }
-        __syncthreads();
-        if (tid < 18) {
-            double v = 0.0;
-            for (int w = 0; w < kFinThreads / 64; w++) v += s_wave[w][tid];
-            s_sum[sc][tid] = v;
-        }
-        __syncthreads();
     }
+    __syncthreads();
     // One thread per (scale, statistic): mean for even entries, 4-norm for odd ones.
     for (int k = tid; k < int(ns) * 18; k += kFinThreads) {
         const int sc = k / 18, i = k % 18;

@@ -241,7 +241,7 @@ def test_autotuned_policy_keeps_parity(rc):
     c = rc.Context(0)
     t = c.autotune(4)
     assert 1 <= t.nclass <= 7
-    assert all(t.th[i] in (8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256) for i in range(t.nclass))
+    assert all(t.th[i] in (2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256) for i in range(t.nclass))
     img = rc.synth_rgb8(12, 700, 530)
     dist = O.reconstruct(O.build_coefs(img, 60, O.S420))
     assert abs(c.ssimulacra2(img, dist, rc.FP32) - O.ssimulacra2(img, dist)) <= TOL32
