@@ -155,22 +155,27 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
         const EncCand& K = J.cand[ci];
         const float* div = K.div[cls];
         // Quantise: natural position n = v*8+u reads the transposed DCT output at u*8+v (encode.rs:45-54,203-205).
-        int32_t qv[64];
-#pragma unroll
-        for (int nidx = 0; nidx < 64; nidx++) {
-            const int f = (nidx % 8) * 8 + nidx / 8;
-            int32_t r = round_f32(__fmul_rn(blk[f], div[f]));
-            qv[nidx] = r < -2047 ? -2047 : (r > 2047 ? 2047 : r);
-        }
         int16_t* co = K.coef[comp];
         if (co) {
-            uint32_t pk[32];
-#pragma unroll
-            for (int k = 0; k < 32; k++)
-                pk[k] = (uint32_t(qv[zigzag(2 * k)]) & 0xFFFF) | (uint32_t(qv[zigzag(2 * k + 1)]) << 16);
+            // Coefficients in zigzag order, packed two per dword as they are computed (no 64-entry array beside blk).
             uint4* dst = reinterpret_cast<uint4*>(co + blk_index * 64);
 #pragma unroll
-            for (int k = 0; k < 8; k++) dst[k] = make_uint4(pk[4 * k], pk[4 * k + 1], pk[4 * k + 2], pk[4 * k + 3]);
+            for (int k8 = 0; k8 < 8; k8++) {
+                uint32_t pk[4];
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    uint32_t pair = 0;
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+                        const int nidx = zigzag(8 * k8 + 2 * h + e), f = (nidx % 8) * 8 + nidx / 8;
+                        int32_t r = round_f32(__fmul_rn(blk[f], div[f]));
+                        r = r < -2047 ? -2047 : (r > 2047 ? 2047 : r);
+                        pair |= (uint32_t(r) & 0xFFFFu) << (16 * e);
+                    }
+                    pk[h] = pair;
+                }
+                dst[k8] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+            }
         }
         float* sp = K.scaled[comp];
         if (sp) {
@@ -189,10 +194,16 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
         }
         uint8_t* rp = K.rec[comp];
         if (rp) {
+            // Dequantise straight from the rounded values (decode.rs:1317-1322), then the IDCT.
             const float* mult = K.mult[cls];
             float b2[64];
 #pragma unroll
-            for (int nidx = 0; nidx < 64; nidx++) b2[nidx] = __fmul_rn(float(qv[nidx]), mult[nidx]);
+            for (int nidx = 0; nidx < 64; nidx++) {
+                const int f = (nidx % 8) * 8 + nidx / 8;
+                int32_t r = round_f32(__fmul_rn(blk[f], div[f]));
+                r = r < -2047 ? -2047 : (r > 2047 ? 2047 : r);
+                b2[nidx] = __fmul_rn(float(r), mult[nidx]);
+            }
             idct_block(b2);
             const uint32_t stride = bw_comp * 8;
             uint8_t* out = rp + uint64_t(by) * 8 * stride + bx * 8;
