@@ -379,3 +379,18 @@ def test_device_requantisation_matches_the_host(ctx, rc):
     np.testing.assert_array_equal(ctx.reconstruct(host), O.reconstruct(host))
     s = ctx.ssimulacra2(O.reconstruct(src), ctx.reconstruct(host), rc.FP32)
     assert abs(s - found.score) <= TOL32
+
+
+@pytest.mark.parametrize("w,h", [(65535, 9), (9, 4099), (4097, 17)])
+def test_extreme_aspect_ratios(ctx, rc, w, h):
+    # The longest side JPEG allows (encode.rs:417) and very tall strips: encode and reconstruct bit-exact, the score
+    # within the FP32 bound (the 8-pixel scale floor leaves one or two scales).
+    img = rc.synth_rgb8(60 + h, w, h)
+    for subs in (O.S420, O.S444):
+        dev = ctx.build_coefs(img, 70, subs)
+        ora = O.build_coefs(img, 70, subs)
+        for a, b in zip(dev.comps, ora.comps):
+            np.testing.assert_array_equal(a.coefs, b.coefs)
+        rec = ctx.reconstruct(dev)
+        np.testing.assert_array_equal(rec, O.reconstruct(ora))
+        assert abs(ctx.ssimulacra2(img, rec, rc.FP32) - O.ssimulacra2(img, rec)) <= TOL32
