@@ -294,6 +294,16 @@ int rcg_trellis(const rcg_coef_image* plain, const float* const scaled[4], int16
 int rcg_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t sizes[256], const float a[64],
                         int16_t out[64]);
 
+/* The target search of rcg_find_quality with the reference's default make = build_coefs(trellis on)
+ * (cli/src/pipeline.rs:238 with EncodeOptions::trellis, encode.rs:527-537): every bisection step encodes on the
+ * device with the hand-off values, runs rcg_trellis on the host, uploads its coefficients, and reconstructs and
+ * scores them on the device against a reference state built once per call (on a side stream, while the first
+ * step's encode and trellis run). Same arguments, guard band and results as rcg_find_quality; coefs receive the
+ * chosen quality's trellis coefficients. */
+int rcg_find_quality_trellis(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height,
+                             uint32_t channels, double target, int max_quality, int subsampling, rcg_found* out,
+                             int16_t* coefs[3], uint8_t qtables[128]);
+
 /* ------------------------------------------------------------------------------------------------------------
  * DCT-domain requantisation (JPEG -> JPEG). */
 typedef struct {
@@ -317,6 +327,43 @@ int rcg_requantize(const rcg_coef_image* ci, int quality, int trellis, rcg_requa
  * planes share one geometry with sampling ratios of 1 or 2; otherwise RCG_UNSUPPORTED. */
 int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, double target, int max_quality, int trellis,
                              rcg_found* out, rcg_requantized* winner);
+
+/* Device bytes that one search call takes from its context (the arena it sizes, descriptors included), for
+ * admission against an HBM budget. mode: RCG_SEARCH_PLAIN (rcg_find_quality), RCG_SEARCH_TRELLIS
+ * (rcg_find_quality_trellis), RCG_SEARCH_REQUANT (rcg_requant_find_quality over a source of that size; an upper
+ * bound, the 4:4:4 block grid). No counterpart in the reference: it feeds the CLI's replacement of
+ * cli/src/budget.rs:117-119 (estimate_bytes) with device footprints. */
+typedef enum { RCG_SEARCH_PLAIN = 0, RCG_SEARCH_TRELLIS = 1, RCG_SEARCH_REQUANT = 2 } rcg_search_mode;
+int rcg_search_bytes(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling, int mode,
+                     size_t* out);
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Reading a JPEG source on the host (no device): what the CLI needs to route a JPEG -> JPEG job and to feed the
+ * requantisation search. */
+typedef struct {
+    uint32_t width, height, ncomp;  /* ncomp 1, 3 or 4 */
+    uint8_t id[4], h[4], v[4], tq[4];
+    uint32_t blocks_w[4], blocks_h[4];  /* MCU-padded block grid of each component */
+    uint8_t qtables[4][64];         /* natural order: each slot's table as its components use it (clamped to 255) */
+    int present[4];
+    int adobe_transform;            /* -1: no Adobe APP14 marker */
+    int source_quality;             /* IJG quality estimate of the luma table (jpeg/src/quality.rs:17-22) */
+    int coded_rgb;                  /* three components stored as RGB (decode.rs:481-482) */
+    int progressive, arithmetic, lossless, hierarchical, precision;
+    int orientation;                /* EXIF orientation 1..8, 1 when absent (decode.rs:572-594) */
+    int table16;                    /* a used table has an entry above 255 or equal to 0 */
+} rcg_jpeg_info;
+
+/* Replaces rc_jpeg::decode::parse(data)?.info() (jpeg/src/decode.rs:397-489) plus exif_orientation (:572): the
+ * frame facts, walking the markers without decoding the scans. For a hierarchical file only the DHP facts are
+ * filled. RCG_INVALID for a malformed file, RCG_UNSUPPORTED for a precision or component count the reference
+ * rejects. */
+int rcg_jpeg_parse(const uint8_t* data, size_t len, rcg_jpeg_info* info);
+/* Replaces Parsed::into_coefs (decode.rs:529-569) for sequential and progressive Huffman-coded 8-bit DCT frames:
+ * coefs[c] (blocks_w*blocks_h*64 int16, zigzag order per block) receive component c's quantised coefficients,
+ * decoded as the reference's decoder does (T.81 Annex F/G). RCG_UNSUPPORTED for arithmetic-coded, lossless,
+ * hierarchical and 12-bit frames; RCG_INVALID for corrupt entropy-coded data. */
+int rcg_jpeg_read_coefs(const uint8_t* data, size_t len, int16_t* coefs[4]);
 
 /* ------------------------------------------------------------------------------------------------------------
  * Entropy coding on the host (no device, no context): the stage after the search, which serialises the chosen

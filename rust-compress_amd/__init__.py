@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "librcgpu.so")
 OK, INVALID, UNSUPPORTED, DEVICE, OOM = 0, 1, 2, 3, 4
 AUTO, S444, S422, S420 = 0, 1, 2, 3
 FP32, FP64 = 0, 1
+SEARCH_PLAIN, SEARCH_TRELLIS, SEARCH_REQUANT = 0, 1, 2
 
 _STATUS = {INVALID: "invalid", UNSUPPORTED: "unsupported", DEVICE: "device", OOM: "out of memory"}
 
@@ -60,6 +61,16 @@ MAX_CAND = 16
 class _Requantized(ctypes.Structure):
     _fields_ = [("coefs", ctypes.c_void_p * 4), ("qtables", (ctypes.c_uint8 * 64) * 4), ("present", ctypes.c_int * 4),
                 ("tq", ctypes.c_uint8 * 4)]
+
+
+class _JpegInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("id", ctypes.c_uint8 * 4), ("h", ctypes.c_uint8 * 4), ("v", ctypes.c_uint8 * 4),
+                ("tq", ctypes.c_uint8 * 4), ("blocks_w", ctypes.c_uint32 * 4), ("blocks_h", ctypes.c_uint32 * 4),
+                ("qtables", (ctypes.c_uint8 * 64) * 4), ("present", ctypes.c_int * 4), ("adobe_transform", ctypes.c_int),
+                ("source_quality", ctypes.c_int), ("coded_rgb", ctypes.c_int), ("progressive", ctypes.c_int),
+                ("arithmetic", ctypes.c_int), ("lossless", ctypes.c_int), ("hierarchical", ctypes.c_int),
+                ("precision", ctypes.c_int), ("orientation", ctypes.c_int), ("table16", ctypes.c_int)]
 
 
 class BatchJob(ctypes.Structure):
@@ -166,6 +177,10 @@ def lib():
         L.rcg_requantize.argtypes = [ctypes.POINTER(_CoefImage), I, I, ctypes.POINTER(_Requantized)]
         L.rcg_requant_find_quality.argtypes = [P, ctypes.POINTER(_CoefImage), D, I, I, ctypes.POINTER(_Found),
                                                ctypes.POINTER(_Requantized)]
+        L.rcg_find_quality_trellis.argtypes = [P, P, U, U, U, D, I, I, ctypes.POINTER(_Found), P, P]
+        L.rcg_search_bytes.argtypes = [P, U, U, U, I, I, ctypes.POINTER(ctypes.c_size_t)]
+        L.rcg_jpeg_parse.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(_JpegInfo)]
+        L.rcg_jpeg_read_coefs.argtypes = [P, ctypes.c_size_t, P]
         L.rcg_profile.argtypes = [P, I]
         L.rcg_profile_get.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)]
         _LIB = L
@@ -315,6 +330,12 @@ class Context:
                                               ctypes.byref(f), ctypes.byref(r)))
         return Found(f.quality, f.score, _from_requantized(ci, r, outs), f.steps)
 
+    def search_bytes(self, width, height, channels, subsampling=AUTO, mode=SEARCH_PLAIN):
+        """Device bytes one search call takes from this context (rcg_search_bytes), for HBM admission."""
+        n = ctypes.c_size_t()
+        _check(lib().rcg_search_bytes(self._h, width, height, channels, subsampling, mode, ctypes.byref(n)))
+        return n.value
+
     def reconstruct(self, ci):
         keep = []
         c_ci = _coef_image_struct(ci, keep, device=False)
@@ -338,7 +359,8 @@ class Context:
             return score.value, stats[: ns.value].copy()
         return score.value
 
-    def find_quality(self, img, target, max_quality=100, subsampling=S420):
+    def find_quality(self, img, target, max_quality=100, subsampling=S420, trellis=False):
+        """The target search (rcg_find_quality; rcg_find_quality_trellis with trellis, the CLI default)."""
         img = np.ascontiguousarray(img, dtype=np.uint8)
         h, w = img.shape[:2]
         ch = _channels(img)
@@ -348,8 +370,9 @@ class Context:
         bufs = [np.zeros(L.blocks_w[c] * L.blocks_h[c] * 64, np.int16) for c in range(L.ncomp)]
         ptrs = (ctypes.c_void_p * 3)(*([b.ctypes.data for b in bufs] + [None] * (3 - len(bufs))))
         qt = np.zeros(128, np.uint8)
-        _check(lib().rcg_find_quality(self._h, _ptr(img), w, h, ch, float(target), max_quality, subsampling,
-                                      ctypes.byref(f), ctypes.cast(ptrs, ctypes.c_void_p), _ptr(qt)))
+        fn = lib().rcg_find_quality_trellis if trellis else lib().rcg_find_quality
+        _check(fn(self._h, _ptr(img), w, h, ch, float(target), max_quality, subsampling, ctypes.byref(f),
+                  ctypes.cast(ptrs, ctypes.c_void_p), _ptr(qt)))
         Lq = coef_layout(w, h, ch, f.quality, subsampling)
         if ch == 1:
             comps = [CoefComponent(1, 1, 1, 0, Lq.blocks_w[0], bufs[0][: Lq.blocks_w[0] * Lq.blocks_h[0] * 64])]
@@ -651,6 +674,56 @@ def requantize(ci, quality, trellis=False):
     r, outs = _requantized(ci)
     _check(lib().rcg_requantize(ctypes.byref(s), quality, 1 if trellis else 0, ctypes.byref(r)))
     return _from_requantized(ci, r, outs)
+
+
+@dataclass
+class JpegInfo:
+    """rc_jpeg::decode::Info (jpeg/src/decode.rs:311-331) with the layout, tables and EXIF orientation."""
+    width: int
+    height: int
+    sampling: list           # (h, v) per component
+    ids: list
+    tq: list
+    blocks: list             # (blocks_w, blocks_h) per component, MCU-padded
+    qtables: list            # natural-order uint8[64] per slot, or None
+    adobe_transform: int
+    source_quality: int
+    coded_rgb: bool
+    cmyk: bool
+    progressive: bool
+    arithmetic: bool
+    lossless: bool
+    hierarchical: bool
+    precision: int
+    orientation: int
+    table16: bool
+
+
+def jpeg_info(data):
+    """The frame facts of a JPEG file (rcg_jpeg_parse); host only."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    i = _JpegInfo()
+    _check(lib().rcg_jpeg_parse(_ptr(buf), len(buf), ctypes.byref(i)))
+    n = i.ncomp
+    return JpegInfo(i.width, i.height, [(i.h[c], i.v[c]) for c in range(n)], [i.id[c] for c in range(n)],
+                    [i.tq[c] for c in range(n)], [(i.blocks_w[c], i.blocks_h[c]) for c in range(n)],
+                    [np.frombuffer(bytes(i.qtables[s]), np.uint8).copy() if i.present[s] else None for s in range(4)],
+                    i.adobe_transform, i.source_quality, bool(i.coded_rgb), n == 4, bool(i.progressive),
+                    bool(i.arithmetic), bool(i.lossless), bool(i.hierarchical), i.precision, i.orientation,
+                    bool(i.table16))
+
+
+def read_jpeg(data, info=None):
+    """The quantised coefficients of a Huffman-coded 8-bit DCT JPEG (rcg_jpeg_read_coefs) as a CoefImage, the
+    equivalent of rc_jpeg::decode::parse(data)?.into_coefs() (jpeg/src/decode.rs:529-569); host only."""
+    info = info or jpeg_info(data)
+    buf = np.frombuffer(bytes(data), np.uint8)
+    outs = [np.zeros(bw * bh * 64, np.int16) for bw, bh in info.blocks]
+    ptrs = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None] * (4 - len(outs))))
+    _check(lib().rcg_jpeg_read_coefs(_ptr(buf), len(buf), ctypes.cast(ptrs, ctypes.c_void_p)))
+    comps = [CoefComponent(info.ids[c], info.sampling[c][0], info.sampling[c][1], info.tq[c], info.blocks[c][0],
+                           outs[c]) for c in range(len(outs))]
+    return CoefImage(info.width, info.height, comps, list(info.qtables), info.adobe_transform)
 
 
 def rdo_quantize_ac(table, kappa, sizes, a):

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "rcg_host.h"
 
@@ -817,6 +818,19 @@ namespace {
 
 constexpr double kGuard = 0.05;  // |f32 score - target| below this is settled with the f64 scorer
 
+// Device bytes of one target search over a width x height image (sources, candidate planes, coefficients, hand-off
+// values with trellis, and the FP32 and FP64 SSIMULACRA2 state), not counting the descriptor staging.
+size_t search_need(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling, bool trellis) {
+    const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    const PairPlan P32 = plan_pair(width, height, ctx->tune32);
+    const PairPlan P64 = plan_pair(width, height, ctx->tune64);
+    size_t need = Arena::align(size_t(width) * height * channels) + plane_bytes(Lmax) + coef_bytes(Lmax) +
+                  ssim_bytes(P32, RCG_PREC_FP32) + ssim_bytes(P64, RCG_PREC_FP64);
+    if (trellis)
+        for (uint32_t c = 0; c < Lmax.ncomp; c++) need += Arena::align(Lmax.coef_count(c) * sizeof(float));
+    return need;
+}
+
 // Score one candidate quality on the device: encode + reconstruct planes, SSIMULACRA2 against the source.
 // The reference state of each precision is built by its first evaluation and reused by the later ones.
 int score_candidate(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_src, int quality,
@@ -866,11 +880,10 @@ int find_quality_impl(rcg_ctx* ctx, const uint8_t* pixels, bool host_src, uint32
     }
     // Auto subsampling depends on the quality; size for the larger (4:4:4) layout.
     const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    const size_t src_bytes = size_t(width) * height * channels;
     const PairPlan P32 = plan_pair(width, height, ctx->tune32);
     const PairPlan P64 = plan_pair(width, height, ctx->tune64);
-    const size_t src_bytes = size_t(width) * height * channels;
-    const size_t need = Arena::align(src_bytes) + plane_bytes(Lmax) + coef_bytes(Lmax) +
-                        ssim_bytes(P32, RCG_PREC_FP32) + ssim_bytes(P64, RCG_PREC_FP64);
+    const size_t need = search_need(ctx, width, height, channels, subsampling, false);
     Call call(ctx);
     RCG_TRY(call.begin(need));
     const uint8_t* d_src = pixels;
@@ -951,9 +964,213 @@ int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, 
 }  // extern "C"
 
 // ================================================================================================================
+// The search with trellis quantisation (the CLI default, encode.rs:527-537): make = build_coefs(trellis on)
+
+namespace {
+
+// Host side of one trellis candidate: the plain coefficients and hand-off values read back, the trellis output.
+struct TrellisHost {
+    std::vector<int16_t> plain[3], tr[3], best[3];
+    std::vector<float> scaled[3];
+};
+
+// FP64 re-score of a candidate whose FP32 score fell inside the guard band around the target.
+int rescore_fp64(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_ref, uint8_t* const rec[3],
+                 const SsimBuffers& B64, bool& a64_ready, double& s) {
+    call.D.off = 0;
+    const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
+    StagedSsim s64 = stage_ssim(call.D, P64, bytes_src(d_ref, L.channels), recon_src(L, rec), B64, a64_ready,
+                                RCG_PREC_FP64);
+    RCG_TRY(call.D.upload(ctx->stream));
+    RCG_TRY(launch_staged_ssim(ctx, s64));
+    RCG_HIP(hipMemcpyAsync(&s, B64.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    a64_ready = true;
+    return RCG_OK;
+}
+
+// One candidate: encode on the device with the hand-off values, read back; the host trellis; upload; IDCT and
+// SSIMULACRA2 on the device. On the first candidate the reference state is built on the side stream while the
+// encode, the read-back and the host trellis run. At quality 100 there is no trellis pass (encode.rs:527): the
+// encode reconstructs its planes itself and only the coefficients are read back.
+int score_trellis(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_src, int quality,
+                  int16_t* const d_co[3], float* const d_sc[3], uint8_t* const rec[3], TrellisHost& H,
+                  const SsimBuffers& B32, const SsimBuffers& B64, bool& a32_ready, bool& a64_ready, double target,
+                  double& score) {
+    const bool use_trellis = quality < 100;
+    call.D.off = 0;
+    int16_t* co[3] = {d_co[0], L.ncomp == 3 ? d_co[1] : nullptr, L.ncomp == 3 ? d_co[2] : nullptr};
+    float* sc[3] = {d_sc[0], L.ncomp == 3 ? d_sc[1] : nullptr, L.ncomp == 3 ? d_sc[2] : nullptr};
+    uint8_t* rc3[3] = {rec[0], L.ncomp == 3 ? rec[1] : nullptr, L.ncomp == 3 ? rec[2] : nullptr};
+    const StagedEncode se = stage_encode(call.D, L, d_src, quality, co, use_trellis ? nullptr : rc3,
+                                         use_trellis ? sc : nullptr);
+    const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
+    StagedSsim st = stage_ssim(call.D, P32, bytes_src(d_src, L.channels), recon_src(L, rec), B32, a32_ready,
+                               RCG_PREC_FP32);
+    RCG_TRY(call.D.upload(ctx->stream));
+    if (st.build_ref) RCG_TRY(fork_side(ctx));
+    RCG_TRY(launch_encode(ctx, se.job, se.begins, 1, se.tiles));
+    RCG_TRY(launch_staged_refs(ctx, st));
+    for (uint32_t c = 0; c < L.ncomp; c++) H.tr[c].resize(L.coef_count(c));
+    if (use_trellis) {
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            H.plain[c].resize(L.coef_count(c));
+            H.scaled[c].resize(L.coef_count(c));
+            RCG_HIP(hipMemcpyAsync(H.plain[c].data(), d_co[c], L.coef_count(c) * sizeof(int16_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            RCG_HIP(hipMemcpyAsync(H.scaled[c].data(), d_sc[c], L.coef_count(c) * sizeof(float),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        }
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        uint8_t qt[2][64];
+        quant_tables(quality, qt[0], qt[1]);
+        rcg_coef_image ci;
+        std::memset(&ci, 0, sizeof(ci));
+        ci.width = L.w;
+        ci.height = L.h;
+        ci.ncomp = L.ncomp;
+        ci.adobe_transform = -1;
+        ci.qtables[0] = qt[0];
+        ci.qtables[1] = L.ncomp == 3 ? qt[1] : nullptr;
+        const float* sp[4] = {nullptr, nullptr, nullptr, nullptr};
+        int16_t* op[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            rcg_coef_component& k = ci.comps[c];
+            k.id = uint8_t(c + 1);
+            k.h = uint8_t(c == 0 ? L.hmax : 1);
+            k.v = uint8_t(c == 0 ? L.vmax : 1);
+            k.tq = uint8_t(c == 0 ? 0 : 1);
+            k.blocks_w = L.bw[c];
+            k.blocks_h = L.bh[c];
+            k.coefs = H.plain[c].data();
+            sp[c] = H.scaled[c].data();
+            op[c] = H.tr[c].data();
+        }
+        RCG_TRY(rcg_trellis(&ci, sp, op));
+        IdctPlane ip[3] = {};
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            // The plain coefficients on the device are no longer needed: the trellis output takes their place.
+            RCG_HIP(hipMemcpyAsync(d_co[c], H.tr[c].data(), L.coef_count(c) * sizeof(int16_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            ip[c].coefs = d_co[c];
+            ip[c].blocks_w = L.bw[c];
+            ip[c].blocks_h = L.bh[c];
+            ip[c].plane = rec[c];
+            mult_table(qt[c == 0 ? 0 : 1], ip[c].mult);
+        }
+        RCG_TRY(launch_idct(ctx, ip, int(L.ncomp)));
+    } else {
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            RCG_HIP(hipMemcpyAsync(H.tr[c].data(), d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    }
+    RCG_TRY(launch_staged_ssim(ctx, st));
+    double s = 0.0;
+    RCG_HIP(hipMemcpyAsync(&s, B32.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    a32_ready = true;
+    if (std::isfinite(s) && std::fabs(s - target) < kGuard)
+        RCG_TRY(rescore_fp64(ctx, call, L, d_src, rec, B64, a64_ready, s));
+    score = std::isfinite(s) ? s : -DBL_MAX;
+    return RCG_OK;
+}
+
+}  // namespace
+
+extern "C" int rcg_find_quality_trellis(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height,
+                                        uint32_t channels, double target, int max_quality, int subsampling,
+                                        rcg_found* out, int16_t* coefs[3], uint8_t qtables[128]) {
+    if (!ctx || !pixels || !out) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    if (width < 8 || height < 8) {
+        set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
+        return RCG_UNSUPPORTED;
+    }
+    const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    const PairPlan P32 = plan_pair(width, height, ctx->tune32);
+    const PairPlan P64 = plan_pair(width, height, ctx->tune64);
+    const size_t src_bytes = size_t(width) * height * channels;
+    const size_t need = search_need(ctx, width, height, channels, subsampling, true);
+    try {
+        Call call(ctx);
+        RCG_TRY(call.begin(need));
+        uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+        RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+        uint8_t* rec[3] = {nullptr, nullptr, nullptr};
+        int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+        float* d_sc[3] = {nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < Lmax.ncomp; c++) {
+            rec[c] = static_cast<uint8_t*>(ctx->arena.take(Lmax.plane_bytes(c)));
+            d_co[c] = static_cast<int16_t*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(int16_t)));
+            d_sc[c] = static_cast<float*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(float)));
+        }
+        const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
+        const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
+        bool a32 = false, a64 = false;
+        TrellisHost H;
+        auto keep = [&](const Layout& Lq) {
+            for (uint32_t c = 0; c < Lq.ncomp; c++) H.best[c] = H.tr[c];
+        };
+        // jpeg/src/target.rs:37-55
+        int lo = 1, hi = std::max(1, std::min(100, max_quality));
+        int steps = 0;
+        double s = 0.0;
+        Layout L = make_layout(width, height, channels, hi, subsampling);
+        RCG_TRY(score_trellis(ctx, call, L, d_src, hi, d_co, d_sc, rec, H, B32, B64, a32, a64, target, s));
+        steps++;
+        keep(L);
+        int best_q = hi;
+        double best_s = s;
+        if (s >= target) {
+            while (lo < hi) {
+                const int mid = lo + (hi - lo) / 2;
+                L = make_layout(width, height, channels, mid, subsampling);
+                RCG_TRY(score_trellis(ctx, call, L, d_src, mid, d_co, d_sc, rec, H, B32, B64, a32, a64, target, s));
+                steps++;
+                if (s >= target) {
+                    hi = mid;
+                    best_q = mid;
+                    best_s = s;
+                    keep(L);
+                } else {
+                    lo = mid + 1;
+                }
+            }
+        }
+        out->quality = best_q;
+        out->score = best_s;
+        out->steps = steps;
+        if (coefs) {
+            const Layout Lb = make_layout(width, height, channels, best_q, subsampling);
+            for (uint32_t c = 0; c < Lb.ncomp; c++)
+                if (coefs[c]) std::memcpy(coefs[c], H.best[c].data(), Lb.coef_count(c) * sizeof(int16_t));
+        }
+        if (qtables) quant_tables(best_q, qtables, qtables + 64);
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return RCG_OOM;
+    }
+    return RCG_OK;
+}
+
+// ================================================================================================================
 // JPEG -> JPEG: the DCT-domain requantisation search (cli/src/pipeline.rs:369-386)
 
 namespace {
+
+// Device bytes of the requantisation search over the block grid L: per component the source and candidate
+// coefficients and sample planes, the decoded reference image, and the FP32 and FP64 SSIMULACRA2 state.
+size_t requant_need(rcg_ctx* ctx, const Layout& L) {
+    const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
+    const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
+    size_t need = Arena::align(size_t(L.w) * L.h * L.channels) + ssim_bytes(P32, RCG_PREC_FP32) +
+                  ssim_bytes(P64, RCG_PREC_FP64);
+    for (uint32_t c = 0; c < L.ncomp; c++) {
+        const size_t nb = size_t(L.bw[c]) * L.bh[c];
+        need += 2 * Arena::align(nb * 64 * sizeof(int16_t)) + 2 * Arena::align(nb * 64);
+    }
+    return need;
+}
 
 // One requantised candidate scored against the decoded source: its sample planes (requantised on the device inside
 // the IDCT, or uploaded after the host trellis), then SSIMULACRA2 with the FP64 re-score inside the guard band.
@@ -1062,11 +1279,10 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
     const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
     const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
     const size_t ref_bytes = size_t(L.w) * L.h * L.channels;
-    size_t need = Arena::align(ref_bytes) + ssim_bytes(P32, RCG_PREC_FP32) + ssim_bytes(P64, RCG_PREC_FP64);
+    const size_t need = requant_need(ctx, L);
     std::vector<std::vector<int16_t>> host_src(ci->ncomp);
     for (uint32_t c = 0; c < ci->ncomp; c++) {
         const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
-        need += 2 * Arena::align(nb * 64 * sizeof(int16_t)) + 2 * Arena::align(nb * 64);
         if (trellis) host_src[c].assign(ci->comps[c].coefs, ci->comps[c].coefs + nb * 64);
     }
     Call call(ctx);
@@ -1137,6 +1353,28 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
     out->score = best_s;
     out->steps = steps;
     if (winner) RCG_TRY(rcg_requantize(ci, best_q, trellis, winner));
+    return RCG_OK;
+}
+
+extern "C" int rcg_search_bytes(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling,
+                                int mode, size_t* out) {
+    if (!ctx || !out) return RCG_INVALID;
+    RCG_TRY(check_image(width, height, channels));
+    size_t need = 0;
+    switch (mode) {
+        case RCG_SEARCH_PLAIN: need = search_need(ctx, width, height, channels, subsampling, false); break;
+        case RCG_SEARCH_TRELLIS: need = search_need(ctx, width, height, channels, subsampling, true); break;
+        case RCG_SEARCH_REQUANT: {
+            // The source's block grid is at most the 4:4:4 one of its size.
+            const Layout L = make_layout(width, height, channels, 100, RCG_S444);
+            need = requant_need(ctx, L);
+            break;
+        }
+        default:
+            set_error("unknown search mode");
+            return RCG_INVALID;
+    }
+    *out = need + kDescBytes + 4096;
     return RCG_OK;
 }
 

@@ -10,7 +10,7 @@
 //   rc_jpeg::target::{Found, find_quality_with}              jpeg/src/target.rs:11-56
 //
 // Header-only; link with librcgpu.so. Every device call goes through the C ABI of include/rcgpu.h. Trellis
-// quantisation is not implemented on the device: build_coefs with trellis = true reports Unsupported.
+// quantisation (EncodeOptions::trellis, on by default as in the reference) runs on the host between device passes.
 #pragma once
 
 #include <array>
@@ -407,12 +407,13 @@ inline Found find_quality_with(Device& dev, const Image& reference, double targe
     return best;
 }
 
-// The CLI's --target-ssimulacra2 search (cli/src/pipeline.rs:207-247) with make = build_coefs (trellis off) and
-// metric = ssimulacra2, run device-resident by rcg_find_quality: one upload, one score read back per step.
+// The CLI's --target-ssimulacra2 search (cli/src/pipeline.rs:207-247) with make = build_coefs(opts) and
+// metric = ssimulacra2, run device-resident: rcg_find_quality with trellis off (one upload, one score read back per
+// step), rcg_find_quality_trellis with the reference's default trellis (the host trellis between the device encode
+// and the device scoring of every step).
 inline Result<Found> find_jpeg_quality(Device& dev, const Image& reference, double target, const EncodeOptions& opts) {
     if (reference.width < MIN_SIDE || reference.height < MIN_SIDE)
         return Error{Error::Unsupported, "--target-ssimulacra2 needs an image of at least 8x8 pixels"};
-    if (opts.trellis) return Error{Error::Unsupported, "trellis quantisation runs on the CPU; set trellis = false"};
     const Image src = reference.flatten(opts.background);
     const uint32_t ch = uint32_t(channels(src.color));
     rcg_layout L;
@@ -427,8 +428,10 @@ inline Result<Found> find_jpeg_quality(Device& dev, const Image& reference, doub
     }
     rcg_found f{};
     uint8_t qt[128];
-    rc = rcg_find_quality(dev.raw(), src.data.data(), src.width, src.height, ch, target, 100,
-                          subsampling_code(opts.subsampling), &f, ptrs, qt);
+    rc = (opts.trellis ? rcg_find_quality_trellis : rcg_find_quality)(dev.raw(), src.data.data(), src.width,
+                                                                       src.height, ch, target, 100,
+                                                                       subsampling_code(opts.subsampling), &f, ptrs,
+                                                                       qt);
     if (rc != RCG_OK) return Error::from_status(rc);
     rcg_layout Lq;
     rcg_coef_layout(src.width, src.height, ch, f.quality, subsampling_code(opts.subsampling), &Lq);

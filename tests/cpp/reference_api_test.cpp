@@ -148,6 +148,26 @@ int main() {
               "q100 is plain");
         std::printf("ok trellis_hand_off\n");
     }
+    {  // the device search with the trellis (the CLI default) equals the generic bisection over make = build_coefs
+       // with trellis, and returns the winner's trellis coefficients
+        Image ref = textured(144, 104);
+        EncodeOptions t = opts(100, Subsampling::Auto);
+        t.trellis = true;
+        auto make = [&](uint8_t q) {
+            EncodeOptions o = t;
+            o.quality = q;
+            return build_coefs(dev, ref, o).value();
+        };
+        auto metric = [&](const Image& a, const Image& b) { return ssimulacra2(dev, a, b).unwrap_or(-1.7976931348623157e308); };
+        for (double target : {60.0, 85.0}) {
+            Found f = find_quality_with(dev, ref, target, 100, make, metric);
+            Result<Found> g = find_jpeg_quality(dev, ref, target, t);
+            CHECK(g.is_ok() && g.value().quality == f.quality, "trellis search %d vs %d", g.value().quality, f.quality);
+            for (size_t c = 0; c < f.coefs.comps.size(); c++)
+                CHECK(g.value().coefs.comps[c].coefs == f.coefs.comps[c].coefs, "winner coefficients, component %zu", c);
+        }
+        std::printf("ok trellis_search_equals_the_generic_bisection\n");
+    }
     {  // JPEG -> JPEG (cli/src/pipeline.rs:369-386): requantising a q92 file towards a target; q100 is lossless
        // (cli/tests/cmyk.rs:38-54) and the device search's winner equals requantize at the chosen quality
         Image img = textured(176, 120);

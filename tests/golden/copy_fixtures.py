@@ -22,3 +22,9 @@ for name in ("rgb8.png", "gray8.png", "rgb8_420.jpg", "rgb8_444.jpg", "gray8.jpg
 # Four-component baseline files of cli/tests/cmyk.rs:38-54 (requantisation at q100 changes no coefficient).
 for name in ("cmyk.jpg", "ycck.jpg", "ycck_444.jpg"):
     shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, name))
+# JPEG sources of the reader (jpeg/tests/jpeg_decode.rs:6,77,126; jpeg/tests/arith.rs:168): baseline, progressive
+# (jpegtran-style scan script with successive approximation) and optimised-table files, plus one each of the frame
+# types the reader describes but does not decode (arithmetic, 12-bit, lossless, hierarchical progressive).
+for name in ("rgb8.jpg", "rgb8_prog.jpg", "rgb8_opt.jpg", "arith_420_seq.jpg", "jpeg12_444.jpg",
+             "lossless_gray_p8_pred1.jpg", "hier_dct_progressive.jpg"):
+    shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, name))

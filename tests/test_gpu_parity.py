@@ -323,6 +323,75 @@ def test_trellis_hand_off_is_bit_exact(ctx, rc, w, h, q, subs):
         np.testing.assert_array_equal(g.coefs, want)
 
 
+def _oracle_trellis_search(img, target, subs):
+    # cli/src/pipeline.rs:238 with the reference's default trellis: make(q) = build_coefs(img, q, trellis on)
+    # (encode.rs:527-537), metric = f64 SSIMULACRA2, bisection of jpeg/src/target.rs:37-55.
+    cache = {}
+
+    def make(q):
+        ci, scaled = O.build_scaled(img, q, subs)
+        if q < 100:
+            for c, t in zip(ci.comps, O.trellis(ci, scaled)):
+                c.coefs = np.asarray(t, np.int16)
+        return ci
+
+    def score(q):
+        if q not in cache:
+            cache[q] = O.ssimulacra2(img, O.reconstruct(make(q)))
+        return cache[q]
+
+    lo, hi = 1, 100
+    best = hi
+    if score(hi) >= target:
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            if score(mid) >= target:
+                hi = mid
+                best = mid
+            else:
+                lo = mid + 1
+    return best, cache, make
+
+
+@pytest.mark.parametrize("w,h,subs,gray,target", [(256, 192, O.S420, False, 70.0), (301, 199, O.AUTO, False, 85.0),
+                                                  (97, 61, O.S444, False, 55.0), (120, 90, O.S420, True, 75.0),
+                                                  (200, 120, O.S422, False, 92.0)])
+def test_trellis_search_chooses_the_oracles_quality(ctx, rc, w, h, subs, gray, target):
+    img = rc.synth_rgb8(700 + w, w, h)
+    if gray:
+        img = np.ascontiguousarray(img[:, :, 0])
+    found = ctx.find_quality(img, target, 100, subs, trellis=True)
+    want, scores, make = _oracle_trellis_search(img, target, subs)
+    assert found.quality == want, (found.quality, want, sorted(scores.items()))
+    assert abs(found.score - scores[want]) <= TOL32
+    assert found.steps == len(scores)
+    for a, b in zip(found.coefs.comps, make(want).comps):
+        np.testing.assert_array_equal(a.coefs, b.coefs)
+
+
+def test_trellis_search_reuses_the_context(ctx, rc):
+    # Two searches in a row on one context (the arena and reference state are reset per call) agree with fresh ones.
+    a = rc.synth_rgb8(1, 160, 120)
+    b = rc.synth_rgb8(2, 96, 64)
+    fa = ctx.find_quality(a, 70.0, 100, rc.S420, trellis=True)
+    fb = ctx.find_quality(b, 70.0, 100, rc.S420, trellis=True)
+    fa2 = ctx.find_quality(a, 70.0, 100, rc.S420, trellis=True)
+    assert (fa.quality, fa.score) == (fa2.quality, fa2.score)
+    want_b, _, _ = _oracle_trellis_search(b, 70.0, O.S420)
+    assert fb.quality == want_b
+
+
+def test_jpeg_reader_feeds_the_requant_search(ctx, rc):
+    # A file read by the product's reader searches like the oracle's view of the same coefficients.
+    path = os.path.join(os.path.dirname(__file__), "golden", "rgb8_prog.jpg")
+    data = open(path, "rb").read()
+    ci = rc.read_jpeg(data)
+    found = ctx.requant_find_quality(ci, 60.0, 100, True)
+    want, _ = _oracle_requant_search(ci, 60.0, True)
+    assert found.quality == want
+    np.testing.assert_array_equal(ctx.reconstruct(ci), O.reconstruct(ci))
+
+
 def _oracle_requant_search(src, target, trellis):
     # cli/src/pipeline.rs:369-386 with the oracle: reference = reconstruct(ci), make(q) = requantize(ci, q, trellis),
     # metric = f64 SSIMULACRA2, bisection of jpeg/src/target.rs:37-55.
