@@ -138,7 +138,9 @@ typedef struct {
  * for --target-ssimulacra2 (cli/src/pipeline.rs:207-247): make = build_coefs(trellis off), metric = ssimulacra2,
  * serial bisection over 1..=max_quality starting at max_quality (target.rs:37-55). Scores use FP32 maps; a score
  * within 0.05 of the target is re-scored in FP64 before the comparison, so the choice equals the reference's.
- * coefs (may be NULL) receive the chosen quality's coefficients, qtables (may be NULL) its tables.
+ * coefs (may be NULL) receive the chosen quality's coefficients, qtables (may be NULL) its tables. Each coefs[c]
+ * must hold the largest grid the search can meet: rcg_coef_layout at quality 100, and under RCG_AUTO also at
+ * quality 1 (4:2:0 pads luma to 16-pixel MCUs, so its luma grid can exceed the 4:4:4 one).
  * RCG_UNSUPPORTED below 8x8 pixels (pipeline.rs:215-217).
  */
 int rcg_find_quality(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,

@@ -365,9 +365,10 @@ class Context:
         h, w = img.shape[:2]
         ch = _channels(img)
         f = _Found()
-        # The chosen quality's layout is only known afterwards; size for the larger 4:4:4 one when Auto.
-        L = coef_layout(w, h, ch, 100, S444 if subsampling == AUTO else subsampling)
-        bufs = [np.zeros(L.blocks_w[c] * L.blocks_h[c] * 64, np.int16) for c in range(L.ncomp)]
+        # The chosen quality's layout is only known afterwards; size for every layout the search can meet (under
+        # Auto both 4:4:4 and 4:2:0, whose 16-pixel MCUs can give luma an extra block row or column).
+        Ls = [coef_layout(w, h, ch, q, subsampling) for q in ((100, 1) if subsampling == AUTO else (100,))]
+        bufs = [np.zeros(max(L.blocks_w[c] * L.blocks_h[c] for L in Ls) * 64, np.int16) for c in range(Ls[0].ncomp)]
         ptrs = (ctypes.c_void_p * 3)(*([b.ctypes.data for b in bufs] + [None] * (3 - len(bufs))))
         qt = np.zeros(128, np.uint8)
         fn = lib().rcg_find_quality_trellis if trellis else lib().rcg_find_quality

@@ -112,6 +112,23 @@ Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int s
     return L;
 }
 
+// The largest block grids a search over every quality can produce. Under Auto a search meets both 4:4:4 (q >= 90)
+// and 4:2:0 (q < 90), and 4:2:0 pads luma to 16-pixel MCUs, so its luma grid can exceed the 4:4:4 one by a block
+// row or column (9x8 pixels: 2x2 blocks against 2x1). Buffers of a search are sized by this layout.
+Layout max_layout(uint32_t w, uint32_t h, uint32_t channels, int subsampling) {
+    Layout L = make_layout(w, h, channels, 100, subsampling);
+    if (subsampling == RCG_AUTO && channels != 1) {
+        const Layout S = make_layout(w, h, channels, 1, subsampling);
+        L.pw = std::max(L.pw, S.pw);
+        L.ph = std::max(L.ph, S.ph);
+        for (int c = 0; c < 3; c++) {
+            L.bw[c] = std::max(L.bw[c], S.bw[c]);
+            L.bh[c] = std::max(L.bh[c], S.bh[c]);
+        }
+    }
+    return L;
+}
+
 PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
     PairPlan P{};
     P.w = w;
@@ -821,7 +838,7 @@ constexpr double kGuard = 0.05;  // |f32 score - target| below this is settled w
 // Device bytes of one target search over a width x height image (sources, candidate planes, coefficients, hand-off
 // values with trellis, and the FP32 and FP64 SSIMULACRA2 state), not counting the descriptor staging.
 size_t search_need(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling, bool trellis) {
-    const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    const Layout Lmax = max_layout(width, height, channels, subsampling);
     const PairPlan P32 = plan_pair(width, height, ctx->tune32);
     const PairPlan P64 = plan_pair(width, height, ctx->tune64);
     size_t need = Arena::align(size_t(width) * height * channels) + plane_bytes(Lmax) + coef_bytes(Lmax) +
@@ -878,8 +895,8 @@ int find_quality_impl(rcg_ctx* ctx, const uint8_t* pixels, bool host_src, uint32
         set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
         return RCG_UNSUPPORTED;
     }
-    // Auto subsampling depends on the quality; size for the larger (4:4:4) layout.
-    const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    // Auto subsampling depends on the quality: size for every layout the search can meet.
+    const Layout Lmax = max_layout(width, height, channels, subsampling);
     const size_t src_bytes = size_t(width) * height * channels;
     const PairPlan P32 = plan_pair(width, height, ctx->tune32);
     const PairPlan P64 = plan_pair(width, height, ctx->tune64);
@@ -1086,7 +1103,7 @@ extern "C" int rcg_find_quality_trellis(rcg_ctx* ctx, const uint8_t* pixels, uin
         set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
         return RCG_UNSUPPORTED;
     }
-    const Layout Lmax = make_layout(width, height, channels, 100, subsampling == RCG_AUTO ? RCG_S444 : subsampling);
+    const Layout Lmax = max_layout(width, height, channels, subsampling);
     const PairPlan P32 = plan_pair(width, height, ctx->tune32);
     const PairPlan P64 = plan_pair(width, height, ctx->tune64);
     const size_t src_bytes = size_t(width) * height * channels;
@@ -1365,8 +1382,9 @@ extern "C" int rcg_search_bytes(rcg_ctx* ctx, uint32_t width, uint32_t height, u
         case RCG_SEARCH_PLAIN: need = search_need(ctx, width, height, channels, subsampling, false); break;
         case RCG_SEARCH_TRELLIS: need = search_need(ctx, width, height, channels, subsampling, true); break;
         case RCG_SEARCH_REQUANT: {
-            // The source's block grid is at most the 4:4:4 one of its size.
-            const Layout L = make_layout(width, height, channels, 100, RCG_S444);
+            // The search takes sources with sampling ratios 1 or 2: their grids are at most the larger of the
+            // 4:4:4 and 4:2:0 ones of that size, per component.
+            const Layout L = max_layout(width, height, channels, RCG_AUTO);
             need = requant_need(ctx, L);
             break;
         }

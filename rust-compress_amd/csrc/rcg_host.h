@@ -116,6 +116,8 @@ struct Layout {
     size_t plane_bytes(int c) const { return size_t(bw[c]) * 8 * bh[c] * 8; }
 };
 Layout make_layout(uint32_t w, uint32_t h, uint32_t channels, int quality, int subsampling);
+// Per-component maxima of the block grids a search over every quality meets (Auto: 4:4:4 and 4:2:0).
+Layout max_layout(uint32_t w, uint32_t h, uint32_t channels, int subsampling);
 void quant_tables(int quality, uint8_t luma[64], uint8_t chroma[64]);
 void fill_cand_tables(const uint8_t qy[64], const uint8_t qc[64], EncCand& c);
 

@@ -121,17 +121,22 @@ struct rcg_server {
     size_t footprint_of(const rcg_request& r) const {
         const PairPlan P = plan_pair(r.width, r.height, ctx->tune32);
         size_t b = Arena::align(size_t(r.width) * r.height * r.channels) + ref_state_bytes(P);
-        Item it;
-        std::memset(&it, 0, sizeof(it));
-        it.w = r.width;
-        it.h = r.height;
-        it.ch = r.channels;
-        it.ncand = uint32_t(std::max(1, opts.speculate));
-        for (uint32_t c = 0; c < it.ncand; c++) {
-            it.quality[c] = 100;  // 4:4:4 under Auto is the largest layout
-            it.subsampling[c] = r.subsampling;
+        // Under Auto a tick's candidates are 4:4:4 (q >= 90) or 4:2:0 (q < 90): charge the larger working set.
+        size_t work = 0;
+        for (int q : {100, 1}) {
+            Item it;
+            std::memset(&it, 0, sizeof(it));
+            it.w = r.width;
+            it.h = r.height;
+            it.ch = r.channels;
+            it.ncand = uint32_t(std::max(1, opts.speculate));
+            for (uint32_t c = 0; c < it.ncand; c++) {
+                it.quality[c] = q;
+                it.subsampling[c] = r.subsampling;
+            }
+            work = std::max(work, score_items_bytes(ctx, &it, 1));
         }
-        return b + score_items_bytes(ctx, &it, 1);
+        return b + work;
     }
 
     void finish(Request* r, double t) {

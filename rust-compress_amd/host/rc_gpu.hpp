@@ -13,6 +13,7 @@
 // quantisation (EncodeOptions::trellis, on by default as in the reference) runs on the host between device passes.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <functional>
@@ -416,14 +417,16 @@ inline Result<Found> find_jpeg_quality(Device& dev, const Image& reference, doub
         return Error{Error::Unsupported, "--target-ssimulacra2 needs an image of at least 8x8 pixels"};
     const Image src = reference.flatten(opts.background);
     const uint32_t ch = uint32_t(channels(src.color));
-    rcg_layout L;
-    int rc = rcg_coef_layout(src.width, src.height, ch, 100,
-                             opts.subsampling == Subsampling::Auto ? RCG_S444 : subsampling_code(opts.subsampling), &L);
+    // The chosen quality's layout is only known afterwards: size for every layout the search can meet (under Auto
+    // both 4:4:4 and 4:2:0, whose 16-pixel MCUs can give luma an extra block row or column).
+    rcg_layout L, L2;
+    int rc = rcg_coef_layout(src.width, src.height, ch, 100, subsampling_code(opts.subsampling), &L);
+    if (rc == RCG_OK) rc = rcg_coef_layout(src.width, src.height, ch, 1, subsampling_code(opts.subsampling), &L2);
     if (rc != RCG_OK) return Error::from_status(rc);
     std::vector<std::vector<int16_t>> bufs(L.ncomp);
     int16_t* ptrs[3] = {nullptr, nullptr, nullptr};
     for (uint32_t c = 0; c < L.ncomp; c++) {
-        bufs[c].resize(size_t(L.blocks_w[c]) * L.blocks_h[c] * 64);
+        bufs[c].resize(std::max(size_t(L.blocks_w[c]) * L.blocks_h[c], size_t(L2.blocks_w[c]) * L2.blocks_h[c]) * 64);
         ptrs[c] = bufs[c].data();
     }
     rcg_found f{};

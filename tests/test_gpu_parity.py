@@ -182,6 +182,25 @@ def test_find_quality_matches_oracle(ctx, rc, target):
         _assert_coefs_equal(found.coefs, O.build_coefs(img, q, O.S420))
 
 
+@pytest.mark.parametrize("w,h", [(9, 8), (8, 9), (24, 8), (8, 40), (17, 9), (67, 45), (1009, 1009)])
+@pytest.mark.parametrize("target", [30.0, 97.0])
+def test_auto_search_where_420_pads_more_than_444(ctx, rc, w, h, target):
+    # Under Auto a search meets 4:4:4 (q >= 90) and 4:2:0 (q < 90); at these sizes the 4:2:0 luma grid, padded to
+    # 16-pixel MCUs, has more blocks than the 4:4:4 one. Every buffer of the search must hold both.
+    img = rc.synth_rgb8(w * 7 + h, w, h)
+    for trellis in (False, True):
+        found = ctx.find_quality(img, target, 100, rc.AUTO, trellis=trellis)
+        if trellis:
+            want, scores, make = _oracle_trellis_search(img, target, O.AUTO)
+            s = scores[want]
+        else:
+            want, s, _ = O.find_quality(img, target, 100, O.AUTO)
+            make = lambda q: O.build_coefs(img, q, O.AUTO)
+        assert found.quality == want, (w, h, trellis, found.quality, want)
+        assert abs(found.score - s) <= TOL32
+        _assert_coefs_equal(found.coefs, make(want))
+
+
 def test_find_quality_rejects_small(ctx, rc):
     with pytest.raises(rc.RcgError) as e:
         ctx.find_quality(np.zeros((7, 100, 3), np.uint8), 70.0)
