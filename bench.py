@@ -72,7 +72,7 @@ def measure_traffic(size):
     import tempfile
     out = tempfile.mkdtemp(prefix="rcg_traffic_")
     step = ["--steps", "1", "--warmup", "0", "--size", str(size), "--no-cpu", "--no-plain", "--no-batch",
-            "--no-serve", "--no-traffic"]
+            "--no-serve", "--no-cli", "--no-traffic"]
     copy = ["--copy-calibration", "512"]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-image search")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-image batched search")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
+    ap.add_argument("--cli-files", type=int, default=24)
     ap.add_argument("--batch-images", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -248,6 +250,9 @@ def main():
     if not args.no_serve:
         progress("serving configuration")
         out["serving"] = bench_serve(rc, args, tune)
+    if not args.no_cli:
+        progress("command-line end-to-end run")
+        out["cli"] = bench_cli(rc, args)
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -382,6 +387,46 @@ def bench_serve(rc, args, tune=None):
             "mean_batch": round(st.mean_batch, 1), "ticks": int(st.ticks),
             "peak_footprint_gib": round(st.peak_footprint / 2 ** 30, 2),
             "failed": int(sum(c.status != 0 for c in done.values()))}
+
+
+def bench_cli(rc, args):
+    """The command-line driver end to end (python -m rust-compress_amd): PNG files in a directory, log-uniform
+    0.3-4 MP, aspect 0.5-2, one --target-ssimulacra2 70 run with the default trellis and one with --no-trellis.
+    Wall time includes reading and decoding the PNGs, the searches, entropy coding and writing the JPEGs."""
+    import contextlib
+    import io as _io
+    import shutil
+    import tempfile
+    import numpy as np
+    from PIL import Image
+    import importlib
+    cli = importlib.import_module("rust-compress_amd.cli")
+    rng = np.random.default_rng(91)
+    root = tempfile.mkdtemp(prefix="rcg-cli-", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        src = os.path.join(root, "in")
+        os.makedirs(src)
+        total_mp = 0.0
+        for i in range(args.cli_files):
+            mp = float(np.exp(rng.uniform(np.log(0.3), np.log(4.0))))
+            aspect = float(rng.uniform(0.5, 2.0))
+            w = max(8, int(round((mp * 1e6 * aspect) ** 0.5)))
+            h = max(8, int(round(mp * 1e6 / w)))
+            Image.fromarray(rc.synth_rgb8(8000 + i, w, h)).save(os.path.join(src, f"f{i:03d}.png"), compress_level=1)
+            total_mp += w * h / 1e6
+        res = {"files": args.cli_files, "megapixels": round(total_mp, 2), "target": 70.0}
+        for name, extra in (("trellis", []), ("no_trellis", ["--no-trellis"])):
+            out = os.path.join(root, name)
+            buf = _io.StringIO()
+            t0 = time.perf_counter()
+            with contextlib.redirect_stdout(buf):
+                code = cli.main([src, "-f", "jpeg", "--target-ssimulacra2", "70", "-o", out, "--quiet"] + extra)
+            t1 = time.perf_counter()
+            res[name] = {"seconds": round(t1 - t0, 3), "files_per_s": round(args.cli_files / (t1 - t0), 2),
+                         "mp_per_s": round(total_mp / (t1 - t0), 2), "exit": code}
+        return res
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
 
 
 if __name__ == "__main__":

@@ -188,10 +188,34 @@ template <class V>
 __device__ __forceinline__ void gstore(V* p, V v) {
     *(RCG_GLOBAL V*)p = v;
 }
+// A pointer that is uniform over the wave (a row pointer built from the workgroup's row and job fields), passed
+// through readfirstlane so that it is an opaque scalar value: the compiler then keeps each access as that base plus
+// a 32-bit offset (the scalar-base form of the global instructions) instead of reassociating base + plane stride +
+// offset into one 64-bit vector add per access. Only for uniform pointers: a per-lane pointer would take lane 0's
+// value.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<T*>((uint64_t(hi) << 32) | lo);
+}
+// Element at byte offset `byte_off` of a uniform row pointer.
+template <class V>
+__device__ __forceinline__ V gload_b(const V* row, uint32_t byte_off) {
+    return *(const RCG_GLOBAL V*)(reinterpret_cast<const char*>(uniform_ptr(row)) + byte_off);
+}
+template <class V>
+__device__ __forceinline__ void gstore_b(V* row, uint32_t byte_off, V v) {
+    *(RCG_GLOBAL V*)(reinterpret_cast<char*>(uniform_ptr(row)) + byte_off) = v;
+}
 
 // Row loaders of linear-light RGB. begin_row() takes the row the whole workgroup is on (uniform), so row pointers
-// are scalar; load() takes the thread's column. The source kind is a template parameter: each kernel instance
-// carries only the loader it uses.
+// are scalar. prep(x) computes the thread's byte offsets within a row once, for its column x; fresh() re-issues them
+// at the top of every row (an empty asm, no instruction), so the compiler forms each access as a uniform row
+// pointer plus a 32-bit offset (scalar-base addressing) rather than hoisting 64-bit offsets out of the walk and
+// adding them to the row pointer with a 64-bit vector add per access. load() reads the thread's column of the
+// current row. The source kind is a template parameter: each kernel instance carries only the loader it uses.
 template <uint32_t K>
 struct Loader;
 
@@ -201,21 +225,23 @@ struct Loader<kSrcBytes> {
     const uint8_t* base;
     const uint8_t* row;
     uint32_t w, ch;
+    uint32_t ox;  // byte offset of the thread's column
     __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t) {
         base = S.bytes;
         ch = S.channels;
         w = w_;
     }
+    __device__ __forceinline__ void prep(uint32_t x) { ox = ch == 1 ? x : 3 * x; }
+    __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox)); }
     __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w * ch; }
-    __device__ __forceinline__ void load(uint32_t x, const float* lut, float out[3]) const {
+    __device__ __forceinline__ void load(const float* lut, float out[3]) const {
         if (ch == 1) {
-            const float l = lut[gload(row + x)];
+            const float l = lut[gload_b(row, ox)];
             out[0] = out[1] = out[2] = l;
         } else {
-            const uint8_t* p = row + 3 * x;
-            out[0] = lut[gload(p)];
-            out[1] = lut[gload(p + 1)];
-            out[2] = lut[gload(p + 2)];
+            out[0] = lut[gload_b(row, ox)];
+            out[1] = lut[gload_b(row + 1, ox)];
+            out[2] = lut[gload_b(row + 2, ox)];
         }
     }
 };
@@ -227,16 +253,19 @@ struct Loader<kSrcPlanes> {
     const float* row;
     uint64_t n;
     uint32_t w;
+    uint32_t ox;
     __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t h) {
         base = S.planes;
         w = w_;
         n = uint64_t(w_) * h;
     }
+    __device__ __forceinline__ void prep(uint32_t x) { ox = 4 * x; }
+    __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox)); }
     __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w; }
-    __device__ __forceinline__ void load(uint32_t x, const float*, float out[3]) const {
-        out[0] = gload(row + x);
-        out[1] = gload(row + n + x);
-        out[2] = gload(row + 2 * n + x);
+    __device__ __forceinline__ void load(const float*, float out[3]) const {
+        out[0] = gload_b(row, ox);
+        out[1] = gload_b(row + n, ox);
+        out[2] = gload_b(row + 2 * n, ox);
     }
 };
 
@@ -249,7 +278,10 @@ struct Loader<kSrcRecon> {
     const uint8_t* pc[2];
     uint32_t stride_y, stride_c, cw, ch, fx, fy, ncomp;
     const uint8_t* ry;
-    uint32_t c0, c1, yodd;  // chroma row offsets (near, far) of the current row
+    const uint8_t* ra[2];   // chroma rows of the current row: nearer
+    const uint8_t* rb[2];   // and farther (equal to ra unless vertically subsampled)
+    uint32_t ox, oi, ofi;   // the thread's luma column, its chroma column and the farther chroma column
+    int32_t rnd2, rnd4;     // the horizontal rounding terms of the thread's column parity
     __device__ __forceinline__ void init(const SsimSrc& S, uint32_t, uint32_t) {
         ncomp = S.ncomp;
         py = S.comp[0].p;
@@ -262,39 +294,53 @@ struct Loader<kSrcRecon> {
         fx = S.comp[1].fx;
         fy = S.comp[1].fy;
     }
+    __device__ __forceinline__ void prep(uint32_t x) {
+        ox = x;
+        if (fx == 1) {
+            oi = ofi = x;
+        } else {
+            oi = x >> 1;
+            ofi = (x & 1) == 0 ? (oi > 0 ? oi - 1 : 0) : min(oi + 1, cw - 1);
+        }
+        rnd2 = (x & 1) ? 2 : 1;
+        rnd4 = (x & 1) ? 7 : 8;
+    }
+    __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox), "+v"(oi), "+v"(ofi)); }
     __device__ __forceinline__ void begin_row(uint32_t r) {
-        yodd = r & 1;
         ry = py + uint64_t(r) * stride_y;
+        uint64_t c0, c1;
         if (fy == 2) {
             const uint32_t jj = r >> 1;
-            const uint32_t far = yodd == 0 ? (jj > 0 ? jj - 1 : 0) : min(jj + 1, ch - 1);
-            c0 = jj * stride_c;
-            c1 = far * stride_c;
+            const uint32_t far = (r & 1) == 0 ? (jj > 0 ? jj - 1 : 0) : min(jj + 1, ch - 1);
+            c0 = uint64_t(jj) * stride_c;
+            c1 = uint64_t(far) * stride_c;
         } else {
-            c0 = c1 = r * stride_c;
+            c0 = c1 = uint64_t(r) * stride_c;
+        }
+        for (int k = 0; k < 2; k++) {
+            ra[k] = pc[k] + c0;
+            rb[k] = pc[k] + c1;
         }
     }
-    __device__ __forceinline__ int32_t chroma(int k, uint32_t x) const {
-        const uint8_t* a = pc[k] + c0;
-        const uint8_t* b = pc[k] + c1;
-        if (fx == 1) return gload(a + x);
-        const uint32_t i = x >> 1;
-        const uint32_t fi = (x & 1) == 0 ? (i > 0 ? i - 1 : 0) : min(i + 1, cw - 1);
+    __device__ __forceinline__ int32_t chroma(int k) const {
+        const uint8_t* a = ra[k];
+        const uint8_t* b = rb[k];
+        if (fx == 1) return gload_b(a, oi);
         if (fy == 2) {
-            const int32_t nearv = 3 * int32_t(gload(a + i)) + int32_t(gload(b + i));
-            const int32_t far = 3 * int32_t(gload(a + fi)) + int32_t(gload(b + fi));
-            return (3 * nearv + far + ((x & 1) ? 7 : 8)) >> 4;
+            const int32_t nearv = 3 * int32_t(gload_b(a, oi)) + int32_t(gload_b(b, oi));
+            const int32_t far = 3 * int32_t(gload_b(a, ofi)) + int32_t(gload_b(b, ofi));
+            return (3 * nearv + far + rnd4) >> 4;
         }
-        return (3 * int32_t(gload(a + i)) + int32_t(gload(a + fi)) + ((x & 1) ? 2 : 1)) >> 2;
+        return (3 * int32_t(gload_b(a, oi)) + int32_t(gload_b(a, ofi)) + rnd2) >> 2;
     }
-    __device__ __forceinline__ void load(uint32_t x, const float* lut, float out[3]) const {
-        const int32_t y = gload(ry + x);
+    __device__ __forceinline__ void load(const float* lut, float out[3]) const {
+        const int32_t y = gload_b(ry, ox);
         if (ncomp == 1) {
             out[0] = out[1] = out[2] = lut[y];
             return;
         }
         int32_t r, g, b;
-        ycc_to_rgb(y, chroma(0, x), chroma(1, x), r, g, b);
+        ycc_to_rgb(y, chroma(0), chroma(1), r, g, b);
         out[0] = lut[r];
         out[1] = lut[g];
         out[2] = lut[b];
@@ -310,25 +356,28 @@ struct Loader<kSrcRefCache> {
     const float* orow;
     uint64_t n;
     uint32_t w;
+    uint32_t ox;
     __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t h) {
         base = S.planes;
         w = w_;
         n = uint64_t(w_) * h;
     }
+    __device__ __forceinline__ void prep(uint32_t x) { ox = 4 * x; }
+    __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox)); }
     __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w; }
-    __device__ __forceinline__ void load(uint32_t x, const float*, float out[3]) const {
-        out[0] = gload(row + x);
-        out[1] = gload(row + n + x);
-        out[2] = gload(row + 2 * n + x);
+    __device__ __forceinline__ void load(const float*, float out[3]) const {
+        out[0] = gload_b(row, ox);
+        out[1] = gload_b(row + n, ox);
+        out[2] = gload_b(row + 2 * n, ox);
     }
     // a, mu_a and sigma_aa at an output row.
     __device__ __forceinline__ void begin_out_row(uint32_t r) { orow = base + uint64_t(r) * w; }
-    __device__ __forceinline__ void load_stats(uint32_t x, float a[3], float mu[3], float sig[3]) const {
+    __device__ __forceinline__ void load_stats(float a[3], float mu[3], float sig[3]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            a[c] = gload(orow + c * n + x);
-            mu[c] = gload(orow + (3 + c) * n + x);
-            sig[c] = gload(orow + (6 + c) * n + x);
+            a[c] = gload_b(orow + c * n, ox);
+            mu[c] = gload_b(orow + (3 + c) * n, ox);
+            sig[c] = gload_b(orow + (6 + c) * n, ox);
         }
     }
 };
@@ -352,6 +401,7 @@ struct Walk {
     const T* taps;
     T (*s_q)[3 * Form<T>::NQ][NT];
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
+    uint32_t po;  // byte offset of the lane's column pair in the next pyramid level
     bool in_img, is_out, want_next, ds_writer;
     int yy;
     T ra[11][3], rb[11][3];  // XYB rings of the reference (cached: (a - b)^2 instead) and of the distorted image
@@ -362,9 +412,10 @@ struct Walk {
 // Next-scale 2x2 mean of one image row (core/src/ssimulacra2.rs:225-247). x0v/x1v hold the pair's left and right
 // columns; the first row of a block is kept in v0/v1 and the block is written on its second row (or on the last row
 // of an odd-height image), summed in f64 row by row, left to right, as the reference does.
+template <bool kUniformDst>
 __device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[3], float v0[3], float v1[3],
                                             bool first_row, bool emit, int gx, int yy, int w, int nw, int nh,
-                                            float* dst) {
+                                            uint32_t po, float* dst) {
     if (first_row) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -375,8 +426,9 @@ __device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[
     if (!emit || !dst) return;
     const int gx0 = gx & ~1;
     const bool two_cols = gx0 + 1 < w;
-    const int oy = yy >> 1, ox = gx0 >> 1;
-    const uint64_t n2 = uint64_t(nw) * nh, o = uint64_t(oy) * nw + ox;
+    const int oy = yy >> 1;  // uniform: yy is the workgroup's row; po = 4 * (gx0 / 2) is the lane's byte offset
+    const uint64_t n2 = uint64_t(nw) * nh;
+    float* orow = dst + uint64_t(oy) * nw;
     // Dividing by the pixel count (1, 2 or 4), as the reference does, is the exact multiplication by its inverse.
     const double inv = first_row ? (two_cols ? 0.5 : 1.0) : (two_cols ? 0.25 : 0.5);
 #pragma unroll
@@ -387,7 +439,10 @@ __device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[
             sum += double(x0v[c]);
             if (two_cols) sum += double(x1v[c]);
         }
-        gstore(dst + c * n2 + o, float(sum * inv));
+        if constexpr (kUniformDst)
+            gstore_b(orow + c * n2, po, float(sum * inv));
+        else
+            gstore(reinterpret_cast<float*>(reinterpret_cast<char*>(orow + c * n2) + po), float(sum * inv));
     }
 }
 
@@ -404,14 +459,17 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
     float la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
     st.la.begin_row(uint32_t(r));
     st.lb.begin_row(uint32_t(r));
+    st.la.fresh();
+    st.lb.fresh();
+    asm volatile("" : "+v"(st.po));
     // Beyond the left/right edges the reference's samples are 0, i.e. -offset in the shifted domain, for both images
     // (so their difference is 0).
     T xa[3], xb[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) xa[c] = xb[c] = -Xyb<T>::off(c);
     if (st.in_img) {
-        st.la.load(uint32_t(st.gx), st.lut, la);
-        st.lb.load(uint32_t(st.gx), st.lut, lb);
+        st.la.load(st.lut, la);
+        st.lb.load(st.lut, lb);
         Xyb<T>::eval(T(lb[0]), T(lb[1]), T(lb[2]), xb[0], xb[1], xb[2]);
         if constexpr (kCached) {
 #pragma unroll
@@ -446,8 +504,10 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
         }
         const bool first_row = ((yy - st.y0) & 1) == 0;
         const bool emit = (!first_row || yy == h - 1) && st.ds_writer;
+        // Cached, every lane writes the distorted image's level (a uniform pointer); uncached, each lane of a pair
+        // writes its own image's level.
         float* dst = odd ? st.b_next : (kCached ? st.b_next : st.a_next);
-        pyramid_row(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, dst);
+        pyramid_row<kCached>(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.po, dst);
     }
     if (yy >= st.y0 + 5) {
         // Vertical blur for output row yy-5: centre in slot (S+6)%11, distance t in (S+6-t)%11 and (S+6+t)%11.
@@ -474,7 +534,7 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
             if constexpr (kCached) {
                 // The reference's a, mu_a and sigma_aa at this output pixel, loaded ahead of the horizontal blur.
                 st.la.begin_out_row(uint32_t(yy - 5));
-                st.la.load_stats(uint32_t(st.gx), ac, ma, sa);
+                st.la.load_stats(ac, ma, sa);
             }
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -550,6 +610,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     st.gx = st.x0 - 6 + tid;
     st.in_img = st.gx >= 0 && st.gx < st.w;
     st.is_out = tid >= 6 && tid < 6 + OW && st.gx < st.w;
+    // Offsets of the lane's column (used only where the lane is inside the image).
+    st.la.prep(uint32_t(st.gx));
+    st.lb.prep(uint32_t(st.gx));
+    st.po = 4u * (uint32_t(st.gx & ~1) >> 1);
     st.y_end = min(st.y0 + TH, st.h);
     st.ylast = st.y_end + 4;
     st.want_next = st.a_next != nullptr || st.b_next != nullptr;
@@ -607,6 +671,7 @@ struct RefWalk {
     float (*s_q)[6][NT];
     uint64_t n;
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
+    uint32_t oc, po;  // byte offsets of the lane's column in the cache planes and in the next pyramid level
     bool in_img, is_out, ds_writer;
     int yy;
     float ra[11][3];
@@ -621,8 +686,10 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
     const int r = yy < 0 ? -yy - 1 : (yy >= h ? 2 * h - 1 - yy : yy);
     float la[3] = {0, 0, 0};
     st.la.begin_row(uint32_t(r));
+    st.la.fresh();
+    asm volatile("" : "+v"(st.oc), "+v"(st.po));
     if (st.in_img) {
-        st.la.load(uint32_t(st.gx), st.lut, la);
+        st.la.load(st.lut, la);
         Xyb<float>::eval(la[0], la[1], la[2], st.ra[S][0], st.ra[S][1], st.ra[S][2]);
     } else {
 #pragma unroll
@@ -638,7 +705,7 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
         }
         const bool first_row = ((yy - st.y0) & 1) == 0;
         const bool emit = (!first_row || yy == h - 1) && st.ds_writer;
-        pyramid_row(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.a_next);
+        pyramid_row<true>(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.po, st.a_next);
     }
     if (yy >= st.y0 + 5) {
         constexpr int SC = (S + 6) % 11;
@@ -660,7 +727,7 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
         }
         __syncthreads();
         if (st.is_out) {
-            float* o = st.cache + uint64_t(yy - 5) * st.w + st.gx;
+            float* o = st.cache + uint64_t(yy - 5) * st.w;  // uniform row; the lane's column is the offset
             const uint64_t n = st.n;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -673,9 +740,9 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                     for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t] + row[t], sum);
                     v[i] = sum;
                 }
-                gstore(o + c * n, st.ra[SC][c]);
-                gstore(o + (3 + c) * n, v[0]);
-                gstore(o + (6 + c) * n, mad(-v[0], v[0], v[1]));
+                gstore_b(o + c * n, st.oc, st.ra[SC][c]);
+                gstore_b(o + (3 + c) * n, st.oc, v[0]);
+                gstore_b(o + (6 + c) * n, st.oc, mad(-v[0], v[0], v[1]));
             }
         }
     }
@@ -713,6 +780,9 @@ __global__ __launch_bounds__(NT) void k_ssim_ref(const RefJob* __restrict__ jobs
     st.gx = st.x0 - 6 + tid;
     st.in_img = st.gx >= 0 && st.gx < st.w;
     st.is_out = tid >= 6 && tid < 6 + OW && st.gx < st.w;
+    st.la.prep(uint32_t(st.gx));
+    st.oc = 4u * uint32_t(st.gx);
+    st.po = 4u * (uint32_t(st.gx & ~1) >> 1);
     st.y_end = min(st.y0 + int(J.th), st.h);
     st.ylast = st.y_end + 4;
     st.nw = (st.w + 1) >> 1;
