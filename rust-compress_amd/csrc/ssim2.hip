@@ -387,6 +387,35 @@ __device__ __forceinline__ float swap_pair(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 
+// Where a thread keeps the first row of its current 2x2 pyramid blocks (k = c: left column of channel c, k = 3 + c:
+// the right one). The candidate kernels keep it in a per-thread LDS slot, registers being their scarcer resource;
+// the reference kernel, short of LDS occupancy rather than registers, keeps it in registers.
+template <int NT>
+struct PyrLds {
+    float* p;  // the thread's slot: element k at p[k * NT]
+    __device__ __forceinline__ void set(int k, float v) { p[k * NT] = v; }
+    __device__ __forceinline__ float get(int k) const { return p[k * NT]; }
+};
+struct PyrRegs {
+    float v[6];
+    __device__ __forceinline__ void set(int k, float x) { v[k] = x; }
+    __device__ __forceinline__ float get(int k) const { return v[k]; }
+};
+
+// LDS layout of the vertically blurred quantities exchanged for the horizontal blur (two row buffers). The f32
+// kernels interleave them per column (quantity q of column x at x * QS + q, QS odd: the lanes of a wave hit distinct
+// banks), so all taps of all quantities lie within the immediate-offset reach of one base address; the f64 kernel
+// keeps one plane per quantity.
+template <int NQS, int NT, bool kInterleaved>
+struct QBuf {
+    static constexpr int QS = kInterleaved ? (NQS | 1) : NQS;     // column stride (interleaved) or plane count
+    static constexpr int kBuf = QS * NT;                         // elements per row buffer
+    static constexpr int kCol = kInterleaved ? QS : 1;           // distance between neighbouring columns
+    static __device__ __forceinline__ int at(int buf, int q, int x) {
+        return buf * kBuf + (kInterleaved ? x * QS + q : q * NT + x);
+    }
+};
+
 // Per-workgroup constants and per-thread state of the scale kernel's row walk. The f32 kernel reads the reference
 // side from its cache (KA = kSrcRefCache); the f64 kernel computes both sides from pixels or pyramid planes.
 template <typename T, int NT, uint32_t KA, uint32_t KB>
@@ -399,28 +428,29 @@ struct Walk {
     float* b_next;
     const float* lut;
     const T* taps;
-    T (*s_q)[3 * Form<T>::NQ][NT];
+    using Q = QBuf<3 * Form<T>::NQ, NT, sizeof(T) == 4>;
+    T* s_q;
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
     uint32_t po;  // byte offset of the lane's column pair in the next pyramid level
     bool in_img, is_out, want_next, ds_writer;
     int yy;
     T ra[11][3], rb[11][3];  // XYB rings of the reference (cached: (a - b)^2 instead) and of the distorted image
     T acc[18];
-    float v0[3], v1[3];  // first row of the current 2x2 blocks: left and right column, this lane's image
+    PyrLds<NT> pv;  // first row of the current 2x2 pyramid blocks
 };
 
 // Next-scale 2x2 mean of one image row (core/src/ssimulacra2.rs:225-247). x0v/x1v hold the pair's left and right
-// columns; the first row of a block is kept in v0/v1 and the block is written on its second row (or on the last row
-// of an odd-height image), summed in f64 row by row, left to right, as the reference does.
-template <bool kUniformDst>
-__device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[3], float v0[3], float v1[3],
-                                            bool first_row, bool emit, int gx, int yy, int w, int nw, int nh,
-                                            uint32_t po, float* dst) {
+// columns; the first row of a block is kept in pv and the block is written on its second row (or on the last row of
+// an odd-height image), summed in f64 row by row, left to right, as the reference does.
+template <bool kUniformDst, class P>
+__device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[3], P& pv, bool first_row,
+                                            bool emit, int gx, int yy, int w, int nw, int nh, uint32_t po,
+                                            float* dst) {
     if (first_row) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            v0[c] = x0v[c];
-            v1[c] = x1v[c];
+            pv.set(c, x0v[c]);
+            pv.set(3 + c, x1v[c]);
         }
     }
     if (!emit || !dst) return;
@@ -433,8 +463,8 @@ __device__ __forceinline__ void pyramid_row(const float x0v[3], const float x1v[
     const double inv = first_row ? (two_cols ? 0.5 : 1.0) : (two_cols ? 0.25 : 0.5);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        double sum = double(v0[c]);
-        if (two_cols) sum += double(v1[c]);
+        double sum = double(pv.get(c));
+        if (two_cols) sum += double(pv.get(3 + c));
         if (!first_row) {
             sum += double(x0v[c]);
             if (two_cols) sum += double(x1v[c]);
@@ -507,7 +537,7 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
         // Cached, every lane writes the distorted image's level (a uniform pointer); uncached, each lane of a pair
         // writes its own image's level.
         float* dst = odd ? st.b_next : (kCached ? st.b_next : st.a_next);
-        pyramid_row<kCached>(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.po, dst);
+        pyramid_row<kCached>(x0v, x1v, st.pv, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.po, dst);
     }
     if (yy >= st.y0 + 5) {
         // Vertical blur for output row yy-5: centre in slot (S+6)%11, distance t in (S+6-t)%11 and (S+6+t)%11.
@@ -526,7 +556,7 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
                 Form<T>::pair(k[t], st.ra[su][c], st.ra[sd][c], st.rb[su][c], st.rb[sd][c], q);
             }
 #pragma unroll
-            for (int i = 0; i < NQ; i++) st.s_q[buf][c * NQ + i][tid] = q[i];
+            for (int i = 0; i < NQ; i++) st.s_q[Walk<T, NT, KA, KB>::Q::at(buf, c * NQ + i, tid)] = q[i];
         }
         __syncthreads();
         if (st.is_out) {
@@ -538,13 +568,17 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
             }
 #pragma unroll
             for (int c = 0; c < 3; c++) {
+                // One channel at a time: the scheduler may not pull the next channel's LDS reads ahead of this
+                // channel's arithmetic, which would keep all 99 taps live at once.
+                __builtin_amdgcn_sched_barrier(0);
                 T v[NQ];
 #pragma unroll
                 for (int i = 0; i < NQ; i++) {
-                    const T* row = &st.s_q[buf][c * NQ + i][tid];
+                    using Q = typename Walk<T, NT, KA, KB>::Q;
+                    const T* row = &st.s_q[Q::at(buf, c * NQ + i, tid)];
                     T sum = k[0] * row[0];
 #pragma unroll
-                    for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t] + row[t], sum);
+                    for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t * Q::kCol] + row[t * Q::kCol], sum);
                     v[i] = sum;
                 }
                 T a0, m1, d;
@@ -581,9 +615,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     const SsimJob* __restrict__ jobs, const Unit* __restrict__ units, Taps<T> taps, const float* __restrict__ lut_g) {
     constexpr int OW = NT - 12;
     constexpr int NWAVES = NT / 64;
-    __shared__ T s_q[2][3 * Form<T>::NQ][NT];
+    __shared__ T s_q[2 * Walk<T, NT, KA, KB>::Q::kBuf];
     __shared__ float s_lut[256];
     __shared__ double s_red[NWAVES][18];
+    __shared__ float s_pyr[6 * NT];
 
     const Unit u = units ? units[blockIdx.x] : Unit{0u, blockIdx.x};  // no table: one job, tiles in order
     if (u.job == kIdleJob) return;  // uniform over the workgroup, before any barrier
@@ -626,8 +661,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     st.yy = st.y0 - 5;
 #pragma unroll
     for (int i = 0; i < 18; i++) st.acc[i] = T(0);
+    st.pv.p = s_pyr + tid;
 #pragma unroll
-    for (int c = 0; c < 3; c++) st.v0[c] = st.v1[c] = 0.0f;
+    for (int k = 0; k < 6; k++) st.pv.set(k, 0.0f);
     __syncthreads();
 
     while (walk_row<0>(st) && walk_row<1>(st) && walk_row<2>(st) && walk_row<3>(st) && walk_row<4>(st) &&
@@ -668,14 +704,15 @@ struct RefWalk {
     float* cache;
     const float* lut;
     const float* taps;
-    float (*s_q)[6][NT];
+    using Q = QBuf<6, NT, false>;  // plane per quantity: the interleaved form needs more registers than it saves here
+    float* s_q;
     uint64_t n;
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
     uint32_t oc, po;  // byte offsets of the lane's column in the cache planes and in the next pyramid level
     bool in_img, is_out, ds_writer;
     int yy;
     float ra[11][3];
-    float v0[3], v1[3];
+    PyrRegs pv;  // first row of the current 2x2 pyramid blocks
 };
 
 template <int S, int NT, uint32_t KA>
@@ -705,7 +742,7 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
         }
         const bool first_row = ((yy - st.y0) & 1) == 0;
         const bool emit = (!first_row || yy == h - 1) && st.ds_writer;
-        pyramid_row<true>(x0v, x1v, st.v0, st.v1, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.po, st.a_next);
+        pyramid_row<true>(x0v, x1v, st.pv, first_row, emit, st.gx, yy, st.w, st.nw, st.nh, st.po, st.a_next);
     }
     if (yy >= st.y0 + 5) {
         constexpr int SC = (S + 6) % 11;
@@ -722,8 +759,8 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                 m = mad(k[t], au + ad, m);
                 q = mad(k[t], mad(au, au, ad * ad), q);
             }
-            st.s_q[buf][2 * c][tid] = m;
-            st.s_q[buf][2 * c + 1][tid] = q;
+            st.s_q[RefWalk<NT, KA>::Q::at(buf, 2 * c, tid)] = m;
+            st.s_q[RefWalk<NT, KA>::Q::at(buf, 2 * c + 1, tid)] = q;
         }
         __syncthreads();
         if (st.is_out) {
@@ -734,10 +771,11 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                 float v[2];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
-                    const float* row = &st.s_q[buf][2 * c + i][tid];
+                    using Q = typename RefWalk<NT, KA>::Q;
+                    const float* row = &st.s_q[Q::at(buf, 2 * c + i, tid)];
                     float sum = k[0] * row[0];
 #pragma unroll
-                    for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t] + row[t], sum);
+                    for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t * Q::kCol] + row[t * Q::kCol], sum);
                     v[i] = sum;
                 }
                 gstore_b(o + c * n, st.oc, st.ra[SC][c]);
@@ -754,7 +792,7 @@ template <int NT, uint32_t KA>
 __global__ __launch_bounds__(NT) void k_ssim_ref(const RefJob* __restrict__ jobs, const Unit* __restrict__ units,
                                                  Taps<float> taps, const float* __restrict__ lut_g) {
     constexpr int OW = NT - 12;
-    __shared__ float s_q[2][6][NT];
+    __shared__ float s_q[2 * RefWalk<NT, KA>::Q::kBuf];
     __shared__ float s_lut[256];
     const Unit u = units ? units[blockIdx.x] : Unit{0u, blockIdx.x};  // no table: one job, tiles in order
     if (u.job == kIdleJob) return;
@@ -790,7 +828,7 @@ __global__ __launch_bounds__(NT) void k_ssim_ref(const RefJob* __restrict__ jobs
     st.ds_writer = tid >= 6 && tid < 6 + OW && st.gx < st.w && (st.gx & 1) == 0;
     st.yy = st.y0 - 5;
 #pragma unroll
-    for (int c = 0; c < 3; c++) st.v0[c] = st.v1[c] = 0.0f;
+    for (int k = 0; k < 6; k++) st.pv.set(k, 0.0f);
     __syncthreads();
     while (ref_row<0>(st) && ref_row<1>(st) && ref_row<2>(st) && ref_row<3>(st) && ref_row<4>(st) &&
            ref_row<5>(st) && ref_row<6>(st) && ref_row<7>(st) && ref_row<8>(st) && ref_row<9>(st) &&
