@@ -244,6 +244,8 @@ def main():
                                "kind": "port",
                                "sample": f"{len(sample)} candidate evaluations (q50 4:2:0, q90 4:4:4) of the same "
                                          f"{W}x{H} image: oracle build_coefs + reconstruct + f64 SSIMULACRA2"}
+        progress("scores against the f64 oracle")
+        out["precision"] = precision_report(rc, ctx, O, img, cands, r0)
     if not args.no_batch:
         progress("batched configuration")
         out["batched"] = bench_batch(rc, ctx, args)
@@ -258,6 +260,35 @@ def main():
 
 
 CANDS_B = (40, 50, 60, 70, 80, 85, 90, 95)
+
+
+def precision_report(rc, ctx, O, img, cands, r0):
+    """Every score of the headline step, and the precision configuration (2048x2048 and 3000x2001 synthetic images
+    against their q60 4:2:0 decode), with its error against the f64 oracle. The distorted images are the device's
+    reconstructions (bit-exact with the oracle's); the oracle only scores them."""
+    import numpy as np
+    head = []
+    for (q, s), got in zip(cands, list(r0.scores)[:len(cands)]):
+        rec = ctx.reconstruct(ctx.build_coefs(img, q, s))
+        want = O.ssimulacra2(img, rec)
+        head.append({"quality": q, "subsampling": "4:4:4" if s == rc.S444 else "4:2:0", "fp32": round(got, 6),
+                     "oracle_f64": round(want, 6), "abs_err": float(f"{abs(got - want):.3e}")})
+    configs = []
+    for w, h in ((2048, 2048), (3000, 2001)):
+        src = rc.synth_rgb8(11, w, h)
+        dist = ctx.reconstruct(ctx.build_coefs(src, 60, rc.S420))
+        want, wst = O.ssimulacra2(src, dist, with_stats=True)
+        s32, st32 = ctx.ssimulacra2(src, dist, rc.FP32, with_stats=True)
+        s64, st64 = ctx.ssimulacra2(src, dist, rc.FP64, with_stats=True)
+        configs.append({"size": f"{w}x{h}", "oracle_f64": round(want, 6), "fp32": round(s32, 6),
+                        "fp64": round(s64, 9), "abs_err_fp32": float(f"{abs(s32 - want):.3e}"),
+                        "abs_err_fp64": float(f"{abs(s64 - want):.3e}"),
+                        "per_scale_max_abs_err_fp32": [float(f"{v:.3e}") for v in
+                                                       np.abs(st32 - wst).max(axis=(1, 2))],
+                        "per_scale_max_abs_err_fp64": [float(f"{v:.3e}") for v in
+                                                       np.abs(st64 - wst).max(axis=(1, 2))]})
+    return {"bound": 0.02, "headline": head, "max_abs_err_headline": max(x["abs_err"] for x in head),
+            "configs": configs}
 
 
 def bench_batch(rc, ctx, args):

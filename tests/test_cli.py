@@ -359,3 +359,23 @@ def test_many_files_in_one_run_get_what_each_gets_alone(tmp_path, capsys, extra)
         alone = capsys.readouterr().out
         assert _note_quality(line)[0] == _note_quality(alone)[0]
         assert open(os.path.join(out, f"f{i}.jpg"), "rb").read() == open(alone_out, "rb").read()
+
+
+@pytest.mark.gpu
+def test_an_exif_orientation_is_baked_into_the_pixels(tmp_path, capsys):
+    # pipeline.rs:366-420: a JPEG with an orientation to apply leaves the DCT domain; the decoded pixels are turned
+    # upright (core/src/image.rs:194-219) and searched as pixels, so a 6 (90 degrees clockwise) swaps the sides.
+    img = rc.synth_rgb8(31, 96, 64)
+    ex = Image.Exif()
+    ex[0x0112] = 6
+    src = os.path.join(tmp_path, "rot.jpg")
+    Image.fromarray(img).save(src, "JPEG", quality=98, exif=ex.tobytes())
+    out = os.path.join(tmp_path, "o.jpg")
+    assert cli.main([src, "--target-ssimulacra2", "60", "-o", out, "--no-trellis"]) == 0
+    text = capsys.readouterr().out
+    assert "(no gain" not in text  # a q98 source searched down to 60 shrinks, so the re-encode is written
+    q, _ = _note_quality(text)
+    assert Image.open(out).size == (64, 96)
+    upright = np.ascontiguousarray(cli.upright(O.reconstruct(rc.read_jpeg(open(src, "rb").read())), 6))
+    want_q, _ = _oracle_search(upright, 60.0, lambda qq: O.build_coefs(upright, qq, O.AUTO))
+    assert q == want_q

@@ -206,3 +206,37 @@ def test_truncated_entropy_data_decodes_with_zero_bits():
     cut = good[: len(good) - 200] + b"\xff\xd9"
     a, b = rc.read_jpeg(good), rc.read_jpeg(cut)
     np.testing.assert_array_equal(a.comps[0].coefs[:64 * 4], b.comps[0].coefs[:64 * 4])
+
+
+@pytest.mark.parametrize("name", ["rgb8.jpg", "rgb8_prog.jpg", "ycck.jpg", "arith_420_seq.jpg"])
+def test_hostile_inputs_fail_cleanly(name):
+    # Truncations, byte flips and random runs over real files (cli/tests/hostile.rs feeds the reference the same
+    # kind): every call returns OK, Invalid or Unsupported, and a file that parses reads into buffers of its size.
+    rng = np.random.default_rng(len(name))
+    src = np.frombuffer(_read(name), np.uint8)
+    for it in range(150):
+        d = src.copy()
+        kind = it % 3
+        if kind == 0:
+            d = d[: int(rng.integers(0, len(d) + 1))]
+        elif kind == 1:
+            idx = rng.integers(0, len(d), size=int(rng.integers(1, 9)))
+            d[idx] ^= rng.integers(1, 256, size=len(idx)).astype(np.uint8)
+        else:
+            a = int(rng.integers(0, len(d)))
+            n = int(rng.integers(1, 64))
+            d[a:a + n] = rng.integers(0, 256, size=len(d[a:a + n])).astype(np.uint8)
+        data = d.tobytes()
+        try:
+            info = rc.jpeg_info(data)
+        except rc.RcgError as e:
+            assert e.status in (rc.INVALID, rc.UNSUPPORTED)
+            continue
+        if sum(bw * bh for bw, bh in info.blocks) > 1 << 22:
+            continue
+        try:
+            ci = rc.read_jpeg(data, info)
+        except rc.RcgError as e:
+            assert e.status in (rc.INVALID, rc.UNSUPPORTED)
+            continue
+        assert all(len(c.coefs) == bw * bh * 64 for c, (bw, bh) in zip(ci.comps, info.blocks))
