@@ -354,6 +354,7 @@ class Gpu:
         self._waiters = {}
         self._next_id = 0
         self._poller = None
+        self._poll_error = None
         self._stop = False
 
     def _context(self):
@@ -394,6 +395,8 @@ class Gpu:
                 self._server = Server(self.device, hbm_budget=self.budget.limit)
                 self._poller = threading.Thread(target=self._poll, daemon=True)
                 self._poller.start()
+            if self._poll_error is not None:
+                raise self._poll_error
             self._next_id += 1
             rid = self._next_id
             ev = threading.Event()
@@ -409,12 +412,21 @@ class Gpu:
             return self._waiters.pop(rid)[1]
 
     def _poll(self):
-        while not self._stop:
-            for c in self._server.poll(64, 50):
-                with self._lock:
-                    w = self._waiters.get(c.id)
-                if w is not None:
-                    w[1] = (c.status, c.quality, c.score)
+        try:
+            while not self._stop:
+                for c in self._server.poll(64, 50):
+                    with self._lock:
+                        w = self._waiters.get(c.id)
+                    if w is not None:
+                        w[1] = (c.status, c.quality, c.score)
+                        w[0].set()
+        except RcgError as e:
+            # A failing poll ends the serving loop: every waiting search, and every later one, fails with it.
+            with self._lock:
+                self._poll_error = e
+                for w in self._waiters.values():
+                    if w[1] is None:
+                        w[1] = (e.status, 0, 0.0)
                     w[0].set()
 
     def close(self):
@@ -584,6 +596,9 @@ class Pipeline:
             r.status, r.error = "failed", str(_from_rcg(e))
         except OSError as e:
             r.status, r.error = "failed", f"i/o error: {e.strerror or e}"
+        except (MemoryError, ValueError) as e:
+            # One file's failure stays that file's (main.rs:57-75 reports per file and carries on).
+            r.status, r.error = "failed", f"internal error: {type(e).__name__}: {e}"
         return r
 
 

@@ -379,3 +379,30 @@ def test_an_exif_orientation_is_baked_into_the_pixels(tmp_path, capsys):
     upright = np.ascontiguousarray(cli.upright(O.reconstruct(rc.read_jpeg(open(src, "rb").read())), 6))
     want_q, _ = _oracle_search(upright, 60.0, lambda qq: O.build_coefs(upright, qq, O.AUTO))
     assert q == want_q
+
+
+def test_a_failing_serving_loop_fails_its_searches_instead_of_hanging(monkeypatch):
+    class FailingServer:
+        def __init__(self, *a, **k):
+            pass
+
+        def submit(self, *a, **k):
+            pass
+
+        def poll(self, *a, **k):
+            time.sleep(0.05)
+            raise rc.RcgError(rc.DEVICE, "poll failed")
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(cli, "Server", FailingServer)
+    gpu = cli.Gpu(0, 1, 1 << 30)
+    results = []
+    t = threading.Thread(target=lambda: results.append(gpu.search(np.zeros((8, 8, 3), np.uint8), 70.0, rc.AUTO)))
+    t.start()
+    t.join(10)
+    assert not t.is_alive() and results and results[0][0] == rc.DEVICE
+    with pytest.raises(rc.RcgError):
+        gpu.search(np.zeros((8, 8, 3), np.uint8), 70.0, rc.AUTO)
+    gpu.close()
