@@ -295,23 +295,28 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
 
 // The reference-cache launches (when the pair builds the reference state) go to the side stream, which the caller
 // has forked from the context's stream (fork_side()) when st.build_ref; work enqueued on the context's stream after
-// the fork runs beside them, and launch_staged_ssim() joins them.
+// the fork runs beside them. launch_staged_ssim() starts scale 0 once the scale-0 cache is built (ev_mid) and joins
+// the side stream before scale 1.
 int launch_staged_refs(rcg_ctx* ctx, const StagedSsim& st) {
     if (!st.build_ref) return RCG_OK;
-    for (uint32_t s = 0; s < st.P.nscales; s++)
+    for (uint32_t s = 0; s < st.P.nscales; s++) {
         RCG_TRY(launch_ssim_ref(ctx, st.ref_jobs[s], nullptr, st.P.tiles_ref[s], int(st.P.nt[s]),
                                 s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes), ctx->side));
+        if (s == 0) RCG_TRY(mark_side(ctx));
+    }
     return RCG_OK;
 }
 
 int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
     const bool f64 = st.precision == RCG_PREC_FP64;
-    if (st.build_ref) RCG_TRY(join_side(ctx));
+    if (st.build_ref) RCG_TRY(wait_side_mark(ctx));
     for (uint32_t s = 0; s < st.P.nscales; s++) {
+        if (s == 1 && st.build_ref) RCG_TRY(join_side(ctx));
         const uint32_t ka = !f64 ? uint32_t(kSrcRefCache) : (s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes));
         RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], nullptr, st.P.tiles[s], int(st.P.nt[s]), ka,
                                   s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes)));
     }
+    if (st.build_ref && st.P.nscales < 2) RCG_TRY(join_side(ctx));
     return launch_finalize(ctx, st.fin, 1);
 }
 
@@ -523,6 +528,7 @@ int rcg_create(int device, rcg_ctx** out) {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         set_error(std::string("stream/event creation: ") + hipGetErrorString(e));
@@ -566,6 +572,7 @@ void rcg_destroy(rcg_ctx* ctx) {
     if (ctx->pinned.base) (void)hipHostFree(ctx->pinned.base);
     if (ctx->d_lut) (void)hipFree(ctx->d_lut);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_mid) (void)hipEventDestroy(ctx->ev_mid);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -78,6 +78,16 @@ int join_side(rcg_ctx* ctx) {
     return RCG_OK;
 }
 
+int mark_side(rcg_ctx* ctx) {
+    RCG_HIP(hipEventRecord(ctx->ev_mid, ctx->side));
+    return RCG_OK;
+}
+
+int wait_side_mark(rcg_ctx* ctx) {
+    RCG_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_mid, 0));
+    return RCG_OK;
+}
+
 }  // namespace rcg
 
 extern "C" int rcg_profile(rcg_ctx* ctx, int enable) {
@@ -211,6 +221,7 @@ size_t desc_bytes(rcg_ctx* ctx, const Item* items, int n) {
 // items whose reference state is not built) and one scale launch; then one pooling launch.
 struct StagedLaunch {
     bool ref;
+    int scale;
     const void* jobs;
     const Unit* units;
     uint32_t nunits;
@@ -257,6 +268,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     const std::vector<Unit> units = xcd_units(groups);
                     StagedLaunch g;
                     g.ref = true;
+                    g.scale = s;
                     g.jobs = D.put(rj.data(), rj.size());
                     g.units = D.put(units.data(), units.size());
                     g.nunits = uint32_t(units.size());
@@ -307,6 +319,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                 const std::vector<Unit> units = xcd_units(groups);
                 StagedLaunch g;
                 g.ref = false;
+                g.scale = s;
                 g.jobs = D.put(sj.data(), sj.size());
                 g.units = D.put(units.data(), units.size());
                 g.nunits = uint32_t(units.size());
@@ -325,19 +338,40 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
     return out;
 }
 
-// The reference-cache launches go to the side stream, which the caller has forked from the context's stream; the
-// caller enqueues work that does not need them (the encode) on the context's stream and join_side()s before
-// launch_pairs().
+// The reference-cache launches go to the side stream, which the caller has forked from the context's stream: the
+// scale-0 ones, then ev_mid (mark_side), then the later scales. The caller enqueues work that does not need them (the
+// encode) on the context's stream, then launch_pairs() with `refs` set.
 int launch_refs_side(rcg_ctx* ctx, const StagedPairs& sp) {
-    for (const StagedLaunch& g : sp.refs)
+    size_t k = 0;
+    for (; k < sp.refs.size() && sp.refs[k].scale == 0; k++) {
+        const StagedLaunch& g = sp.refs[k];
         RCG_TRY(launch_ssim_ref(ctx, static_cast<const RefJob*>(g.jobs), g.units, g.nunits, g.nt, g.ka, ctx->side));
+    }
+    RCG_TRY(mark_side(ctx));
+    for (; k < sp.refs.size(); k++) {
+        const StagedLaunch& g = sp.refs[k];
+        RCG_TRY(launch_ssim_ref(ctx, static_cast<const RefJob*>(g.jobs), g.units, g.nunits, g.nt, g.ka, ctx->side));
+    }
     return RCG_OK;
 }
 
-int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp) {
-    for (const StagedLaunch& g : sp.scales)
+// The candidate launches, scale by scale, then the pooling launch. With `refs` (launch_refs_side() enqueued this
+// pass's reference state), the scale-0 launches wait for the scale-0 reference caches only and the later ones for
+// the whole side stream, so the later reference scales are built while scale 0 is scored.
+int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp, bool refs) {
+    size_t k = 0;
+    if (refs) RCG_TRY(wait_side_mark(ctx));
+    for (; k < sp.scales.size() && sp.scales[k].scale == 0; k++) {
+        const StagedLaunch& g = sp.scales[k];
         RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
                                   g.ka, g.kb));
+    }
+    if (refs) RCG_TRY(join_side(ctx));
+    for (; k < sp.scales.size(); k++) {
+        const StagedLaunch& g = sp.scales[k];
+        RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
+                                  g.ka, g.kb));
+    }
     return launch_finalize(ctx, sp.fin, sp.npairs);
 }
 
@@ -471,9 +505,10 @@ int enqueue_items(rcg_ctx* ctx, const ScoreState& S) {
     if (refs) RCG_TRY(fork_side(ctx));
     if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
     if (refs) RCG_TRY(launch_refs_side(ctx, S.fp32));
-    if (refs) RCG_TRY(join_side(ctx));
-    if (!S.pairs.empty()) {
-        RCG_TRY(launch_pairs(ctx, S.fp32));
+    if (S.pairs.empty()) {
+        if (refs) RCG_TRY(join_side(ctx));
+    } else {
+        RCG_TRY(launch_pairs(ctx, S.fp32, refs));
         RCG_HIP(hipMemcpyAsync(S.h_scores, S.d_s32, S.pairs.size() * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));
     }
@@ -496,7 +531,7 @@ int finish_items(rcg_ctx* ctx, ScoreState& S, Item* items, int n) {
     S.D.off = S.fp32_desc_end;
     const StagedPairs f64 = stage_pairs(S.D, items, n, S.bufs, settle, RCG_PREC_FP64, S.d_s64);
     RCG_TRY(S.D.upload(ctx->stream));
-    RCG_TRY(launch_pairs(ctx, f64));
+    RCG_TRY(launch_pairs(ctx, f64, false));
     RCG_HIP(hipMemcpyAsync(S.h_scores, S.d_s64, settle.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     RCG_HIP(hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < settle.size(); i++) {

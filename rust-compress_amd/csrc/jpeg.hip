@@ -29,24 +29,101 @@ __device__ __forceinline__ int find_job_by_tile(const uint32_t* begins, int njob
     return lo;
 }
 
-// Byte-exact load of the dword at absolute address a (4-aligned), reading only bytes inside [lo, hi).
-__device__ __forceinline__ uint32_t load_dword_guarded(uintptr_t a, uintptr_t lo, uintptr_t hi) {
-    if (a >= lo && a + 4 <= hi) return *reinterpret_cast<const uint32_t*>(a);
-    uint32_t v = 0;
-    for (int k = 0; k < 4; k++) {
-        uintptr_t b = a + k;
-        if (b >= lo && b < hi) v |= uint32_t(*reinterpret_cast<const uint8_t*>(b)) << (8 * k);
+// Byte-exact load of the 16 bytes at absolute address a (16-aligned), reading only bytes inside [lo, hi); the
+// bytes outside read as zero.
+__device__ __forceinline__ uint4 load_chunk_guarded(uintptr_t a, uintptr_t lo, uintptr_t hi) {
+    uint32_t v[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 16; k++) {
+        const uintptr_t b = a + k;
+        if (b >= lo && b < hi) v[k >> 2] |= uint32_t(*reinterpret_cast<const uint8_t*>(b)) << (8 * (k & 3));
     }
-    return v;
+    return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
-constexpr int kRawDw = (kEncTileW * 3) / 4 + 2;  // dwords per staged source row (1536 bytes + alignment slack)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // a native 16-byte vector: kept in registers when
+                                                             // held in a constant-indexed array
+
+constexpr int kRawChunks = (kEncTileW * 3 + 15 + 15) / 16;  // 16-byte chunks per staged source row (97)
+constexpr int kRawRow = kRawChunks * 16;                     // LDS bytes per staged source row
+constexpr int kRawLoads = (16 * kRawChunks + kEncThreads - 1) / kEncThreads;  // chunk loads per thread, 16 rows
+// LDS of one workgroup, carved per subsampling mode: the Y rows (rows x 512), the two chroma planes (2 x 8 x 512/hmax),
+// then the staged source rows (rows x kRawRow). 4:2:0 is the largest: 16 Y rows, 256-wide chroma, 16 source rows.
+// Its last 16 bytes lie beyond every carve-up: the discard slot of the source staging.
+constexpr int kEncLds = 16 * kEncTileW + 2 * 8 * (kEncTileW / 2) + 16 * kRawRow + 16;
+static_assert(kEncLds - 16 >= 8 * kEncTileW + 2 * 8 * kEncTileW + 8 * kRawRow, "4:4:4 carve-up must fit");
+static_assert(kEncLds + 64 <= 40960, "four encode workgroups per CU");
+
+// The 4 bytes at LDS byte offset o (any alignment), from the two dwords that hold them.
+__device__ __forceinline__ uint32_t lds_bytes4(const uint8_t* base, uint32_t o) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(base + (o & ~3u));
+    return __builtin_amdgcn_alignbyte(d[1], d[0], o & 3u);
+}
+
+// Gray8: the staged rows are the Y rows; 4 samples per item.
+__device__ __forceinline__ void convert_gray(const uint8_t* s_raw, const uint32_t* s_off, uint8_t* s_y, int tid) {
+    for (int it = tid; it < 8 * (kEncTileW / 4); it += kEncThreads) {
+        const int py = it / (kEncTileW / 4), q = it % (kEncTileW / 4);
+        *reinterpret_cast<uint32_t*>(s_y + py * kEncTileW + 4 * q) =
+            lds_bytes4(s_raw, uint32_t(py) * kRawRow + s_off[py] + 4 * uint32_t(q));
+    }
+}
+
+// RGB8 -> YCbCr in 16-bit fixed point (encode.rs:466-477) and the HM x VM box mean of Cb and Cr (encode.rs:478-499).
+// One item is 4 chroma samples: 4*HM pixels of each of VM rows, read as dwords and realigned with v_alignbyte; its Y
+// bytes and chroma samples leave as dword stores.
+template <int HM, int VM>
+__device__ __forceinline__ void convert_rgb(const uint8_t* s_raw, const uint32_t* s_off, uint8_t* s_y, uint8_t* s_c,
+                                            int tid) {
+    constexpr int CW = kEncTileW / HM;  // chroma samples per row
+    constexpr int QPR = CW / 4;         // items per chroma row
+    constexpr int DW = 3 * HM;          // dwords of RGB per row of one item
+    constexpr int NSH = (HM - 1) + (VM - 1);
+    constexpr int HALF = (1 << NSH) >> 1;
+    for (int it = tid; it < 8 * QPR; it += kEncThreads) {
+        const int gy = it / QPR, q = it % QPR;
+        int sb[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int dy = 0; dy < VM; dy++) {
+            const int py = gy * VM + dy;
+            const uint32_t o = uint32_t(py) * kRawRow + s_off[py] + uint32_t(q) * (12 * HM);
+            const uint32_t* d = reinterpret_cast<const uint32_t*>(s_raw + (o & ~3u));
+            uint32_t wd[DW + 1], a[DW];
+#pragma unroll
+            for (int k = 0; k <= DW; k++) wd[k] = d[k];
+#pragma unroll
+            for (int k = 0; k < DW; k++) a[k] = __builtin_amdgcn_alignbyte(wd[k + 1], wd[k], o & 3u);
+            uint32_t ypk[HM];
+#pragma unroll
+            for (int p = 0; p < 4 * HM; p++) {
+                const int32_t r = (a[(3 * p) / 4] >> (8 * ((3 * p) % 4))) & 0xFF;
+                const int32_t g = (a[(3 * p + 1) / 4] >> (8 * ((3 * p + 1) % 4))) & 0xFF;
+                const int32_t b = (a[(3 * p + 2) / 4] >> (8 * ((3 * p + 2) % 4))) & 0xFF;
+                const uint32_t y = uint32_t((19595 * r + 38470 * g + 7471 * b + 32768) >> 16) & 0xFF;
+                const int32_t cb = (-11059 * r - 21709 * g + 32768 * b + (128 << 16) + 32767) >> 16;
+                const int32_t cr = (32768 * r - 27439 * g - 5329 * b + (128 << 16) + 32767) >> 16;
+                ypk[p / 4] = (p % 4 == 0) ? y : (ypk[p / 4] | (y << (8 * (p % 4))));
+                sb[p / HM] += cb & 0xFF;
+                sr[p / HM] += cr & 0xFF;
+            }
+#pragma unroll
+            for (int k = 0; k < HM; k++)
+                *reinterpret_cast<uint32_t*>(s_y + py * kEncTileW + q * (4 * HM) + 4 * k) = ypk[k];
+        }
+        uint32_t cbp = 0, crp = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            // The rounded mean of HM*VM samples (a power of two; the sums are non-negative).
+            cbp |= (uint32_t((sb[j] + HALF) >> NSH) & 0xFF) << (8 * j);
+            crp |= (uint32_t((sr[j] + HALF) >> NSH) & 0xFF) << (8 * j);
+        }
+        *reinterpret_cast<uint32_t*>(s_c + gy * CW + 4 * q) = cbp;
+        *reinterpret_cast<uint32_t*>(s_c + 8 * CW + gy * CW + 4 * q) = crp;
+    }
+}
 
 __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict__ jobs,
                                                          const uint32_t* __restrict__ tile_begins, int njobs) {
-    __shared__ __align__(16) uint32_t s_raw[16][kRawDw];
-    __shared__ __align__(16) uint8_t s_y[16][kEncTileW];
-    __shared__ __align__(16) uint8_t s_c[2][8][kEncTileW];
+    __shared__ __align__(16) uint8_t s_mem[kEncLds];
     __shared__ uint32_t s_off[16];
 
     const int tid = threadIdx.x;
@@ -55,59 +132,78 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
     const uint32_t tile = blockIdx.x - J.tile_begin;
     const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
     const int C = J.channels;
-    const int hmax = J.hmax, vmax = J.vmax;
+    const int hmax = J.hmax, vmax = J.vmax;  // each 1 or 2
     const int rows = 8 * vmax;
     const uint32_t x0 = tx * kEncTileW, y0 = ty * rows;
     const uint32_t w = J.w, h = J.h;
     const uint32_t n = min(uint32_t(kEncTileW), w - x0);  // real source pixels in this strip (x0 < w always)
+    const int cw = kEncTileW / hmax;                       // chroma samples per row of this strip
+    uint8_t* const s_y = s_mem;
+    uint8_t* const s_c = s_mem + rows * kEncTileW;
+    uint8_t* const s_raw = s_c + 2 * 8 * cw;
 
-    // Phase A: stage the strip's source rows (bottom rows replicated) in LDS with aligned dword loads.
+    // Phase A: stage the strip's source rows (bottom rows replicated) in LDS. Row r starts at byte s_off[r] of its
+    // LDS row and is read as the aligned 16-byte chunks that cover it.
     const uintptr_t base = reinterpret_cast<uintptr_t>(J.src), end = base + J.src_bytes;
-    const uint32_t ndw = (n * C + 3 + 3) / 4;
-    if (tid < rows) {
-        const uint32_t sy = min(y0 + tid, h - 1);
-        const uintptr_t g0 = base + (uint64_t(sy) * w + x0) * C;
-        s_off[tid] = uint32_t(g0 & 3);
+    const uint32_t nbytes = n * C;
+    const uint32_t nchunks = uint32_t(rows) * kRawChunks;
+    auto row_start = [&](uint32_t r) {
+        return base + (uint64_t(min(y0 + r, h - 1)) * w + x0) * C;
+    };
+    const uintptr_t first = row_start(0) & ~uintptr_t(15);
+    const uintptr_t last = (row_start(rows - 1) + nbytes + 15) & ~uintptr_t(15);
+    if (first >= base && last <= end) {
+        // Every chunk of the strip lies inside the source buffer: unconditional 16-byte loads, all in flight before
+        // the first is used (a lane without a chunk of its own reloads the strip's first and discards it).
+        const uint32_t raw_off = uint32_t(s_raw - s_mem);
+        u32x4 v[kRawLoads];
+        uint32_t lds[kRawLoads];
+#pragma unroll
+        for (int k = 0; k < kRawLoads; k++) {
+            const uint32_t c = tid + k * kEncThreads;
+            const uint32_t r = c / kRawChunks, i = c - r * kRawChunks;
+            const uintptr_t g0 = row_start(r);
+            const uintptr_t a = (g0 & ~uintptr_t(15)) + 16 * uintptr_t(i);
+            const bool ok = c < nchunks && a < g0 + nbytes;
+            lds[k] = ok ? raw_off + r * kRawRow + 16 * i : kEncLds - 16;  // else the discard slot
+            v[k] = *reinterpret_cast<const u32x4*>(ok ? a : first);
+        }
+#pragma unroll
+        for (int k = 0; k < kRawLoads; k++) *reinterpret_cast<u32x4*>(s_mem + lds[k]) = v[k];
+    } else {
+        // A strip at either end of the source buffer: a chunk that reaches outside it goes byte by byte.
+        for (uint32_t c = tid; c < nchunks; c += kEncThreads) {
+            const uint32_t r = c / kRawChunks, i = c - r * kRawChunks;
+            const uintptr_t g0 = row_start(r);
+            const uintptr_t a = (g0 & ~uintptr_t(15)) + 16 * uintptr_t(i);
+            if (a < g0 + nbytes)
+                *reinterpret_cast<uint4*>(s_raw + r * kRawRow + 16 * i) =
+                    (a >= base && a + 16 <= end) ? *reinterpret_cast<const uint4*>(a) : load_chunk_guarded(a, base, end);
+        }
     }
-    for (uint32_t idx = tid; idx < uint32_t(rows) * ndw; idx += kEncThreads) {
-        const uint32_t r = idx / ndw, i = idx - r * ndw;
-        const uint32_t sy = min(y0 + r, h - 1);
-        const uintptr_t g0 = base + (uint64_t(sy) * w + x0) * C;
-        s_raw[r][i] = load_dword_guarded((g0 & ~uintptr_t(3)) + 4 * uintptr_t(i), base, end);
-    }
+    if (tid < rows) s_off[tid] = uint32_t(row_start(tid) & 15);
     __syncthreads();
-
-    // Phase B: colour conversion (16-bit fixed point, encode.rs:466-477) and box downsampling (encode.rs:478-499)
-    // of one hmax x vmax pixel group per iteration; right-edge replication by clamping the column.
-    const uint8_t* raw = reinterpret_cast<const uint8_t*>(&s_raw[0][0]);
-    const int gw = kEncTileW / hmax;
-    for (int idx = tid; idx < gw * 8; idx += kEncThreads) {
-        const int gx = idx % gw, gy = idx / gw;
-        int sb = 0, sr = 0;
-        for (int dy = 0; dy < vmax; dy++) {
-            const int py = gy * vmax + dy;
-            const uint8_t* row = raw + py * (kRawDw * 4) + s_off[py];
-            for (int dx = 0; dx < hmax; dx++) {
-                const int px = gx * hmax + dx;
-                const int lx = min(uint32_t(px), n - 1);
-                if (C == 1) {
-                    s_y[py][px] = row[lx];
-                } else {
-                    const int32_t r = row[lx * 3], g = row[lx * 3 + 1], b = row[lx * 3 + 2];
-                    s_y[py][px] = uint8_t((19595 * r + 38470 * g + 7471 * b + 32768) >> 16);
-                    const int32_t cb = (-11059 * r - 21709 * g + 32768 * b + (128 << 16) + 32767) >> 16;
-                    const int32_t cr = (32768 * r - 27439 * g - 5329 * b + (128 << 16) + 32767) >> 16;
-                    sb += cb & 0xFF;
-                    sr += cr & 0xFF;
-                }
-            }
+    if (n < uint32_t(kEncTileW)) {
+        // The strip ends inside the tile: replicate each row's last pixel to the tile's width (the right-edge padding,
+        // x.min(w - 1), of encode.rs:453-458 and 466-470), so that the conversion reads whole groups of pixels.
+        const uint32_t pad = (kEncTileW - n) * C;
+        for (uint32_t idx = tid; idx < uint32_t(rows) * pad; idx += kEncThreads) {
+            const uint32_t r = idx / pad, k = idx - r * pad;
+            uint8_t* row = s_raw + r * kRawRow + s_off[r];
+            row[nbytes + k] = row[nbytes - C + k % C];
         }
-        if (C == 3) {
-            const int nn = hmax * vmax;
-            s_c[0][gy][gx] = uint8_t((sb + nn / 2) / nn);
-            s_c[1][gy][gx] = uint8_t((sr + nn / 2) / nn);
-        }
+        __syncthreads();
     }
+
+    // Phase B: colour conversion and chroma box downsampling into the Y rows and chroma planes.
+    if (C == 1)
+        convert_gray(s_raw, s_off, s_y, tid);
+    else if (hmax == 1)
+        convert_rgb<1, 1>(s_raw, s_off, s_y, s_c, tid);
+    else if (vmax == 1)
+        convert_rgb<2, 1>(s_raw, s_off, s_y, s_c, tid);
+    else
+        convert_rgb<2, 2>(s_raw, s_off, s_y, s_c, tid);
     __syncthreads();
 
     // Phase C: one 8x8 block per thread.
@@ -117,7 +213,7 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
     if (tid >= nblocks) return;
     const int cls = __builtin_amdgcn_readfirstlane(tid >= nY ? 1 : 0);  // waves never mix luma and chroma
     const uint8_t* srcrow;
-    int comp;
+    int comp, lds_stride;
     uint32_t bx, by, bw_comp;
     if (tid < nY) {
         const int br = tid / 64, bc = tid % 64;
@@ -126,7 +222,8 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
         by = y0 / 8 + br;
         bw_comp = J.pw / 8;
         if (bx >= bw_comp) return;
-        srcrow = &s_y[br * 8][bc * 8];
+        srcrow = s_y + br * 8 * kEncTileW + bc * 8;
+        lds_stride = kEncTileW;
     } else {
         const int u = tid - nY;
         comp = 1 + u / cpr;
@@ -135,16 +232,17 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
         by = ty;
         bw_comp = J.pw / hmax / 8;
         if (bx >= bw_comp) return;
-        srcrow = &s_c[comp - 1][0][bc * 8];
+        srcrow = s_c + (comp - 1) * 8 * cw + bc * 8;
+        lds_stride = cw;
     }
 
     float blk[64];
 #pragma unroll
     for (int y = 0; y < 8; y++) {
-        const uint2 v = *reinterpret_cast<const uint2*>(srcrow + y * kEncTileW);
+        const uint2 px8 = *reinterpret_cast<const uint2*>(srcrow + y * lds_stride);
 #pragma unroll
         for (int x = 0; x < 8; x++) {
-            const uint32_t word = x < 4 ? v.x : v.y;
+            const uint32_t word = x < 4 ? px8.x : px8.y;
             blk[y * 8 + x] = __fsub_rn(float((word >> (8 * (x & 3))) & 0xFF), 128.0f);
         }
     }
@@ -199,12 +297,13 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
             float b2[64];
 #pragma unroll
             for (int nidx = 0; nidx < 64; nidx++) {
+                // float(clamp(round_f32(x), -2047, 2047)) computed in f32: (x + 1.5*2^23) - 1.5*2^23 is x rounded half
+                // to even, exactly, for |x| < 2^22 (the quantised DCT values here stay below 2^15).
                 const int f = (nidx % 8) * 8 + nidx / 8;
-                int32_t r = round_f32(__fmul_rn(blk[f], div[f]));
-                r = r < -2047 ? -2047 : (r > 2047 ? 2047 : r);
-                b2[nidx] = __fmul_rn(float(r), mult[nidx]);
+                const float r = __fsub_rn(__fadd_rn(__fmul_rn(blk[f], div[f]), 12582912.0f), 12582912.0f);
+                b2[nidx] = __fmul_rn(__builtin_amdgcn_fmed3f(r, -2047.0f, 2047.0f), mult[nidx]);
             }
-            idct_block(b2);
+            idct_block<false>(b2);  // 8x the samples: the 1/8 is folded into the rounding below
             const uint32_t stride = bw_comp * 8;
             uint8_t* out = rp + uint64_t(by) * 8 * stride + bx * 8;
 #pragma unroll
@@ -212,7 +311,10 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
                 uint32_t lo = 0, hi = 0;
 #pragma unroll
                 for (int x = 0; x < 8; x++) {
-                    int32_t v = round_f32(b2[y * 8 + x]) + 128;
+                    // round_f32(x * 0.125) + 128: x + 1.5*2^26 rounds x to a multiple of 8, ties to the even multiple
+                    // of 16, which is round-half-even of x/8; its bits less those of 1.5*2^26 count eighths. Exact for
+                    // |x| < 2^25; here |x| <= 57172 (any block of 8-bit samples, any table of 8-bit steps).
+                    int32_t v = __float_as_int(__fadd_rn(b2[y * 8 + x], 100663296.0f)) - (0x4CC00000 - 128);
                     v = v < 0 ? 0 : (v > 255 ? 255 : v);
                     if (x < 4) lo |= uint32_t(v) << (8 * x);
                     else hi |= uint32_t(v) << (8 * (x - 4));

@@ -115,7 +115,9 @@ __device__ __forceinline__ void fdct_block(float* b) {
 }
 
 // Inverse DCT (jpeg/src/dct.rs:113-123): pass, transpose, pass, transpose, times 0.125. Input b[v*8+u]
-// dequantised and AAN-scaled; output b[y*8+x] level-shifted samples, unrounded.
+// dequantised and AAN-scaled; output b[y*8+x] level-shifted samples, unrounded. kScale = false leaves out the final
+// exact multiplication by 0.125 (the caller folds it into its rounding).
+template <bool kScale = true>
 __device__ __forceinline__ void idct_block(float* b) {
 #pragma unroll
     for (int j = 0; j < 8; j++) {
@@ -134,7 +136,7 @@ __device__ __forceinline__ void idct_block(float* b) {
         for (int i = 0; i < 8; i++) row[i] = b[j * 8 + i];
         idct8(row);
 #pragma unroll
-        for (int i = 0; i < 8; i++) b[j * 8 + i] = __fmul_rn(row[i], 0.125f);
+        for (int i = 0; i < 8; i++) b[j * 8 + i] = kScale ? __fmul_rn(row[i], 0.125f) : row[i];
     }
 }
 

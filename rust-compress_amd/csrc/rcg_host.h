@@ -91,9 +91,10 @@ struct rcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     // Second stream (default priority) for the reference-cache launches, which run beside the encode launch: forked
-    // from and joined back into `stream` with the two events.
+    // from `stream` (ev_fork), marked after the scale-0 reference launches (ev_mid: the candidates' scale-0 launch
+    // waits only for those) and joined back into `stream` (ev_join).
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr;
     std::mutex mu;
     rcg::Arena arena;
     rcg::Pinned pinned;
@@ -156,6 +157,8 @@ int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uin
 // back (the context's stream waits for everything enqueued on the side stream so far).
 int fork_side(rcg_ctx* ctx);
 int join_side(rcg_ctx* ctx);
+int mark_side(rcg_ctx* ctx);       // record ev_mid on the side stream
+int wait_side_mark(rcg_ctx* ctx);  // the context's stream waits for ev_mid
 
 // Jobs job0 .. job0+njobs-1 of a launch that share one tiling of `tiles` tiles (the candidates of one image at one
 // scale, or a single job).

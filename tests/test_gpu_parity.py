@@ -307,6 +307,33 @@ def test_device_entry_points_match_host_ones(ctx, rc):
     assert (f1.quality, f1.score, f1.steps) == (f2.quality, f2.score, f2.steps)
 
 
+@pytest.mark.parametrize("w,h,ch", [(333, 250, 3), (129, 37, 1), (1030, 17, 3)])
+def test_unaligned_device_sources(ctx, rc, w, h, ch):
+    # Source pixels at every byte offset of a 16-byte chunk: the encode stage stages rows as aligned 16-byte chunks,
+    # and a chunk that reaches outside the source buffer is read byte by byte.
+    import ctypes
+    img = rc.synth_rgb8(5, w, h)
+    if ch == 1:
+        img = np.ascontiguousarray(img[:, :, 1])
+    host = ctx.build_coefs(img, 80, rc.S420)
+    want = ctx.find_quality(img, 70.0)
+    buf = rc.DeviceBuffer(ctx, img.nbytes + 64)
+    junk = np.full(img.nbytes + 64, 0xA5, np.uint8)
+    L = rc.coef_layout(w, h, ch, 80, rc.S420)
+    outs = [rc.DeviceBuffer(ctx, L.blocks_w[c] * L.blocks_h[c] * 64 * 2) for c in range(len(host.comps))]
+    for off in (1, 3, 7, 8, 13, 15, 16):
+        for start, a in ((0, junk), (off, img)):
+            a = np.ascontiguousarray(a)
+            rc._check(rc.lib().rcg_memcpy_h2d(ctx.handle, buf.ptr + start, a.ctypes.data_as(ctypes.c_void_p),
+                                              a.nbytes))
+        ctx.build_coefs_dev(buf.ptr + off, w, h, ch, 80, rc.S420, [o.ptr for o in outs])
+        for o, comp in zip(outs, host.comps):
+            got = o.download(np.empty(len(comp.coefs), np.int16))
+            np.testing.assert_array_equal(got, comp.coefs, err_msg=f"offset {off}")
+        f = ctx.find_quality_dev(buf.ptr + off, w, h, ch, 70.0)
+        assert (f.quality, f.score, f.steps) == (want.quality, want.score, want.steps), off
+
+
 def test_written_files_from_device_coefficients_are_byte_identical(ctx, rc):
     # The stage after the search: the device's coefficients, entropy-coded on the host, give the same file as the
     # oracle's coefficients (the coefficients are bit-exact), for baseline and progressive output; and the file
