@@ -64,15 +64,27 @@ def _pmc_run(counter, bench_args, outdir, name):
     return [(k.split("(")[0], i, v) for i, k, v in rows]
 
 
-def measure_traffic(size):
+def tuned_policy(ncand):
+    """The headline's tile policy, autotuned in a child process (so that this process has not touched the GPU
+    when the counter passes run); None if the child failed."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--autotune-only", str(ncand)],
+                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, cwd=ROOT, text=True)
+    lines = r.stdout.strip().splitlines()
+    if r.returncode != 0 or not lines:
+        return None
+    return json.loads(lines[-1])
+
+
+def measure_traffic(size, policy):
     """HBM bytes per launch of the dominant kernel (k_ssim_scale) in one headline step, from FETCH_SIZE and
-    WRITE_SIZE (one rocprofv3 pass each over a one-step child run), each scaled by what the counter reads for a
-    512 MiB device-to-device copy (the same counter's calibration factor). Runs before this process touches the GPU:
-    the passes are child processes."""
+    WRITE_SIZE (one rocprofv3 pass each over a one-step child run under the same tile policy as the timed steps),
+    each scaled by what the counter reads for a 512 MiB device-to-device copy (the same counter's calibration
+    factor). Runs before this process touches the GPU: the passes are child processes."""
     import tempfile
     out = tempfile.mkdtemp(prefix="rcg_traffic_")
     step = ["--steps", "1", "--warmup", "0", "--size", str(size), "--no-cpu", "--no-plain", "--no-batch",
-            "--no-serve", "--no-cli", "--no-traffic"]
+            "--no-serve", "--no-cli", "--no-traffic", "--tile-policy", json.dumps(policy)]
     copy = ["--copy-calibration", "512"]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -98,7 +110,7 @@ def measure_traffic(size):
     write = res["WRITE_SIZE"]["kib_per_launch"] * res["WRITE_SIZE"]["factor"] * 1024
     return {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
             "fetch_factor": round(res["FETCH_SIZE"]["factor"], 3), "write_factor": round(res["WRITE_SIZE"]["factor"], 3),
-            "launches": res["FETCH_SIZE"]["launches"]}
+            "launches": res["FETCH_SIZE"]["launches"], "tile_policy": policy}
 
 
 def main():
@@ -119,7 +131,15 @@ def main():
                     help="only run a device-to-device copy of MIB mebibytes (HBM traffic counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that give roofline.traffic")
+    ap.add_argument("--tile-policy", default=None, metavar="JSON",
+                    help="install this headline tile policy (as the bench line prints it) instead of autotuning")
+    ap.add_argument("--autotune-only", type=int, default=0, metavar="NCAND",
+                    help="only autotune the tile policy for NCAND candidates and print it as JSON")
     args = ap.parse_args()
+    if args.autotune_only:
+        rc = importlib.import_module("rust-compress_amd")
+        print(json.dumps(rc.Context(0).autotune(args.autotune_only).as_dict()), flush=True)
+        return
     if args.copy_calibration:
         # One device-to-device copy of a known size (the traffic counters of a plain copy kernel).
         rc = importlib.import_module("rust-compress_amd")
@@ -136,7 +156,13 @@ def main():
               "(images would shard across GPUs with no collective; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
-    traffic = None if args.no_traffic else measure_traffic(args.size)  # child processes, before any GPU use here
+    cands_n = 6  # q{50,75,90} x {4:2:0, 4:4:4}
+    policy = json.loads(args.tile_policy) if args.tile_policy else None
+    traffic = None
+    if not args.no_traffic:
+        # Child processes, before any GPU use here; the timed steps then run under the policy the passes measured.
+        policy = policy or tuned_policy(cands_n)
+        traffic = measure_traffic(args.size, policy) if policy else None
     rc = importlib.import_module("rust-compress_amd")
     ctx = rc.Context(0)
     W = H = args.size
@@ -144,9 +170,19 @@ def main():
     d_img = ctx.upload(img)  # resident in HBM before the timed region
     ptr = d_img.ptr
     cands = [(q, s) for s in (rc.S420, rc.S444) for q in (50, 75, 90)]
+    assert len(cands) == cands_n
     # Tile heights per scale-height class, measured on this GPU for this workload's candidate count.
-    tune = ctx.autotune(len(cands))
-    progress(f"autotuned tile policy: {tune.as_dict()}")
+    if policy:
+        tune = rc.Tune()
+        tune.wide_nt, tune.narrow_w, tune.nclass = policy["wide_nt"], policy["narrow_w"], len(policy["classes"])
+        for i, (min_h, th) in enumerate(policy["classes"]):
+            tune.min_h[i], tune.th[i] = min_h, th
+        ctx.set_tune(tune)
+        tune = ctx.get_tune()
+        progress(f"installed tile policy: {tune.as_dict()}")
+    else:
+        tune = ctx.autotune(len(cands))
+        progress(f"autotuned tile policy: {tune.as_dict()}")
     target = 70.0
 
     def step():

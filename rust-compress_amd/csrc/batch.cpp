@@ -412,6 +412,40 @@ struct ScoreState {
     size_t fp32_desc_end = 0;
 };
 
+// The reference-cache kernel's tiling, per launch. Its output does not depend on the tiling, and each image's own plan
+// fills one round of the GPU's workgroup slots with that image's tiles, so a launch over many images would walk tiles
+// of a few rows, mostly halo (10 rows per tile). Over the images a launch builds together, tiles are as tall as one
+// round of slots for all their tile columns allows, at most kRefMaxRows, and never shorter than the image's own plan
+// (a single image keeps it). Only lowers tile counts, so the descriptor sizing from the per-image plans still holds.
+constexpr uint32_t kRefMaxRows = 256;
+
+void retile_refs(const SsimTune& T, const Item* items, int n, std::vector<ItemBufs>& bufs) {
+    for (uint32_t s = 0; s < 6; s++) {
+        for (uint32_t nt : {256u, 128u}) {
+            auto builds = [&](int i) {
+                const PairPlan& P = bufs[i].P32;
+                return !items[i].ref_ready && items[i].ncand > 0 && s < P.nscales && P.nt[s] == nt;
+            };
+            uint32_t cols = 0;
+            for (int i = 0; i < n; i++)
+                if (builds(i)) cols += bufs[i].P32.tiles_x[s];
+            if (cols == 0) continue;
+            const uint32_t slots = nt == 128 ? T.ref_slots_narrow : T.ref_slots_wide;
+            const uint32_t rows_of_tiles = std::max(1u, slots / cols);
+            for (int i = 0; i < n; i++) {
+                if (!builds(i)) continue;
+                PairPlan& P = bufs[i].P32;
+                uint32_t tr = (P.sh[s] + rows_of_tiles - 1) / rows_of_tiles;
+                tr = std::min(kRefMaxRows, std::max(4u, (tr + 1) & ~1u));
+                if (tr > P.th_ref[s]) {
+                    P.th_ref[s] = tr;
+                    P.tiles_ref[s] = P.tiles_x[s] * ssim_tiles_y(P.sh[s], int(tr));
+                }
+            }
+        }
+    }
+}
+
 int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, ScoreState& S) {
     size_t npairs = 0;
     for (int i = 0; i < n; i++) npairs += items[i].ncand;
@@ -493,6 +527,7 @@ int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, Score
         S.d_ej = S.D.put(ej.data(), ej.size());
         S.d_eb = S.D.put(ebegins.data(), ebegins.size());
     }
+    retile_refs(ctx->tune32, items, n, S.bufs);
     S.fp32 = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32);
     S.fp32_desc_end = S.D.off;
     return S.D.upload(ctx->stream);
