@@ -344,8 +344,9 @@ def bench_batch(rc, ctx, args):
     gen_s = time.perf_counter() - t0
     ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
     cands = [(q, rc.S420) for q in CANDS_B]
-    # A batch launch holds many pairs of every size: tune for many pairs per launch, not for one image's candidates.
-    tune_b = ctx.autotune(64)
+    # A batch launch holds many pairs of every size and runs many rounds of workgroups: tune for throughput (time per
+    # pair over launches of many rounds), not for one image's candidates.
+    tune_b = ctx.autotune(0)
     total_mp = sum(w * h for w, h in sizes) / 1e6
     ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)  # warm-up (sizes the workspace)
     ctx.synchronize()

@@ -452,7 +452,8 @@ class Context:
         _check(lib().rcg_set_tune(self._h, ctypes.byref(tune)))
 
     def autotune(self, jobs=8):
-        """Measure and install the tile policy for launches of `jobs` same-size pairs (rcg_autotune)."""
+        """Measure and install the tile policy for launches of `jobs` same-size pairs (rcg_autotune); jobs=0 tunes
+        for throughput, for the launches of a large batch."""
         t = Tune()
         _check(lib().rcg_autotune(self._h, jobs, ctypes.byref(t)))
         return t

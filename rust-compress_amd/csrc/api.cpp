@@ -304,6 +304,7 @@ int launch_staged_refs(rcg_ctx* ctx, const StagedSsim& st) {
                                 s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes), ctx->side));
         if (s == 0) RCG_TRY(mark_side(ctx));
     }
+    if (st.P.nscales == 0) RCG_TRY(mark_side(ctx));  // launch_staged_ssim() waits for the mark in any case
     return RCG_OK;
 }
 
@@ -1461,15 +1462,21 @@ extern "C" int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune) {
 // planes in one launch for every tile height, for S = 4096 ... 64, and installs the fastest tile height of each size
 // class.
 extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
+    // jobs = 0: throughput mode, for launches of very many pairs (a batch of many images). Each timing launch then
+    // holds groups of 8 pairs, as many as make about 8 rounds of the GPU's workgroup slots, and the time per pair is
+    // compared: a partly idle last round, which a launch of a few pairs pays and a large batch does not, no longer
+    // favours short tiles.
+    const bool throughput = jobs == 0;
     const int candidates = jobs;
-    if (!ctx || candidates < 1 || candidates > 64) return RCG_INVALID;
+    if (!ctx || candidates < 0 || candidates > 64) return RCG_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     RCG_HIP(hipSetDevice(ctx->device));
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
     // Tile heights: powers of two and the steps between them (a launch's last round of workgroups is the less idle
     // the closer its tile count comes to a multiple of the GPU's workgroup slots). Small scales are latency-bound
-    // (one workgroup walks TH + 10 rows one after another), so very short tiles are tried too.
-    const int ths[] = {2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256};
+    // (one workgroup walks TH + 10 rows one after another), so very short tiles are tried too; a launch of many
+    // rounds (throughput) is best served by tall ones, whose 10-row halo costs least.
+    const int ths[] = {2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256, 320, 384, 512};
     const int nsz = 7;
     const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
     ctx->arena.reset();
@@ -1507,7 +1514,14 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
             if (uint32_t(th) > S) continue;
             const int nt = S <= T.narrow_w ? 128 : T.wide_nt;
             const uint32_t tx = ssim_tiles_x(S, nt), ty = ssim_tiles_y(S, th);
-            if (size_t(tx) * ty * 18 * 8 * candidates > (size_t(32) << 20)) continue;
+            const uint32_t tiles = tx * ty;
+            const int group = throughput ? 8 : candidates;
+            const int pairs = throughput ? int(std::min<uint32_t>(4096u, ((8192u + tiles - 1) / tiles + 7) / 8 * 8))
+                                         : candidates;
+            if (size_t(tiles) * 18 * 8 * pairs > (size_t(32) << 20)) continue;
+            if ((sizeof(SsimJob) + 64) * size_t(pairs) + sizeof(Unit) * (size_t(tiles) * pairs + 64 * kXcds) + 4096 >
+                (size_t(4) << 20))
+                continue;
             // The reference cache of the S x S plane `a` (the same for every tile height; rebuilt per size).
             RefJob rj;
             std::memset(&rj, 0, sizeof(rj));
@@ -1518,8 +1532,8 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
             rj.tiles_y = ty;
             rj.a = planes_src(a);
             rj.cache = cache;
-            std::vector<SsimJob> jobs(candidates);
-            for (int c = 0; c < candidates; c++) {
+            std::vector<SsimJob> jobs(pairs);
+            for (int c = 0; c < pairs; c++) {
                 SsimJob& j = jobs[c];
                 std::memset(&j, 0, sizeof(j));
                 j.w = S;
@@ -1532,7 +1546,9 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
                 j.partials = partials + size_t(c) * tx * ty * 18;
             }
             // The candidates of a tile side by side on one XCD, as the scoring passes launch them.
-            const std::vector<Unit> units = xcd_units({UnitGroup{0, uint32_t(candidates), tx * ty}});
+            std::vector<UnitGroup> groups;
+            for (int g = 0; g < pairs; g += group) groups.push_back(UnitGroup{uint32_t(g), uint32_t(group), tiles});
+            const std::vector<Unit> units = xcd_units(groups);
             Desc D;
             D.host = ctx->pinned.base;
             D.dev = ddesc;
@@ -1544,7 +1560,7 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
             RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes, ctx->stream));
             const uint32_t nu = uint32_t(units.size());
             RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
-            const size_t work = size_t(S) * S * size_t(candidates);
+            const size_t work = size_t(S) * S * size_t(pairs);
             const int reps = work >= (size_t(64) << 20) ? 2 : (work >= (size_t(16) << 20) ? 3 : 10);
             RCG_HIP(hipEventRecord(e0, ctx->stream));
             for (int r = 0; r < reps; r++)
@@ -1553,8 +1569,9 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
             RCG_HIP(hipEventSynchronize(e1));
             float ms = 0;
             RCG_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms / reps < best) {
-                best = ms / reps;
+            const double per = double(ms) / reps / pairs;  // per pair; the pair count is fixed unless throughput
+            if (per < best) {
+                best = per;
                 best_th[si] = th;
             }
         }

@@ -842,20 +842,37 @@ constexpr int kFinThreads = 384;
 
 __global__ __launch_bounds__(kFinThreads) void k_ssim_finalize(const FinalDesc* __restrict__ fj) {
     const FinalDesc& F = fj[blockIdx.x];
+    __shared__ double s_part[6][18][65];  // per scale, statistic and lane (odd stride: no bank conflicts)
     __shared__ double s_sum[6][18];
     __shared__ double s_stat[6][18];
     const int tid = threadIdx.x, lane = tid & 63, sc = tid >> 6;
     const uint32_t ns = F.nscales;
-    // Lane l of wave s sums tiles l, l+64, ... of scale s (each tile's 18 sums are one contiguous 144-byte record);
-    // a fixed butterfly combines the lanes.
+    // Lane l of wave s sums tiles l, l+64, ... of scale s in that order (each tile's 18 sums are one contiguous
+    // 144-byte record; four tiles are loaded ahead of their additions). The lanes' sums are then added in lane order.
     if (uint32_t(sc) < ns) {
         const double* p = F.partials[sc];
         const uint32_t nt = F.ntiles[sc];
         double acc[18];
 #pragma unroll
         for (int i = 0; i < 18; i++) acc[i] = 0.0;
-#pragma unroll 2
-        for (uint32_t t = lane; t < nt; t += 64) {
+        uint32_t t = lane;
+        for (; t + 3 * 64 < nt; t += 4 * 64) {
+            double2 v[4][9];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const double2* rec = reinterpret_cast<const double2*>(p + uint64_t(t + u * 64) * 18);
+#pragma unroll
+                for (int i = 0; i < 9; i++) v[u][i] = rec[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int i = 0; i < 9; i++) {
+                    acc[2 * i] += v[u][i].x;
+                    acc[2 * i + 1] += v[u][i].y;
+                }
+        }
+        for (; t < nt; t += 64) {
             const double2* rec = reinterpret_cast<const double2*>(p + uint64_t(t) * 18);
 #pragma unroll
             for (int i = 0; i < 9; i++) {
@@ -865,12 +882,15 @@ __global__ __launch_bounds__(kFinThreads) void k_ssim_finalize(const FinalDesc* 
             }
         }
 #pragma unroll
-        for (int i = 0; i < 18; i++) {
-            double v = acc[i];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) s_sum[sc][i] = v;
-        }
+        for (int i = 0; i < 18; i++) s_part[sc][i][lane] = acc[i];
+    }
+    __syncthreads();
+    if (tid < int(ns) * 18) {
+        const int s2 = tid / 18, i = tid % 18;
+        const double* row = s_part[s2][i];
+        double v = 0.0;
+        for (int l = 0; l < 64; l++) v += row[l];
+        s_sum[s2][i] = v;
     }
     __syncthreads();
     // One thread per (scale, statistic): mean for even entries, 4-norm for odd ones.

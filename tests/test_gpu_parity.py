@@ -239,6 +239,20 @@ def test_score_batch_mixed_sizes_matches_oracle(ctx, rc):
         assert r.chosen_quality == qs[order[best]]
 
 
+def test_batch_scores_do_not_depend_on_the_rest_of_the_batch(ctx, rc):
+    # A launch over many images tiles the reference caches taller than one image alone does; the cache, and so every
+    # score, must not depend on that tiling: each image's scores in a 24-image batch equal its scores alone, bit for bit.
+    rng = np.random.default_rng(21)
+    sizes = [(int(rng.integers(200, 1300)), int(rng.integers(64, 1100))) for _ in range(22)] + [(3000, 40), (9, 700)]
+    imgs = [rc.synth_rgb8(300 + i, w, h) for i, (w, h) in enumerate(sizes)]
+    together = ctx.score_batch(imgs, CANDS, 70.0)
+    for img, r in zip(imgs, together):
+        alone = ctx.score_batch([img], CANDS, 70.0)[0]
+        assert r.status == 0 and alone.status == 0
+        assert list(r.scores) == list(alone.scores), img.shape
+        assert r.chosen_quality == alone.chosen_quality
+
+
 def test_score_batch_reports_per_job_errors(ctx, rc):
     good = rc.synth_rgb8(1, 64, 64)
     res = ctx.score_batch([good, np.zeros((7, 64, 3), np.uint8)], CANDS, 70.0)
@@ -256,14 +270,21 @@ def test_tile_policy_roundtrip_and_validation(ctx, rc):
     ctx.set_tune(t)
 
 
-def test_autotuned_policy_keeps_parity(rc):
+@pytest.mark.parametrize("jobs", [4, 0])
+def test_autotuned_policy_keeps_parity(rc, jobs):
+    # jobs = 0 is the throughput mode (launches of many rounds, as a large batch runs).
     c = rc.Context(0)
-    t = c.autotune(4)
+    t = c.autotune(jobs)
     assert 1 <= t.nclass <= 7
-    assert all(t.th[i] in (2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256) for i in range(t.nclass))
+    assert all(t.th[i] in (2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256, 320, 384, 512) for i in range(t.nclass))
     img = rc.synth_rgb8(12, 700, 530)
     dist = O.reconstruct(O.build_coefs(img, 60, O.S420))
     assert abs(c.ssimulacra2(img, dist, rc.FP32) - O.ssimulacra2(img, dist)) <= TOL32
+    imgs = [rc.synth_rgb8(40 + i, w, h) for i, (w, h) in enumerate([(1500, 1100), (300, 900), (64, 80)])]
+    for img, r in zip(imgs, c.score_batch(imgs, CANDS, 70.0)):
+        for k, (q, sub) in enumerate(CANDS[:3]):
+            o = O.ssimulacra2(img, O.reconstruct(O.build_coefs(img, q, sub)))
+            assert abs(r.scores[k] - o) <= (TOL64 if r.settled[k] else TOL32), (img.shape, q)
     c.close()
 
 
