@@ -358,20 +358,40 @@ int launch_refs_side(rcg_ctx* ctx, const StagedPairs& sp) {
 // The candidate launches, scale by scale, then the pooling launch. With `refs` (launch_refs_side() enqueued this
 // pass's reference state), the scale-0 launches wait for the scale-0 reference caches only and the later ones for
 // the whole side stream, so the later reference scales are built while scale 0 is scored.
+//
+// A scale with both a wide (256-thread) and a narrow (128-thread) launch runs the narrow one on the side stream,
+// beside the wide one: the narrow launch holds the few images at most 160 pixels wide at that scale, whose tiles walk
+// many rows one after another, so alone it would leave the GPU mostly idle for its duration. The two join before
+// the next scale (an image can move from the wide launch of one scale to the narrow launch of the next).
 int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp, bool refs) {
+    auto launch = [&](const StagedLaunch& g, hipStream_t st) {
+        return launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
+                                 g.ka, g.kb, st);
+    };
+    bool side_open = refs;  // the side stream holds work the context's stream has not joined yet
     size_t k = 0;
-    if (refs) RCG_TRY(wait_side_mark(ctx));
-    for (; k < sp.scales.size() && sp.scales[k].scale == 0; k++) {
-        const StagedLaunch& g = sp.scales[k];
-        RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
-                                  g.ka, g.kb));
+    while (k < sp.scales.size()) {
+        const int s = sp.scales[k].scale;
+        size_t e = k;
+        while (e < sp.scales.size() && sp.scales[e].scale == s) e++;
+        // [k, e): this scale's launches, the wide one first when there are two.
+        const bool split = e - k == 2 && sp.scales[k].nt == 256 && sp.scales[k + 1].nt == 128;
+        if (s > 0 && side_open) {
+            RCG_TRY(join_side(ctx));  // the previous scale's narrow launch (at scale 1 also the reference chain)
+            side_open = false;
+        }
+        if (split) {
+            // The side stream runs the narrow launch after everything enqueued here so far (the encode, earlier
+            // scales) and, at scale 0, after the reference chain it already holds.
+            RCG_TRY(fork_side(ctx));
+            RCG_TRY(launch(sp.scales[k + 1], ctx->side));
+            side_open = true;
+        }
+        if (s == 0 && refs) RCG_TRY(wait_side_mark(ctx));
+        for (size_t i = k; i < (split ? k + 1 : e); i++) RCG_TRY(launch(sp.scales[i], nullptr));
+        k = e;
     }
-    if (refs) RCG_TRY(join_side(ctx));
-    for (; k < sp.scales.size(); k++) {
-        const StagedLaunch& g = sp.scales[k];
-        RCG_TRY(launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
-                                  g.ka, g.kb));
-    }
+    if (side_open) RCG_TRY(join_side(ctx));
     return launch_finalize(ctx, sp.fin, sp.npairs);
 }
 

@@ -150,7 +150,7 @@ int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
                           uint8_t* out);
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
-                      uint32_t kind_a, uint32_t kind_b);
+                      uint32_t kind_a, uint32_t kind_b, hipStream_t stream = nullptr);  // null: the context's stream
 int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
                     uint32_t kind_a, hipStream_t stream);
 // Fork the side stream from the context's stream (everything enqueued there so far is visible to it), and join it

@@ -931,32 +931,33 @@ int upload_weights() {
 }
 
 template <typename T, int NT, uint32_t KA, uint32_t KB>
-static void launch_one(rcg_ctx* ctx, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, const Taps<T>& t) {
-    hipLaunchKernelGGL((k_ssim_scale<T, NT, KA, KB>), dim3(nunits), dim3(NT), 0, ctx->stream, d_jobs, d_units, t,
+static void launch_one(rcg_ctx* ctx, hipStream_t stream, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits,
+                       const Taps<T>& t) {
+    hipLaunchKernelGGL((k_ssim_scale<T, NT, KA, KB>), dim3(nunits), dim3(NT), 0, stream, d_jobs, d_units, t,
                        ctx->d_lut);
 }
 
 template <typename T, int NT>
-static int launch_kinds(rcg_ctx* ctx, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, const Taps<T>& t,
-                        uint32_t ka, uint32_t kb) {
+static int launch_kinds(rcg_ctx* ctx, hipStream_t st, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits,
+                        const Taps<T>& t, uint32_t ka, uint32_t kb) {
     if constexpr (sizeof(T) == 4) {
         if (ka == kSrcRefCache && kb == kSrcRecon)
-            launch_one<T, NT, kSrcRefCache, kSrcRecon>(ctx, d_jobs, d_units, nunits, t);
+            launch_one<T, NT, kSrcRefCache, kSrcRecon>(ctx, st, d_jobs, d_units, nunits, t);
         else if (ka == kSrcRefCache && kb == kSrcBytes)
-            launch_one<T, NT, kSrcRefCache, kSrcBytes>(ctx, d_jobs, d_units, nunits, t);
+            launch_one<T, NT, kSrcRefCache, kSrcBytes>(ctx, st, d_jobs, d_units, nunits, t);
         else if (ka == kSrcRefCache && kb == kSrcPlanes)
-            launch_one<T, NT, kSrcRefCache, kSrcPlanes>(ctx, d_jobs, d_units, nunits, t);
+            launch_one<T, NT, kSrcRefCache, kSrcPlanes>(ctx, st, d_jobs, d_units, nunits, t);
         else {
             set_error("internal: unsupported SSIMULACRA2 source combination");
             return RCG_DEVICE;
         }
     } else {
         if (ka == kSrcBytes && kb == kSrcRecon)
-            launch_one<T, NT, kSrcBytes, kSrcRecon>(ctx, d_jobs, d_units, nunits, t);
+            launch_one<T, NT, kSrcBytes, kSrcRecon>(ctx, st, d_jobs, d_units, nunits, t);
         else if (ka == kSrcBytes && kb == kSrcBytes)
-            launch_one<T, NT, kSrcBytes, kSrcBytes>(ctx, d_jobs, d_units, nunits, t);
+            launch_one<T, NT, kSrcBytes, kSrcBytes>(ctx, st, d_jobs, d_units, nunits, t);
         else if (ka == kSrcPlanes && kb == kSrcPlanes)
-            launch_one<T, NT, kSrcPlanes, kSrcPlanes>(ctx, d_jobs, d_units, nunits, t);
+            launch_one<T, NT, kSrcPlanes, kSrcPlanes>(ctx, st, d_jobs, d_units, nunits, t);
         else {
             set_error("internal: unsupported SSIMULACRA2 source combination");
             return RCG_DEVICE;
@@ -966,19 +967,20 @@ static int launch_kinds(rcg_ctx* ctx, const SsimJob* d_jobs, const Unit* d_units
 }
 
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
-                      uint32_t kind_a, uint32_t kind_b) {
+                      uint32_t kind_a, uint32_t kind_b, hipStream_t stream) {
     if (nunits == 0) return RCG_OK;
-    ProfScope ps(ctx, "k_ssim_scale");
+    const hipStream_t st = stream ? stream : ctx->stream;
+    ProfScope ps(ctx, "k_ssim_scale", st);
     int rc;
     if (precision == RCG_PREC_FP64) {
         Taps<double> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps64[i];
-        rc = launch_kinds<double, 128>(ctx, d_jobs, d_units, nunits, t, kind_a, kind_b);
+        rc = launch_kinds<double, 128>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b);
     } else {
         Taps<float> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
-        rc = nt == 128 ? launch_kinds<float, 128>(ctx, d_jobs, d_units, nunits, t, kind_a, kind_b)
-                       : launch_kinds<float, 256>(ctx, d_jobs, d_units, nunits, t, kind_a, kind_b);
+        rc = nt == 128 ? launch_kinds<float, 128>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b)
+                       : launch_kinds<float, 256>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b);
     }
     if (rc != RCG_OK) return rc;
     RCG_HIP(hipGetLastError());
