@@ -197,7 +197,8 @@ void rcg_batch_plan_destroy(rcg_plan* plan);
  * class i with h >= min_h[i] (classes in descending min_h, the last with min_h = 0), and 128-thread workgroups when
  * its width is at most narrow_w (else wide_nt = 256). The policy depends only on an image's own size, so a score
  * never depends on what else shares a launch; different policies change scores only in the last bits.
- * rcg_autotune times the scale kernel for th in {2,4,...,256,320,384,512} at each size class
+ * rcg_autotune times the scale kernel for th in {2, 4, 6, 8, 12, 16, 24, 32, 40, ..., 112 (steps of 8), 128, 144,
+ * 160, 176, 208, 256, 320, 384, 512} at each size class
  * (4096 down to 64 rows) and installs the fastest per class; it returns the installed policy. `jobs` (1..64) is how
  * many same-size pairs share a launch in the workload the policy is for: the candidates of one image for a single
  * search or sweep, more for a batch of many images (their launches hold many pairs of every size, so short tiles,

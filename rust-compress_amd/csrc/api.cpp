@@ -1472,11 +1472,12 @@ extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     RCG_HIP(hipSetDevice(ctx->device));
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
-    // Tile heights: powers of two and the steps between them (a launch's last round of workgroups is the less idle
-    // the closer its tile count comes to a multiple of the GPU's workgroup slots). Small scales are latency-bound
+    // Tile heights: a fine grid (a launch's last round of workgroups is the less idle the closer its tile count comes
+    // to a multiple of the GPU's workgroup slots, and one step of the grid moves the count by a few percent). Small scales are latency-bound
     // (one workgroup walks TH + 10 rows one after another), so very short tiles are tried too; a launch of many
     // rounds (throughput) is best served by tall ones, whose 10-row halo costs least.
-    const int ths[] = {2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256, 320, 384, 512};
+    const int ths[] = {2,  4,  6,  8,   12,  16,  24,  32,  40,  48,  56,  64,  72,  80,  88,  96,
+                       104, 112, 128, 144, 160, 176, 208, 256, 320, 384, 512};
     const int nsz = 7;
     const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
     ctx->arena.reset();

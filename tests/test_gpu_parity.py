@@ -276,7 +276,9 @@ def test_autotuned_policy_keeps_parity(rc, jobs):
     c = rc.Context(0)
     t = c.autotune(jobs)
     assert 1 <= t.nclass <= 7
-    assert all(t.th[i] in (2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 160, 208, 256, 320, 384, 512) for i in range(t.nclass))
+    grid = (2, 4, 6, 8, 12, 16, 24, 32, 40, 48, 56, 64, 72, 80, 88, 96, 104, 112, 128, 144, 160, 176, 208, 256, 320,
+            384, 512)
+    assert all(t.th[i] in grid for i in range(t.nclass))
     img = rc.synth_rgb8(12, 700, 530)
     dist = O.reconstruct(O.build_coefs(img, 60, O.S420))
     assert abs(c.ssimulacra2(img, dist, rc.FP32) - O.ssimulacra2(img, dist)) <= TOL32
