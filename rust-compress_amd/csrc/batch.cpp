@@ -163,11 +163,13 @@ std::vector<Unit> xcd_units(const std::vector<UnitGroup>& groups) {
     for (const UnitGroup& g : groups) total += size_t(g.njobs) * g.tiles;
     for (auto& l : lists) l.reserve(total / kXcds + 64);
     for (const UnitGroup& g : groups) {
-        for (uint32_t t = 0; t < g.tiles; t++) {
+        const uint32_t chunk = std::max(1u, g.chunk);
+        for (uint32_t t0 = 0; t0 < g.tiles; t0 += chunk) {
             int x = 0;
             for (int i = 1; i < kXcds; i++)
                 if (lists[i].size() < lists[x].size()) x = i;
-            for (uint32_t c = 0; c < g.njobs; c++) lists[x].push_back(Unit{g.job0 + c, t});
+            for (uint32_t t = t0; t < std::min(g.tiles, t0 + chunk); t++)
+                for (uint32_t c = 0; c < g.njobs; c++) lists[x].push_back(Unit{g.job0 + c, t});
         }
     }
     size_t len = 0;
@@ -210,7 +212,9 @@ size_t desc_bytes(rcg_ctx* ctx, const Item* items, int n) {
         const PairPlan P32 = plan_pair(it.w, it.h, ctx->tune32);
         const PairPlan P64 = plan_pair(it.w, it.h, ctx->tune64);
         size_t units = 0;
-        for (uint32_t s = 0; s < P32.nscales; s++) units += size_t(P32.tiles[s]) * it.ncand + P32.tiles_ref[s];
+        // A reference launch keeps tile rows together on one XCD: its idle padding is at most kXcds - 1 tile rows.
+        for (uint32_t s = 0; s < P32.nscales; s++)
+            units += size_t(P32.tiles[s]) * it.ncand + P32.tiles_ref[s] + size_t(kXcds) * P32.tiles_x[s];
         for (uint32_t s = 0; s < P64.nscales; s++) units += size_t(P64.tiles[s]) * it.ncand;
         d += units * sizeof(Unit);
     }
@@ -261,7 +265,9 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     const Item& it = items[i];
                     const PairPlan& P = bufs[i].P32;
                     if (uint32_t(s) >= P.nscales || int(P.nt[s]) != nt) continue;
-                    groups.push_back(UnitGroup{uint32_t(rj.size()), 1, P.tiles_ref[s]});
+                    // A tile row on one XCD: neighbouring tiles share the cache lines at their seams (a tile row
+                    // segment is 976 bytes), which then meet in one L2 instead of reaching HBM as partial lines.
+                    groups.push_back(UnitGroup{uint32_t(rj.size()), 1, P.tiles_ref[s], P.tiles_x[s]});
                     rj.push_back(make_ref_job(P, uint32_t(s), bytes_src(it.d_src, it.ch), it.ref));
                 }
                 if (!rj.empty()) {

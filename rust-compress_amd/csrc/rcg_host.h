@@ -164,10 +164,12 @@ int wait_side_mark(rcg_ctx* ctx);  // the context's stream waits for ev_mid
 // scale, or a single job).
 struct UnitGroup {
     uint32_t job0, njobs, tiles;
+    uint32_t chunk = 1;  // consecutive tiles kept together on one XCD
 };
-// The unit table of a launch (see Unit): each tile of a group becomes a chunk of njobs consecutive units on one XCD,
-// and chunks go to the XCD with the fewest units so far (round-robin when the groups are alike). XCDs left shorter
-// than the longest are padded with idle units. A launch of single-job groups gets units in plain tile order.
+// The unit table of a launch (see Unit): each `chunk` consecutive tiles of a group become a chunk of chunk*njobs
+// consecutive units on one XCD, and chunks go to the XCD with the fewest units so far (round-robin when the groups
+// are alike). XCDs left shorter than the longest are padded with idle units. A launch of single-job, single-tile-chunk
+// groups gets units in plain tile order.
 std::vector<Unit> xcd_units(const std::vector<UnitGroup>& groups);
 struct FinalDesc {
     uint32_t nscales;
