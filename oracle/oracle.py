@@ -24,8 +24,8 @@ AUTO, S444, S422, S420 = 0, 1, 2, 3
 
 
 def build():
-    """Compile liboracle.so from oracle.c (gcc, -ffp-contract=off)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    """Compile liboracle.so from oracle.c (gcc, -ffp-contract=off), every time it is called."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
 
 
 def lib():
