@@ -51,8 +51,9 @@ __device__ __forceinline__ double mad(double a, double b, double c) { return fma
 //
 // f64 follows the reference: mu_a, mu_b, blur(a^2), blur(b^2), blur(ab) and
 //   d = 1 - (1 - (mu_a - mu_b)^2) * (2*sigma_ab + C2) / (sigma_aa + sigma_bb + C2).
-// f32 takes the reference image's side from its per-scale cache (a, mu_a and sigma_aa = blur(a^2) - mu_a^2, built
-// once per image by k_ssim_ref) and blurs three quantities of the candidate: mu_b, blur(b^2) and R = blur((a-b)^2).
+// f32 takes the reference image's side from its per-scale cache (a, mu_a and sigma_aa + C2, with sigma_aa =
+// blur(a^2) - mu_a^2, built once per image by k_ssim_ref) and blurs three quantities of the candidate: mu_b, blur(b^2)
+// and R = blur((a-b)^2).
 // With dmu = mu_a - mu_b, V = R - dmu^2 (= sigma_aa + sigma_bb - 2*sigma_ab, the local variance of a - b) and
 // D = sigma_aa + (blur(b^2) - mu_b^2) + C2 (= sigma_aa + sigma_bb + C2), the same map is
 //   d = (V + dmu^2 * (D - V)) / D,
@@ -102,12 +103,12 @@ struct Form<float> {
         q[1] = mad(kt, mad(bu, bu, bd * bd), q[1]);
         q[2] = mad(kt, eu + ed, q[2]);
     }
-    // v: the candidate's blurred quantities; ma, sa: mu_a and sigma_aa from the reference cache.
+    // v: the candidate's blurred quantities; ma, sa: mu_a and sigma_aa + C2 (C2 = 0.0009) from the reference cache.
     static __device__ __forceinline__ float ssim(const float v[3], float ma, float sa) {
         const float m2 = v[0], B2 = v[1], R = v[2];
         const float dmu = ma - m2, dmu2 = dmu * dmu;
         const float V = R - dmu2;
-        const float D = (sa + mad(-m2, m2, B2)) + 0.0009f;
+        const float D = sa + mad(-m2, m2, B2);  // sa holds sigma_aa + C2
         return mad(dmu2, D - V, V) * __builtin_amdgcn_rcpf(D);
     }
 };
@@ -147,13 +148,15 @@ struct Xyb<float> {
         const float M22 = float(1.0 - 0.2434226892454782 - 0.2047674442449682);
         const float BIAS = float(0.0037930732552754493);
         const float CB = float(0.15595420054924863);  // cbrt(BIAS), rounded to f32
-        const float m0 = cbrt_pos(fmaxf(mad(M00, r, mad(M01, g, mad(M02, b, BIAS))), 0.0f)) - CB;
-        const float m1 = cbrt_pos(fmaxf(mad(M10, r, mad(M11, g, mad(M12, b, BIAS))), 0.0f)) - CB;
-        const float m2 = cbrt_pos(fmaxf(mad(M20, r, mad(M21, g, mad(M22, b, BIAS))), 0.0f)) - CB;
-        const float x = 0.5f * (m0 - m1), y = 0.5f * (m0 + m1);
-        X = x * 14.0f;      // (x*14 + 0.42) - 0.42
-        Y = y - 0.5f;       // (y + 0.01) - 0.51
-        B = m2 - y;         // ((m2 - y) + 0.55) - 0.55
+        // With c_i the cube roots and m_i = c_i - CB (the reference's opsin values): x = (m0 - m1)/2, y = (m0 + m1)/2,
+        // X = 14x, Y = y + 0.01, B = (m2 - y) + 0.55. CB cancels from X and B, and the shifts fold into constants.
+        const float c0 = cbrt_pos(fmaxf(mad(M00, r, mad(M01, g, mad(M02, b, BIAS))), 0.0f));
+        const float c1 = cbrt_pos(fmaxf(mad(M10, r, mad(M11, g, mad(M12, b, BIAS))), 0.0f));
+        const float c2 = cbrt_pos(fmaxf(mad(M20, r, mad(M21, g, mad(M22, b, BIAS))), 0.0f));
+        const float s = c0 + c1;
+        X = 7.0f * (c0 - c1);          // 14x, shifted: (14x + 0.42) - 0.42
+        Y = mad(0.5f, s, -(CB + 0.5f));  // (y + 0.01) - 0.51
+        B = mad(-0.5f, s, c2);           // ((m2 - y) + 0.55) - 0.55
     }
 };
 
@@ -348,7 +351,7 @@ struct Loader<kSrcRecon> {
 };
 
 // The reference image's per-scale cache written by k_ssim_ref: planes [9][h][w] holding the shifted XYB (0-2),
-// its blur mu_a (3-5) and sigma_aa = blur(a^2) - mu_a^2 (6-8). load() returns XYB, not linear RGB.
+// its blur mu_a (3-5) and sigma_aa + C2 = blur(a^2) - mu_a^2 + 0.0009 (6-8). load() returns XYB, not linear RGB.
 template <>
 struct Loader<kSrcRefCache> {
     const float* base;
@@ -692,10 +695,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 // ----------------------------------------------------------------------------------------------------------------
 // Reference cache. k_ssim_ref<NT, KA> walks one image at one scale with the same tiles and border rules as the scale
 // kernel and writes, for every pixel, the quantities of the reference side that do not depend on a candidate: the
-// shifted XYB a (planes 0-2), mu_a = blur(a) (3-5) and sigma_aa = blur(a^2) - mu_a^2 (6-8); it also writes the
-// image's next pyramid level. Every f32 candidate evaluation of the image then reads these 9 floats per pixel instead
-// of recomputing the reference's colour transform and two of its five blurs. The values at a pixel do not depend on
-// the tiling, so the cache is the same whichever tile policy built it.
+// shifted XYB a (planes 0-2), mu_a = blur(a) (3-5) and sigma_aa + C2 = blur(a^2) - mu_a^2 + 0.0009 (6-8); it also
+// writes the image's next pyramid level. Every f32 candidate evaluation of the image then reads these 9 floats per
+// pixel instead of recomputing the reference's colour transform and two of its five blurs. The values at a pixel do
+// not depend on the tiling, so the cache is the same whichever tile policy built it.
 
 template <int NT, uint32_t KA>
 struct RefWalk {
@@ -780,7 +783,7 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                 }
                 gstore_b(o + c * n, st.oc, st.ra[SC][c]);
                 gstore_b(o + (3 + c) * n, st.oc, v[0]);
-                gstore_b(o + (6 + c) * n, st.oc, mad(-v[0], v[0], v[1]));
+                gstore_b(o + (6 + c) * n, st.oc, mad(-v[0], v[0], v[1]) + 0.0009f);  // sigma_aa + C2
             }
         }
     }
