@@ -417,6 +417,15 @@ struct QBuf {
     static __device__ __forceinline__ int at(int buf, int q, int x) {
         return buf * kBuf + (kInterleaved ? x * QS + q : q * NT + x);
     }
+    static constexpr int qoff(int q) { return kInterleaved ? q : q * NT; }  // at(buf, q, x) - at(buf, 0, x)
+    // Index of the lowest element a horizontal 11-tap window of column x reads (valid for x >= 5), made opaque to the
+    // compiler: taps addressed from it have non-negative constant offsets, which the LDS instructions take as
+    // immediates, where offsets below a centre column would each need a base register of their own.
+    static __device__ __forceinline__ int window(int buf, int x) {
+        int lo = at(buf, 0, x) - 5 * kCol;
+        asm volatile("" : "+v"(lo));
+        return lo;
+    }
 };
 
 // Per-workgroup constants and per-thread state of the scale kernel's row walk. The f32 kernel reads the reference
@@ -569,6 +578,8 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
                 st.la.begin_out_row(uint32_t(yy - 5));
                 st.la.load_stats(ac, ma, sa);
             }
+            using Q = typename Walk<T, NT, KA, KB>::Q;
+            const T* win = &st.s_q[Q::window(buf, tid)];  // output lanes only: tid >= 6
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 // One channel at a time: the scheduler may not pull the next channel's LDS reads ahead of this
@@ -577,8 +588,7 @@ __device__ __forceinline__ bool walk_row(Walk<T, NT, KA, KB>& st) {
                 T v[NQ];
 #pragma unroll
                 for (int i = 0; i < NQ; i++) {
-                    using Q = typename Walk<T, NT, KA, KB>::Q;
-                    const T* row = &st.s_q[Q::at(buf, c * NQ + i, tid)];
+                    const T* row = win + Q::qoff(c * NQ + i) + 5 * Q::kCol;
                     T sum = k[0] * row[0];
 #pragma unroll
                     for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t * Q::kCol] + row[t * Q::kCol], sum);
@@ -774,6 +784,8 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                 float v[2];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
+                    // Planes of NT floats: a plane's offset does not fit the paired reads' immediates, so each
+                    // quantity keeps its own base (window() does not help here).
                     using Q = typename RefWalk<NT, KA>::Q;
                     const float* row = &st.s_q[Q::at(buf, 2 * c + i, tid)];
                     float sum = k[0] * row[0];
