@@ -265,8 +265,9 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     const Item& it = items[i];
                     const PairPlan& P = bufs[i].P32;
                     if (uint32_t(s) >= P.nscales || int(P.nt[s]) != nt) continue;
-                    // A tile row on one XCD: neighbouring tiles share the cache lines at their seams (a tile row
-                    // segment is 976 bytes), which then meet in one L2 instead of reaching HBM as partial lines.
+                    // A tile row on one XCD, so that neighbouring tiles write the two halves of the cache lines at
+                    // their seams (a tile row segment is 976 bytes) through one L2. Measured on one 4096x4096 image
+                    // with six candidates: the whole step 2.3% faster.
                     groups.push_back(UnitGroup{uint32_t(rj.size()), 1, P.tiles_ref[s], P.tiles_x[s]});
                     rj.push_back(make_ref_job(P, uint32_t(s), bytes_src(it.d_src, it.ch), it.ref));
                 }

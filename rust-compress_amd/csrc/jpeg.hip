@@ -8,10 +8,11 @@
 //   k_upsample_color sample planes -> full-size Gray8 or RGB8 ("fancy" triangle upsampling, fixed-point
 //                    YCbCr->RGB; decode.rs:1404-1445, 1769-1829).
 //
-// Layout: one encode workgroup owns a 512-pixel-wide, one-MCU-tall strip. Its RGB rows are read with aligned
-// dword loads into LDS, converted there, and each of the 192 threads then transforms one 8x8 block held entirely
-// in registers (the AAN butterflies need no cross-lane traffic). Coefficients leave as 16-byte stores, 128 bytes
-// per block, in the reference's raster block order so the host wraps them without a permutation.
+// Layout: one encode workgroup owns a 512-pixel-wide, one-MCU-tall strip. Its RGB rows are read as aligned 16-byte
+// chunks into LDS (all of a thread's loads in flight at once), converted there by a per-layout routine, and each of
+// the 192 threads then transforms one 8x8 block held entirely in registers (the AAN butterflies need no cross-lane
+// traffic). Coefficients leave as 16-byte stores, 128 bytes per block, in the reference's raster block order so the
+// host wraps them without a permutation.
 #include "rcg_device.h"
 #include "rcg_host.h"
 #include "rcg_jobs.h"
