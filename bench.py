@@ -126,7 +126,8 @@ def main():
     ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
     ap.add_argument("--cli-files", type=int, default=24)
     ap.add_argument("--batch-images", type=int, default=256)
-    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests of the serving run (Poisson arrivals at 200 per second)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (HBM traffic counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -428,7 +429,7 @@ def bench_serve(rc, args, tune=None):
     targets = rng.choice([60.0, 70.0, 80.0], n)
     gaps = rng.exponential(1.0 / 200.0, n)
     srv = rc.Server(0, speculate=3, tune=tune)
-    # Warm-up: one request of each pool image size class is not needed; run one request to initialise.
+    # Warm-up: one request initialises the worker's context before the clock starts.
     srv.submit(10 ** 9, pool[0], 70.0)
     while not srv.poll(8, 500):
         pass

@@ -223,7 +223,9 @@ int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out);
  * GPU analogue of cli/src/budget.rs:11-50: a request that alone exceeds the budget runs, but by itself. Every tick
  * scores the next candidate(s) of every admitted request's bisection (jpeg/src/target.rs:37-55) in one batched
  * launch sequence; requests that arrive meanwhile join at the next tick. Sources and reference pyramids stay
- * resident in HBM for the life of a request. Each result equals the one the request gets alone.
+ * resident in HBM for the life of a request; a finished request's device blocks are kept for reuse by later ones (up
+ * to hbm_budget bytes of them), so a steady stream of requests allocates nothing. Each result equals the one the
+ * request gets alone.
  * speculate = 1 scores one bisection midpoint per tick; 3 also scores both possible next midpoints, so a request
  * takes about half the ticks for 1.5x the work.
  * This entry point has no single counterpart in the reference (it replaces the per-file rayon tasks of
