@@ -48,6 +48,28 @@ def test_results_alone_equal_results_in_the_mix(rc, ctx):
         assert st.mean_batch > 1.0, "requests were never batched together"
 
 
+def test_reused_device_blocks_keep_results_alone(rc, ctx):
+    # Each wave is drained before the next is submitted, so later waves run in the device blocks that earlier
+    # requests of the same sizes left behind (their stale sources and reference states must not leak in).
+    sizes = [(700, 500), (640, 560), (700, 500)]
+    srv = rc.Server(0, speculate=3)
+    for wave in range(3):
+        reqs = []
+        for j, (w, h) in enumerate(sizes):
+            i = wave * len(sizes) + j
+            reqs.append((i, rc.synth_rgb8(700 + i, w, h), float([60.0, 70.0, 80.0][(wave + j) % 3])))
+        for i, img, t in reqs:
+            srv.submit(i, img, t)
+        got = _drain(srv, len(reqs))
+        for i, img, t in reqs:
+            f = ctx.find_quality(img, t, 100, rc.S420)
+            assert got[i].status == 0
+            assert (got[i].quality, got[i].score) == (f.quality, f.score), (wave, i)
+    st = srv.stats()
+    srv.close()
+    assert st.requests_done == 3 * len(sizes)
+
+
 def test_budget_admits_oversized_request_alone(rc):
     srv = rc.Server(0, hbm_budget=4096)  # 4 KiB: every request is larger than the whole budget
     imgs = [rc.synth_rgb8(400 + i, 128, 96) for i in range(3)]
