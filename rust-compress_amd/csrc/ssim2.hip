@@ -212,6 +212,11 @@ template <class V>
 __device__ __forceinline__ void gstore_b(V* row, uint32_t byte_off, V v) {
     *(RCG_GLOBAL V*)(reinterpret_cast<char*>(uniform_ptr(row)) + byte_off) = v;
 }
+// gstore_b with the non-temporal hint (the nt bit of the store).
+template <class V>
+__device__ __forceinline__ void gstore_nt_b(V* row, uint32_t byte_off, V v) {
+    __builtin_nontemporal_store(v, (RCG_GLOBAL V*)(reinterpret_cast<char*>(uniform_ptr(row)) + byte_off));
+}
 
 // Row loaders of linear-light RGB. begin_row() takes the row the whole workgroup is on (uniform), so row pointers
 // are scalar. prep(x) computes the thread's byte offsets within a row once, for its column x; fresh() re-issues them
@@ -793,9 +798,12 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                     for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t * Q::kCol] + row[t * Q::kCol], sum);
                     v[i] = sum;
                 }
-                gstore_b(o + c * n, st.oc, st.ra[SC][c]);
-                gstore_b(o + (3 + c) * n, st.oc, v[0]);
-                gstore_b(o + (6 + c) * n, st.oc, mad(-v[0], v[0], v[1]) + 0.0009f);  // sigma_aa + C2
+                // Non-temporal: the cache is written once and read back only by later launches. Measured on one
+                // 4096x4096 image with six candidates, where this launch runs beside the encode launch, the whole
+                // pass was about 2% shorter than with ordinary stores.
+                gstore_nt_b(o + c * n, st.oc, st.ra[SC][c]);
+                gstore_nt_b(o + (3 + c) * n, st.oc, v[0]);
+                gstore_nt_b(o + (6 + c) * n, st.oc, mad(-v[0], v[0], v[1]) + 0.0009f);  // sigma_aa + C2
             }
         }
     }
