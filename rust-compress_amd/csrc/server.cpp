@@ -285,9 +285,9 @@ struct rcg_server {
             }
             const double t1 = now_s();
             if (!fresh.empty()) {
-                // The tick read the results back, so the uploads have completed; synchronise anyway for a tick that
-                // scored nothing (every fresh request failed to materialise).
-                if (items.empty()) (void)hipStreamSynchronize(ctx->stream);
+                // A tick that succeeded read its results back, so the uploads ahead of it have completed. A tick that
+                // scored nothing (every fresh request failed to materialise) or failed part way may not have waited.
+                if (items.empty() || rc != RCG_OK) (void)hipStreamSynchronize(ctx->stream);
                 for (Request* r : fresh) std::vector<uint8_t>().swap(r->pixels);
             }
             std::lock_guard<std::mutex> lk(mu);
