@@ -300,9 +300,11 @@ struct StagedPairs {
 
 // Stages the descriptors to score every pair in `pairs` at `precision` into D. FP32 first builds the reference state
 // of every item that lacks it, then writes each candidate's pyramid; FP64 reads both pyramids from the FP32 pass.
-// The pairs of one item must be consecutive in `pairs` (they form one unit group per scale).
+// The pairs of one item must be consecutive in `pairs` (they form one unit group per scale). `parts` selects what
+// is staged: kStageRefs the reference launches, kStageScales the candidate and pooling launches.
+constexpr int kStageRefs = 1, kStageScales = 2, kStageAll = 3;
 StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<ItemBufs>& bufs,
-                        const std::vector<PairRef>& pairs, int precision, double* d_scores) {
+                        const std::vector<PairRef>& pairs, int precision, double* d_scores, int parts = kStageAll) {
     const bool f64 = precision == RCG_PREC_FP64;
     StagedPairs out;
     out.precision = precision;
@@ -315,7 +317,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
         for (const PairRef& pr : pairs) build[pr.item] = !items[pr.item].ref_ready;
     for (int s = 0; s < 6; s++) {
         for (int nt : {256, 128}) {
-            if (!f64) {
+            if (!f64 && (parts & kStageRefs)) {
                 std::vector<RefJob> rj;
                 std::vector<UnitGroup> groups;
                 for (int i = 0; i < nitems; i++) {
@@ -343,6 +345,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     out.refs.push_back(g);
                 }
             }
+            if (!(parts & kStageScales)) continue;
             std::vector<SsimJob> sj;
             std::vector<UnitGroup> groups;
             for (size_t i = 0; i < pairs.size(); i++) {
@@ -395,6 +398,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
             }
         }
     }
+    if (!(parts & kStageScales)) return out;
     for (size_t i = 0; i < pairs.size(); i++) {
         fins[i].score_out = d_scores + i;
         fins[i].stats_out = nullptr;
@@ -531,7 +535,8 @@ void retile_refs(const SsimTune& T, const Item* items, int n, std::vector<ItemBu
     }
 }
 
-int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, ScoreState& S) {
+// The encode descriptors: arena buffers of every pair, then each item's encode jobs, staged at the start of S.D.
+int prepare_encode(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, ScoreState& S) {
     size_t npairs = 0;
     for (int i = 0; i < n; i++) npairs += items[i].ncand;
     const size_t dbytes = desc_bytes(ctx, items, n);
@@ -612,19 +617,26 @@ int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, Score
         S.d_ej = S.D.put(ej.data(), ej.size());
         S.d_eb = S.D.put(ebegins.data(), ebegins.size());
     }
+    return RCG_OK;
+}
+
+// The FP32 SSIMULACRA2 descriptors, staged after the encode ones.
+void prepare_pairs(rcg_ctx* ctx, Item* items, int n, ScoreState& S) {
     retile_refs(ctx->tune32, items, n, S.bufs);
     S.fp32 = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32);
     S.fp32_desc_end = S.D.off;
+}
+
+int prepare_items(rcg_ctx* ctx, Arena& A, Pinned& pin, Item* items, int n, ScoreState& S) {
+    RCG_TRY(prepare_encode(ctx, A, pin, items, n, S));
+    prepare_pairs(ctx, items, n, S);
     return S.D.upload(ctx->stream);
 }
 
-int enqueue_items(rcg_ctx* ctx, const ScoreState& S) {
-    // The reference caches are built on the side stream beside the encode; the encode is enqueued first so that it
-    // starts while the host is still enqueueing the reference launches.
-    const bool refs = !S.fp32.refs.empty();
-    if (refs) RCG_TRY(fork_side(ctx));
-    if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
-    if (refs) RCG_TRY(launch_refs_side(ctx, S.fp32));
+// Everything after the encode launch: the reference launches (on the side stream, forked already when `refs`; unless
+// the caller launched them already), the candidate launches, the pooling launch and the copy of the FP32 scores.
+int enqueue_after_encode(rcg_ctx* ctx, const ScoreState& S, bool refs, bool launch_refs = true) {
+    if (refs && launch_refs) RCG_TRY(launch_refs_side(ctx, S.fp32));
     if (S.pairs.empty()) {
         if (refs) RCG_TRY(join_side(ctx));
     } else {
@@ -633,6 +645,15 @@ int enqueue_items(rcg_ctx* ctx, const ScoreState& S) {
                                ctx->stream));
     }
     return RCG_OK;
+}
+
+int enqueue_items(rcg_ctx* ctx, const ScoreState& S) {
+    // The reference caches are built on the side stream beside the encode; the encode is enqueued first so that it
+    // starts while the host is still enqueueing the reference launches.
+    const bool refs = !S.fp32.refs.empty();
+    if (refs) RCG_TRY(fork_side(ctx));
+    if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
+    return enqueue_after_encode(ctx, S, refs);
 }
 
 // The stream must be synchronised (the FP32 scores are in S.h_scores).
@@ -666,10 +687,38 @@ int finish_items(rcg_ctx* ctx, ScoreState& S, Item* items, int n) {
 // Score every candidate of every item: one encode launch, one launch per scale, one pooling launch; then FP64
 // settling of the scores inside the guard band of each item's target. The arena must already hold
 // score_items_bytes() beyond what the caller took.
+//
+// The encode is launched as soon as its own descriptors are uploaded; the host stages the SSIMULACRA2 descriptors
+// (tens of microseconds for one large image) while it runs. They go up on the side stream ahead of the reference
+// launches, which every candidate launch waits for (the scale-0 ones through the side stream's mark, the rest
+// through the join), or, when no reference state is built, on the context's stream behind the encode. The side
+// stream is forked before the encode launch, so the reference launches do not wait for the encode.
 int score_items(rcg_ctx* ctx, Item* items, int n) {
     ScoreState S;
-    RCG_TRY(prepare_items(ctx, ctx->arena, ctx->pinned, items, n, S));
-    RCG_TRY(enqueue_items(ctx, S));
+    RCG_TRY(prepare_encode(ctx, ctx->arena, ctx->pinned, items, n, S));
+    retile_refs(ctx->tune32, items, n, S.bufs);
+    const StagedPairs refs_sp = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32, kStageRefs);
+    const size_t first_end = S.D.off;
+    RCG_TRY(S.D.upload_range(0, first_end, ctx->stream));
+    const bool refs = !refs_sp.refs.empty();
+    if (refs) RCG_TRY(fork_side(ctx));
+    if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
+    // From here on work is in flight: a failure waits for both streams before it returns, so that no launch of this
+    // call still reads the caller's buffers afterwards.
+    auto rest = [&]() -> int {
+        if (refs) RCG_TRY(launch_refs_side(ctx, refs_sp));
+        S.fp32 = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32, kStageScales);
+        S.fp32.refs = refs_sp.refs;
+        S.fp32_desc_end = S.D.off;
+        RCG_TRY(S.D.upload_range(first_end, S.D.off, ctx->stream));
+        return enqueue_after_encode(ctx, S, refs, false);
+    };
+    const int rc = rest();
+    if (rc != RCG_OK) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamSynchronize(ctx->stream);
+        return rc;
+    }
     RCG_HIP(hipStreamSynchronize(ctx->stream));
     return finish_items(ctx, S, items, n);
 }

@@ -231,12 +231,14 @@ struct Desc {
         off += sizeof(T) * n;
         return d;
     }
-    int upload(hipStream_t s) {
+    int upload(hipStream_t s) { return upload_range(0, off, s); }
+    // Bytes [begin, end) of what was staged.
+    int upload_range(size_t begin, size_t end, hipStream_t s) {
         if (overflow) {
             set_error("internal: descriptor staging overflow");
             return RCG_DEVICE;
         }
-        if (off) RCG_HIP(hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, s));
+        if (end > begin) RCG_HIP(hipMemcpyAsync(dev + begin, host + begin, end - begin, hipMemcpyHostToDevice, s));
         return RCG_OK;
     }
 };
