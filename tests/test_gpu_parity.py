@@ -253,6 +253,20 @@ def test_batch_scores_do_not_depend_on_the_rest_of_the_batch(ctx, rc):
         assert r.chosen_quality == alone.chosen_quality
 
 
+def test_large_call_scores_equal_single_candidate_calls(ctx, rc):
+    # A call of 2^25 candidate-pixels or more uploads its descriptors in two parts and stages the candidate launches
+    # while the encode and reference launches run; a call below that (one candidate of the same image) uploads them
+    # at once. The scores agree bit for bit.
+    img = rc.synth_rgb8(31, 4096, 2048)
+    cands = [(q, s) for s in (rc.S420, rc.S444) for q in (50, 75, 90)]
+    together = ctx.score_batch([img], cands, 70.0)[0]
+    assert together.status == 0
+    for k, cand in enumerate(cands):
+        alone = ctx.score_batch([img], [cand], 70.0)[0]
+        assert alone.status == 0
+        assert alone.scores[0] == together.scores[k], cand
+
+
 def test_score_batch_reports_per_job_errors(ctx, rc):
     good = rc.synth_rgb8(1, 64, 64)
     res = ctx.score_batch([good, np.zeros((7, 64, 3), np.uint8)], CANDS, 70.0)
