@@ -18,8 +18,14 @@
  * Pinning (how each function is checked, see tests/test_oracle_*.py):
  *   orc_build_coefs   - hierarchical fixture frame-0 entropy data (colour 4:2:0, gray, right/bottom padding),
  *                       DQT bytes of the q85 table, DCT-vs-naive and flat-block closed forms.
- *   orc_reconstruct   - DC-only closed form, naive f64 IDCT agreement, constant/ramp upsampling closed forms,
- *                       round-trip PSNR floors; 4:4:4/odd-chroma colour decode is pinned only by restatement.
+ *   orc_reconstruct   - the reference decoder's recorded decodes of two hierarchical fixtures, sample for sample
+ *                       (f32 IDCT outputs over three frames each, the (2,2) upsampling and YCbCr -> RGB through
+ *                       orc_idct_plane / orc_upsample / orc_ycc_to_rgb, tests/test_hier_golden.py, with planted
+ *                       mistakes that must break the match); DC-only closed form, naive f64 IDCT agreement,
+ *                       round-trip PSNR floors. Changes of about one ulp in the IDCT (a re-association) are too
+ *                       small for those files: the exact f32 operation order is pinned by restatement.
+ *   orc_cmyk_to_rgb   - Pillow's conversions of the reference's CMYK/YCCK fixtures within the reference's own
+ *                       tolerance (cli/tests/cmyk.rs:14-80); the formula itself is pinned by restatement.
  *   orc_ssimulacra2   - the 54 py-ssimulacra2 golden pairs (fixtures/ssim2/expected.csv) at 1e-4, identity = 100.
  *   orc_find_quality  - bisection semantics against brute-force search over monotone score tables.
  *   orc_build_lengths - Kraft completeness, the length limit, single-symbol and textbook-Huffman optimality.
@@ -342,33 +348,67 @@ static int build_impl(const uint8_t* pixels, int w, int h, int channels, int qua
     return 0;
 }
 
-/* jpeg/src/decode.rs:1312-1343 -- dequantise, IDCT, level shift and clamp one component into a u8 plane of
- * (blocks_w*8) x (blocks_h*8) samples. qt: natural order. */
-static void idct_plane(const int16_t* coefs, int blocks_w, int blocks_h, const uint8_t qt[64], uint8_t* plane) {
-    float mult[8][8];
+/* jpeg/src/decode.rs:1317-1322 -- dequantisation multipliers q[v*8+u]*AAN[u]*AAN[v] in f32, in that product order. */
+static void dequant_mult(const uint8_t qt[64], float mult[8][8]) {
     for (int v = 0; v < 8; v++)
         for (int u = 0; u < 8; u++) mult[v][u] = (float)qt[v * 8 + u] * AAN[u] * AAN[v];
+}
+
+/* jpeg/src/decode.rs:1327-1337 -- one block: zigzag -> natural dequantisation, the f32 AAN IDCT and round_f32, before
+ * the level shift and the clamp. */
+static void idct_block(const int16_t* src, const float mult[8][8], int32_t out[64]) {
+    Block blk;
+    for (int k = 0; k < 64; k++) {
+        int nn = ZIGZAG[k];
+        blk[nn / 8][nn % 8] = (float)src[k] * mult[nn / 8][nn % 8];
+    }
+    orc_idct(blk);
+    for (int y = 0; y < 8; y++)
+        for (int x = 0; x < 8; x++) out[y * 8 + x] = round_f32(blk[y][x]);
+}
+
+/* jpeg/src/decode.rs:1312-1343 -- idct_planes::<T>(center, lo, max) for one component: sample = round_f32(x) + center
+ * clamped to [lo, hi], in a plane of (blocks_w*8) x (blocks_h*8) int32 samples. An 8-bit picture uses (128, 0, 255)
+ * (decode.rs:1406); the frames of a hierarchical image are not clamped, (128 or 0, i32::MIN/4, i32::MAX/4)
+ * (decode.rs:836-844). */
+void orc_idct_plane(const int16_t* coefs, int blocks_w, int blocks_h, const uint8_t qt[64], int32_t center,
+                    int32_t lo, int32_t hi, int32_t* plane) {
+    float mult[8][8];
+    dequant_mult(qt, mult);
     size_t stride = (size_t)blocks_w * 8;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int by = 0; by < blocks_h; by++)
         for (int bx = 0; bx < blocks_w; bx++) {
-            const int16_t* src = coefs + ((size_t)by * blocks_w + bx) * 64;
-            Block blk;
-            for (int k = 0; k < 64; k++) {
-                int nn = ZIGZAG[k];
-                blk[nn / 8][nn % 8] = (float)src[k] * mult[nn / 8][nn % 8];
-            }
-            orc_idct(blk);
+            int32_t r[64];
+            idct_block(coefs + ((size_t)by * blocks_w + bx) * 64, mult, r);
             for (int y = 0; y < 8; y++)
                 for (int x = 0; x < 8; x++) {
-                    int32_t v = round_f32(blk[y][x]) + 128;
+                    int32_t v = r[y * 8 + x] + center;
+                    plane[((size_t)by * 8 + y) * stride + (size_t)bx * 8 + x] = v < lo ? lo : (v > hi ? hi : v);
+                }
+        }
+}
+
+/* The same with the 8-bit picture's arguments (128, 0, 255), stored as u8 (T = u8, decode.rs:1406). */
+static void idct_plane(const int16_t* coefs, int blocks_w, int blocks_h, const uint8_t qt[64], uint8_t* plane) {
+    float mult[8][8];
+    dequant_mult(qt, mult);
+    size_t stride = (size_t)blocks_w * 8;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int by = 0; by < blocks_h; by++)
+        for (int bx = 0; bx < blocks_w; bx++) {
+            int32_t r[64];
+            idct_block(coefs + ((size_t)by * blocks_w + bx) * 64, mult, r);
+            for (int y = 0; y < 8; y++)
+                for (int x = 0; x < 8; x++) {
+                    int32_t v = r[y * 8 + x] + 128;
                     plane[((size_t)by * 8 + y) * stride + (size_t)bx * 8 + x] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
                 }
         }
 }
 
 /* jpeg/src/decode.rs:1769-1829 -- bring one plane to w x h ("fancy" triangle filter for 2x factors). */
-static void upsample(const uint8_t* p, size_t stride, int cw, int ch, int fx, int fy, int w, int h, uint8_t* out) {
+void orc_upsample(const uint8_t* p, size_t stride, int cw, int ch, int fx, int fy, int w, int h, uint8_t* out) {
 #define PX(xx, yy) ((int32_t)p[(size_t)(yy) * stride + (xx)])
     if (fx == 1 && fy == 1) {
         for (int y = 0; y < h; y++) memcpy(out + (size_t)y * w, p + (size_t)y * stride, (size_t)w);
@@ -409,24 +449,76 @@ static void upsample(const uint8_t* p, size_t stride, int cw, int ch, int fx, in
 #undef PX
 }
 
+/* jpeg/src/decode.rs:1427-1445 -- three full-resolution planes to interleaved RGB8: copied when the components hold
+ * R, G, B directly (Adobe transform 0, or no Adobe marker and ids "RGB"), else fixed-point YCbCr -> RGB. */
+void orc_ycc_to_rgb(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2, size_t n, int rgb_direct, uint8_t* out) {
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)n; ii++) {
+        size_t i = (size_t)ii;
+        int32_t a = p0[i], b = p1[i], c = p2[i];
+        uint8_t* px = out + i * 3;
+        if (rgb_direct) {
+            px[0] = (uint8_t)a; px[1] = (uint8_t)b; px[2] = (uint8_t)c;
+        } else {
+            int32_t cb = b - 128, cr = c - 128;
+            int32_t r = a + ((91881 * cr + 32768) >> 16);
+            int32_t g = a - ((22554 * cb + 46802 * cr + 32768) >> 16);
+            int32_t bl = a + ((116130 * cb + 32768) >> 16);
+            px[0] = (uint8_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
+            px[1] = (uint8_t)(g < 0 ? 0 : (g > 255 ? 255 : g));
+            px[2] = (uint8_t)(bl < 0 ? 0 : (bl > 255 ? 255 : bl));
+        }
+    }
+}
+
+/* jpeg/src/decode.rs:1454-1478 (cmyk_to_rgb) -- four full-resolution planes to RGB8. adobe_transform < 0: no Adobe
+ * marker, the values are ink amounts; with a marker they are stored inverted (255 = no ink), and transform 2 means
+ * the first three planes are YCbCr-coded (converted, clamped and inverted to CMY first). */
+void orc_cmyk_to_rgb(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2, const uint8_t* p3, size_t n,
+                     int adobe_transform, uint8_t* out) {
+    int inverted = adobe_transform >= 0, ycck = adobe_transform == 2;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)n; ii++) {
+        size_t i = (size_t)ii;
+        int32_t c = p0[i], m = p1[i], y = p2[i], k = p3[i];
+        if (ycck) {
+            int32_t yy = c, cb = m - 128, cr = y - 128;
+            int32_t r = yy + ((91881 * cr + 32768) >> 16);
+            int32_t g = yy - ((22554 * cb + 46802 * cr + 32768) >> 16);
+            int32_t b = yy + ((116130 * cb + 32768) >> 16);
+            r = r < 0 ? 0 : (r > 255 ? 255 : r);
+            g = g < 0 ? 0 : (g > 255 ? 255 : g);
+            b = b < 0 ? 0 : (b > 255 ? 255 : b);
+            c = 255 - r; m = 255 - g; y = 255 - b;
+        }
+        int32_t v3[3] = {c, m, y};
+        for (int j = 0; j < 3; j++) {
+            int32_t lv = inverted ? v3[j] : 255 - v3[j];
+            int32_t lk = inverted ? k : 255 - k;
+            out[i * 3 + j] = (uint8_t)((lv * lk + 127) / 255);
+        }
+    }
+}
+
 /*
  * jpeg/src/decode.rs:334-388 + 1390-1451 -- pixels a viewer would show for a coefficient image.
- * ncomp: 1 or 3. comp_h/comp_v: sampling factors; blocks_w/blocks_h: block grid of each component;
+ * ncomp: 1, 3 or 4. comp_h/comp_v: sampling factors; blocks_w/blocks_h: block grid of each component;
  * coefs[c]: zigzag coefficients; qt[c]: 64-entry natural-order table of component c.
  * rgb_direct: the components hold R,G,B directly (ids "RGB" or Adobe transform 0).
- * out: Gray8 (ncomp=1) or RGB8 (ncomp=3), w*h*ncomp bytes.
- * Returns 0, 1 (invalid) or 2 (unsupported: non-integral sampling ratio).
+ * adobe_transform: the Adobe APP14 transform, -1 without the marker (four components: cmyk_to_rgb's rule).
+ * out: Gray8 (ncomp=1) or RGB8 (ncomp=3 or 4), w*h*(ncomp == 1 ? 1 : 3) bytes.
+ * Returns 0, 1 (invalid) or 2 (unsupported: non-integral sampling ratio or another component count).
  */
 int orc_reconstruct(int w, int h, int ncomp, const int* comp_h, const int* comp_v, const int* blocks_w,
                     const int* blocks_h, const int16_t* const* coefs, const uint8_t* const* qt, int rgb_direct,
-                    uint8_t* out) {
-    if (ncomp != 1 && ncomp != 3) return 2;
+                    int adobe_transform, uint8_t* out) {
+    if (ncomp != 1 && ncomp != 3 && ncomp != 4) return 2;
     int hmax = 1, vmax = 1;
     for (int c = 0; c < ncomp; c++) {
         if (comp_h[c] > hmax) hmax = comp_h[c];
         if (comp_v[c] > vmax) vmax = comp_v[c];
     }
-    uint8_t* full[3] = {0, 0, 0};
+    uint8_t* full[4] = {0, 0, 0, 0};
     int status = 0;
     for (int c = 0; c < ncomp; c++) {
         size_t stride = (size_t)blocks_w[c] * 8;
@@ -439,34 +531,18 @@ int orc_reconstruct(int w, int h, int ncomp, const int* comp_h, const int* comp_
             break;
         }
         full[c] = (uint8_t*)malloc((size_t)w * h);
-        upsample(plane, stride, cw, ch, hmax / comp_h[c], vmax / comp_v[c], w, h, full[c]);
+        orc_upsample(plane, stride, cw, ch, hmax / comp_h[c], vmax / comp_v[c], w, h, full[c]);
         free(plane);
     }
     if (status == 0) {
-        if (ncomp == 1) {
+        if (ncomp == 1)
             memcpy(out, full[0], (size_t)w * h);
-        } else {
-#pragma omp parallel for schedule(static)
-            for (int y = 0; y < h; y++)
-                for (int x = 0; x < w; x++) {
-                    size_t i = (size_t)y * w + x;
-                    int32_t a = full[0][i], b = full[1][i], c = full[2][i];
-                    uint8_t* px = out + i * 3;
-                    if (rgb_direct) {
-                        px[0] = (uint8_t)a; px[1] = (uint8_t)b; px[2] = (uint8_t)c;
-                    } else {
-                        int32_t cb = b - 128, cr = c - 128;
-                        int32_t r = a + ((91881 * cr + 32768) >> 16);
-                        int32_t g = a - ((22554 * cb + 46802 * cr + 32768) >> 16);
-                        int32_t bl = a + ((116130 * cb + 32768) >> 16);
-                        px[0] = (uint8_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
-                        px[1] = (uint8_t)(g < 0 ? 0 : (g > 255 ? 255 : g));
-                        px[2] = (uint8_t)(bl < 0 ? 0 : (bl > 255 ? 255 : bl));
-                    }
-                }
-        }
+        else if (ncomp == 3)
+            orc_ycc_to_rgb(full[0], full[1], full[2], (size_t)w * h, rgb_direct, out);
+        else
+            orc_cmyk_to_rgb(full[0], full[1], full[2], full[3], (size_t)w * h, adobe_transform, out);
     }
-    for (int c = 0; c < 3; c++) free(full[c]);
+    for (int c = 0; c < 4; c++) free(full[c]);
     return status;
 }
 
@@ -1064,7 +1140,7 @@ static double score_candidate(const uint8_t* px, int w, int h, int channels, int
     const uint8_t* qq[3] = {qt, qt + 64, qt + 64};
     uint8_t* rec = (uint8_t*)malloc((size_t)w * h * channels);
     double s = -DBL_MAX;
-    if (orc_reconstruct(w, h, channels, ch, cv, bw, bh, cc, qq, 0, rec) == 0) {
+    if (orc_reconstruct(w, h, channels, ch, cv, bw, bh, cc, qq, 0, -1, rec) == 0) {
         double v;
         if (orc_ssimulacra2(px, rec, w, h, channels, &v, NULL, NULL) == 0) s = v;
     }

@@ -41,7 +41,15 @@ def lib():
         L.orc_build_coefs.argtypes = [P, I, I, I, I, I, P, P, P, P]
         L.orc_build_coefs_tables.argtypes = [P, I, I, I, I, I, P, P, P, P, P]
         L.orc_scale_base.argtypes = [P, I, P]
-        L.orc_reconstruct.argtypes = [I, I, I, P, P, P, P, P, P, I, P]
+        L.orc_reconstruct.argtypes = [I, I, I, P, P, P, P, P, P, I, I, P]
+        L.orc_idct_plane.argtypes = [P, I, I, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P]
+        L.orc_idct_plane.restype = None
+        L.orc_upsample.argtypes = [P, ctypes.c_size_t, I, I, I, I, I, I, P]
+        L.orc_upsample.restype = None
+        L.orc_ycc_to_rgb.argtypes = [P, P, P, ctypes.c_size_t, I, P]
+        L.orc_ycc_to_rgb.restype = None
+        L.orc_cmyk_to_rgb.argtypes = [P, P, P, P, ctypes.c_size_t, I, P]
+        L.orc_cmyk_to_rgb.restype = None
         L.orc_ssimulacra2.argtypes = [P, P, I, I, I, P, P, P]
         L.orc_find_quality.argtypes = [P, I, I, I, D, I, I, P, P, P]
         L.orc_bisect_table.argtypes = [P, D, I, P, P]
@@ -229,6 +237,9 @@ def requantize(ci, quality, trellis=False, adobe=-1):
 
 
 def reconstruct(ci):
+    """jpeg/src/decode.rs:334: Gray8 (h, w) for one component, RGB8 (h, w, 3) for three or four. The colour rule
+    follows finish (decode.rs:1423-1445): components stored as RGB when the Adobe transform is 0, or when there is no
+    Adobe marker and the ids are "RGB"; four components go through cmyk_to_rgb (decode.rs:1454)."""
     n = len(ci.comps)
     ch = np.array([c.h for c in ci.comps], np.int32)
     cv = np.array([c.v for c in ci.comps], np.int32)
@@ -239,12 +250,50 @@ def reconstruct(ci):
     cp = (ctypes.c_void_p * n)(*[c.ctypes.data for c in coefs])
     qp = (ctypes.c_void_p * n)(*[q.ctypes.data for q in qts])
     ids = bytes(c.id for c in ci.comps)
-    rgb_direct = 1 if ids == b"RGB" else 0
-    out = np.zeros((ci.height, ci.width) if n == 1 else (ci.height, ci.width, n), np.uint8)
+    adobe = getattr(ci, "adobe_transform", -1)
+    adobe = -1 if adobe is None else int(adobe)
+    rgb_direct = 1 if adobe == 0 or (adobe < 0 and ids == b"RGB") else 0
+    out = np.zeros((ci.height, ci.width) if n == 1 else (ci.height, ci.width, 3), np.uint8)
     rc = lib().orc_reconstruct(ci.width, ci.height, n, _ptr(ch), _ptr(cv), _ptr(bw), _ptr(bh),
-                               ctypes.cast(cp, ctypes.c_void_p), ctypes.cast(qp, ctypes.c_void_p), rgb_direct, _ptr(out))
+                               ctypes.cast(cp, ctypes.c_void_p), ctypes.cast(qp, ctypes.c_void_p), rgb_direct, adobe,
+                               _ptr(out))
     if rc != 0:
         raise ValueError(f"orc_reconstruct failed: {rc}")
+    return out
+
+
+def idct_plane(coefs, blocks_w, blocks_h, qtable, center, lo, hi):
+    """jpeg/src/decode.rs:1312-1343 idct_planes::<i32>(center, lo, max) for one component: int32 samples
+    round_f32(idct(dequantised block)) + center clamped to [lo, hi], shape (blocks_h*8, blocks_w*8)."""
+    c = np.ascontiguousarray(coefs, dtype=np.int16)
+    q = np.ascontiguousarray(qtable, dtype=np.uint8)
+    out = np.zeros((blocks_h * 8, blocks_w * 8), np.int32)
+    lib().orc_idct_plane(_ptr(c), blocks_w, blocks_h, _ptr(q), center, lo, hi, _ptr(out))
+    return out
+
+
+def upsample(plane, cw, ch, fx, fy, w, h):
+    """jpeg/src/decode.rs:1769-1829: a u8 plane (rows of any stride >= cw) brought to w x h."""
+    p = np.ascontiguousarray(plane, dtype=np.uint8)
+    out = np.zeros((h, w), np.uint8)
+    lib().orc_upsample(_ptr(p), p.shape[1], cw, ch, fx, fy, w, h, _ptr(out))
+    return out
+
+
+def ycc_to_rgb(y, cb, cr, rgb_direct=False):
+    """jpeg/src/decode.rs:1427-1445 (and samples_to_image, :1680-1697, for 8 bits): three (h, w) u8 planes to RGB8."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint8) for p in (y, cb, cr)]
+    out = np.zeros(planes[0].shape + (3,), np.uint8)
+    lib().orc_ycc_to_rgb(_ptr(planes[0]), _ptr(planes[1]), _ptr(planes[2]), planes[0].size, 1 if rgb_direct else 0,
+                         _ptr(out))
+    return out
+
+
+def cmyk_to_rgb(c, m, y, k, adobe_transform=-1):
+    """jpeg/src/decode.rs:1454-1478: four (h, w) u8 planes to RGB8."""
+    planes = [np.ascontiguousarray(p, dtype=np.uint8) for p in (c, m, y, k)]
+    out = np.zeros(planes[0].shape + (3,), np.uint8)
+    lib().orc_cmyk_to_rgb(*[_ptr(p) for p in planes], planes[0].size, adobe_transform, _ptr(out))
     return out
 
 

@@ -15,6 +15,10 @@ for name in sorted(os.listdir(os.path.join(SRC, "ssim2"))):
         shutil.copyfile(os.path.join(SRC, "ssim2", name), os.path.join(DST, "ssim2", name))
 for name in ("hier_dct_huffman.jpg", "hier_gray_dct_then_lossless.jpg", "hier_gray_refine_dct.jpg"):
     shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, "hier", name))
+# The reference decoder's recorded output for two of them (jpeg/examples/make_hier.rs:66-96, checked sample for sample
+# by jpeg/tests/hierarchical.rs:260-291): they pin reconstruct's IDCT, upsampling and colour conversion.
+for name in ("hier_dct_huffman.expected.png", "hier_gray_refine_dct.expected.png"):
+    shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, "hier", name))
 # Pictures of jpeg/tests/jpeg.rs:26-59 (PSNR floors) and baseline files of jpeg/tests/jpeg.rs:121-130 (67x45: odd
 # chroma sizes), decoded independently by libjpeg through Pillow in the tests.
 for name in ("rgb8.png", "gray8.png", "rgb8_420.jpg", "rgb8_444.jpg", "gray8.jpg"):

@@ -4,6 +4,8 @@
   frame_part(write_jpeg(build_coefs(picture, q85, trellis off))) (jpeg/src/hierarchical.rs:203-225, 329-339), so the
   bytes from its DQT to just before the next frame's marker are exactly what write_jpeg emits from DQT to EOI.
   The coefficients come from the oracle's build_coefs on the file's own tables (pinned by test_oracle.py).
+* Progressive: frame 0 of hier_dct_progressive.jpg, byte for byte; the reference wrote it as
+  frame_part(write_progressive(build_coefs(...))) (jpeg/src/hierarchical.rs:218-220).
 * Baseline and progressive rewrites of the reference's JPEG fixtures are lossless: an independent decoder
   (libjpeg through Pillow) gives the same pixels as for the original file, and our baseline Huffman reader gets
   the same coefficients back (jpeg/tests/jpeg_decode.rs:74-104 checks the same with the reference's decoders).
@@ -100,6 +102,41 @@ def test_baseline_writer_reproduces_hierarchical_frame0_bytes(name, make, subs, 
     assert _frame_part(got) == want
     # The file around the frame: SOI, the JFIF segment, EOI.
     assert got[:20] == b"\xff\xd8\xff\xe0\x00\x10JFIF\x00\x01\x01\x00\x00\x01\x00\x01\x00\x00"
+
+
+def _dqt_tables_before_first_frame(data):
+    """The natural-order tables of the DQT segments that precede a file's first SOF marker (any SOF type)."""
+    tabs, pos = {}, 2
+    while data[pos] == 0xFF and not (0xC0 <= data[pos + 1] <= 0xCF and data[pos + 1] not in (0xC4, 0xC8, 0xCC)):
+        length = data[pos + 2] << 8 | data[pos + 3]
+        if data[pos + 1] == 0xDB:
+            b = data[pos + 4: pos + 2 + length]
+            while b:
+                assert b[0] >> 4 == 0
+                t = np.zeros(64, np.uint8)
+                for k in range(64):
+                    t[jpeg_scan.ZIGZAG[k]] = b[1 + k]
+                tabs[b[0] & 15] = t
+                b = b[65:]
+        pos += 2 + length
+    return tabs
+
+
+def test_progressive_writer_reproduces_hierarchical_frame0_bytes():
+    # hier_dct_progressive.jpg's first frame is frame_part(progressive::write_progressive(build_coefs(shrink^2(
+    # picture(64, 48, Rgb)), q85, S420, trellis off))) (jpeg/src/hierarchical.rs:209-225, jpeg/examples/
+    # make_hier.rs:86-96): the progressive writer's scan plan and entropy bytes, byte for byte.
+    data = _read("hier_dct_progressive.jpg")
+    want = _frame0(data)
+    assert any(m == 0xC2 for _, m in _markers(want))
+    tabs = _dqt_tables_before_first_frame(data)
+    img = hier_inputs.shrink(hier_inputs.shrink(hier_inputs.picture(64, 48, 3)))
+    ci = O.build_coefs(img, 85, O.S420, tables=(tabs[0], tabs.get(1)))
+    got = rc.write_jpeg(_as_rc(ci), rc.WRITE_PROGRESSIVE)
+    assert _frame_part(got) == want
+    # write_best picks the smaller encoding; for this frame that decision is between two pinned byte strings.
+    best = rc.write_jpeg(_as_rc(ci), rc.WRITE_BEST)
+    assert best in (got, rc.write_jpeg(_as_rc(ci), rc.WRITE_BASELINE))
 
 
 def test_a_planted_coefficient_change_breaks_the_byte_match():
