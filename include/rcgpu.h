@@ -97,7 +97,7 @@ typedef struct {
 } rcg_coef_component;
 typedef struct {
     uint32_t width, height;
-    uint32_t ncomp;                 /* 1 or 3 (4-component CMYK/YCCK is RCG_UNSUPPORTED) */
+    uint32_t ncomp;                 /* 1, 3 or 4 (CMYK/YCCK) */
     rcg_coef_component comps[4];
     const uint8_t* qtables[4];      /* natural-order tables by slot, NULL if absent */
     int adobe_transform;            /* -1 when the file has no Adobe marker */
@@ -105,9 +105,11 @@ typedef struct {
 
 /*
  * Replaces rc_jpeg::decode::reconstruct(&CoefImage) -> Result<Image> (jpeg/src/decode.rs:334): dequantise, f32
- * AAN IDCT, round and clamp, "fancy" upsampling of subsampled components, fixed-point YCbCr->RGB. pixels receives
- * width*height*ncomp bytes. RCG_INVALID for a missing table; RCG_UNSUPPORTED for a non-integral sampling ratio
- * (decode.rs:1314,1410-1412). Bit-exact with the reference.
+ * AAN IDCT, round and clamp, "fancy" upsampling of subsampled components, fixed-point YCbCr->RGB (or a copy for
+ * RGB-coded components), and for four components cmyk_to_rgb with the Adobe transform (decode.rs:1446-1478). pixels
+ * receives width*height bytes for one component (Gray8) and width*height*3 (RGB8) for three or four. RCG_INVALID for a
+ * missing table; RCG_UNSUPPORTED for a non-integral sampling ratio (decode.rs:1314,1410-1412). Bit-exact with the
+ * reference.
  */
 int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels);
 int rcg_reconstruct_dev(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pixels);
@@ -330,15 +332,17 @@ int rcg_requantize(const rcg_coef_image* ci, int quality, int trellis, rcg_requa
  * requantize(ci, q, trellis), metric = SSIMULACRA2, bisection as rcg_find_quality. The source coefficients are
  * uploaded once; the reference image and its SSIMULACRA2 state are built once; each step requantises and
  * reconstructs on the device (the requantisation fused into the IDCT) or, with trellis, requantises on the host.
- * winner (may be NULL) receives requantize(ci, chosen quality, trellis). 1- or 3-component images whose chroma
- * planes share one geometry with sampling ratios of 1 or 2; otherwise RCG_UNSUPPORTED. */
+ * winner (may be NULL) receives requantize(ci, chosen quality, trellis). Any coefficient image reconstruct takes
+ * (1, 3 or 4 components, integral sampling ratios): YCbCr and gray images with luma at full resolution and ratios 1 or
+ * 2 are scored straight from the IDCT's planes; the others (CMYK/YCCK, RGB-coded, other ratios) through reconstruct's
+ * colour stage. */
 int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, double target, int max_quality, int trellis,
                              rcg_found* out, rcg_requantized* winner);
 
 /* Device bytes that one search call takes from its context (the arena it sizes, descriptors included), for
  * admission against an HBM budget. mode: RCG_SEARCH_PLAIN (rcg_find_quality), RCG_SEARCH_TRELLIS
- * (rcg_find_quality_trellis), RCG_SEARCH_REQUANT (rcg_requant_find_quality over a source of that size; an upper
- * bound, the 4:4:4 block grid). No counterpart in the reference: it feeds the CLI's replacement of
+ * (rcg_find_quality_trellis), RCG_SEARCH_REQUANT (rcg_requant_find_quality over a source of that size with
+ * `channels` components, 1, 3 or 4; an upper bound over the sampling layouts). No counterpart in the reference: it feeds the CLI's replacement of
  * cli/src/budget.rs:117-119 (estimate_bytes) with device footprints. */
 typedef enum { RCG_SEARCH_PLAIN = 0, RCG_SEARCH_TRELLIS = 1, RCG_SEARCH_REQUANT = 2 } rcg_search_mode;
 int rcg_search_bytes(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling, int mode,

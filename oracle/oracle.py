@@ -106,6 +106,7 @@ class CoefImage:
     height: int
     comps: list
     qtables: list  # natural-order uint8[64] per slot, or None
+    adobe_transform: int = -1  # the Adobe APP14 transform, -1 without the marker (encode.rs:82)
 
 
 def _channels(img):
@@ -211,8 +212,12 @@ def trellis(ci, scaled):
     return outs
 
 
-def requantize(ci, quality, trellis=False, adobe=-1):
-    """jpeg/src/encode.rs:107-171: CoefImage::requantize(quality, trellis); returns a new CoefImage."""
+def requantize(ci, quality, trellis=False, adobe=None):
+    """jpeg/src/encode.rs:107-171: CoefImage::requantize(quality, trellis); returns a new CoefImage that keeps the
+    Adobe transform. adobe: the source's Adobe transform (-1: none); by default the one `ci` carries."""
+    if adobe is None:
+        adobe = getattr(ci, "adobe_transform", -1)
+        adobe = -1 if adobe is None else int(adobe)
     n = len(ci.comps)
     i32 = lambda v: np.array(v, np.int32)
     tq = i32([c.tq for c in ci.comps])
@@ -233,7 +238,7 @@ def requantize(ci, quality, trellis=False, adobe=-1):
                          1 if trellis else 0, ctypes.cast(op, ctypes.c_void_p), _ptr(tabs), _ptr(present),
                          _ptr(new_tq))
     comps = [CoefComponent(c.id, c.h, c.v, int(t), c.blocks_w, o) for c, t, o in zip(ci.comps, new_tq, outs)]
-    return CoefImage(ci.width, ci.height, comps, [tabs[s].copy() if present[s] else None for s in range(4)])
+    return CoefImage(ci.width, ci.height, comps, [tabs[s].copy() if present[s] else None for s in range(4)], adobe)
 
 
 def reconstruct(ci):

@@ -340,7 +340,7 @@ class Context:
         keep = []
         c_ci = _coef_image_struct(ci, keep, device=False)
         n = len(ci.comps)
-        out = np.empty((ci.height, ci.width) if n == 1 else (ci.height, ci.width, n), np.uint8)
+        out = np.empty((ci.height, ci.width) if n == 1 else (ci.height, ci.width, 3), np.uint8)
         _check(lib().rcg_reconstruct(self._h, ctypes.byref(c_ci), _ptr(out)))
         return out
 

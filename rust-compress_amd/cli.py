@@ -509,8 +509,7 @@ class Pipeline:
             if target is not None:
                 if info.width < 8 or info.height < 8:
                     raise _fail("unsupported", "--target-ssimulacra2 needs an image of at least 8x8 pixels")
-                need = self.gpu.footprint(info.width, info.height, 1 if len(ci.comps) == 1 else 3, AUTO,
-                                          SEARCH_REQUANT)
+                need = self.gpu.footprint(info.width, info.height, len(ci.comps), AUTO, SEARCH_REQUANT)
                 found = self.gpu.run(need, lambda ctx: ctx.requant_find_quality(ci, target, 100, self.trellis))
                 return write_jpeg(found.coefs, WRITE_BEST), f"q{found.quality}, ssimulacra2 {found.score:.4f}", False
             before = [None if t is None else t.copy() for t in ci.qtables]
@@ -525,8 +524,6 @@ class Pipeline:
         if info.lossless or info.hierarchical or info.precision != 8:
             raise _fail("unsupported", "decoding this JPEG frame type is not part of this build")
         ci = read_jpeg(data, info)
-        if len(ci.comps) == 4:
-            raise _fail("unsupported", "CMYK/YCCK sources through pixels are not part of this build")
         w, h = info.width, info.height
         need = self.gpu.footprint(w, h, 1 if len(ci.comps) == 1 else 3, AUTO, SEARCH_PLAIN)
         img = self.gpu.run(need, lambda ctx: ctx.reconstruct(ci))

@@ -679,20 +679,19 @@ int rcg_build_scaled(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32
 
 }  // extern "C"
 
-namespace {
-
-// Validate a coefficient image and derive its sample planes (jpeg/src/decode.rs:1314,1404-1415).
-int check_coef_image(const rcg_coef_image* ci, uint32_t& hmax, uint32_t& vmax, int& rgb_direct) {
+// Validate a coefficient image and derive its sample planes (jpeg/src/decode.rs:1314,1404-1415). rgb_direct: three
+// components stored as R, G, B (Adobe transform 0, or no Adobe marker and ids "RGB"; decode.rs:1423-1426).
+int rcg::check_coef_image(const rcg_coef_image* ci, uint32_t& hmax, uint32_t& vmax, int& rgb_direct) {
     if (!ci || ci->width == 0 || ci->height == 0) return RCG_INVALID;
-    if (ci->ncomp != 1 && ci->ncomp != 3) {
-        set_error("only 1- and 3-component images are reconstructed");
+    if (ci->ncomp != 1 && ci->ncomp != 3 && ci->ncomp != 4) {
+        set_error("only 1-, 3- and 4-component images are reconstructed");
         return RCG_UNSUPPORTED;
     }
     hmax = 1;
     vmax = 1;
     for (uint32_t c = 0; c < ci->ncomp; c++) {
         const rcg_coef_component& k = ci->comps[c];
-        if (k.h == 0 || k.v == 0 || k.tq > 3 || !k.coefs) return RCG_INVALID;
+        if (k.h == 0 || k.v == 0 || k.tq > 3 || !k.coefs || k.blocks_w == 0 || k.blocks_h == 0) return RCG_INVALID;
         hmax = std::max<uint32_t>(hmax, k.h);
         vmax = std::max<uint32_t>(vmax, k.v);
     }
@@ -712,28 +711,32 @@ int check_coef_image(const rcg_coef_image* ci, uint32_t& hmax, uint32_t& vmax, i
             return RCG_INVALID;
         }
     }
-    rgb_direct = ci->adobe_transform == 0 ||
-                 (ci->adobe_transform < 0 && ci->ncomp == 3 && ci->comps[0].id == 'R' && ci->comps[1].id == 'G' &&
-                  ci->comps[2].id == 'B');
+    rgb_direct = ci->ncomp == 3 &&
+                 (ci->adobe_transform == 0 || (ci->adobe_transform < 0 && ci->comps[0].id == 'R' &&
+                                               ci->comps[1].id == 'G' && ci->comps[2].id == 'B'));
     return RCG_OK;
 }
 
-void mult_table(const uint8_t q[64], float mult[64]) {
+// Bytes per pixel of reconstruct's output: Gray8 for one component, RGB8 otherwise (decode.rs:1416-1451).
+uint32_t rcg::out_channels(const rcg_coef_image* ci) { return ci->ncomp == 1 ? 1 : 3; }
+
+void rcg::mult_table(const uint8_t q[64], float mult[64]) {
     for (int v = 0; v < 8; v++)
         for (int u = 0; u < 8; u++) mult[v * 8 + u] = float(q[v * 8 + u]) * kAan[u] * kAan[v];
 }
 
-// IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device.
-int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[3], uint8_t* d_out,
-                          uint32_t hmax, uint32_t vmax, int rgb_direct) {
-    IdctPlane planes[3] = {};
-    PlaneView views[3];
+// IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device. The sample
+// planes go to `planes` (one per component, blocks_w*8 x blocks_h*8 bytes each).
+int rcg::reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4],
+                          uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct) {
+    IdctPlane planes[4] = {};
+    PlaneView views[4];
     for (uint32_t c = 0; c < ci->ncomp; c++) {
         const rcg_coef_component& k = ci->comps[c];
         planes[c].coefs = d_coefs[c];
         planes[c].blocks_w = k.blocks_w;
         planes[c].blocks_h = k.blocks_h;
-        planes[c].plane = static_cast<uint8_t*>(ctx->arena.take(size_t(k.blocks_w) * 8 * k.blocks_h * 8));
+        planes[c].plane = planes_out[c];
         mult_table(ci->qtables[k.tq], planes[c].mult);
         views[c].p = planes[c].plane;
         views[c].stride = k.blocks_w * 8;
@@ -743,10 +746,17 @@ int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t*
         views[c].fy = vmax / k.v;
     }
     RCG_TRY(launch_idct(ctx, planes, int(ci->ncomp)));
-    return launch_upsample_color(ctx, views, int(ci->ncomp), rgb_direct, ci->width, ci->height, d_out);
+    return launch_upsample_color(ctx, views, int(ci->ncomp), rgb_direct, ci->width, ci->height, d_out,
+                                 ci->adobe_transform < 0 ? -1 : ci->adobe_transform);
 }
 
-}  // namespace
+// Sample-plane bytes of a coefficient image (what reconstruct_on_device writes before the colour stage).
+size_t rcg::sample_plane_bytes(const rcg_coef_image* ci) {
+    size_t b = 0;
+    for (uint32_t c = 0; c < ci->ncomp; c++)
+        b += Arena::align(size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64);
+    return b;
+}
 
 extern "C" {
 
@@ -755,23 +765,23 @@ int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels) {
     uint32_t hmax, vmax;
     int rgb_direct;
     RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
-    size_t need = Arena::align(size_t(ci->width) * ci->height * ci->ncomp);
-    for (uint32_t c = 0; c < ci->ncomp; c++) {
-        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
-        need += Arena::align(nb * 64 * sizeof(int16_t)) + Arena::align(nb * 64);
-    }
+    const size_t out_bytes = size_t(ci->width) * ci->height * out_channels(ci);
+    size_t need = Arena::align(out_bytes) + sample_plane_bytes(ci);
+    for (uint32_t c = 0; c < ci->ncomp; c++)
+        need += Arena::align(size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64 * sizeof(int16_t));
     Call call(ctx);
     RCG_TRY(call.begin(need));
-    const int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
     for (uint32_t c = 0; c < ci->ncomp; c++) {
-        const size_t n = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64 * sizeof(int16_t);
-        int16_t* d = static_cast<int16_t*>(ctx->arena.take(n));
-        RCG_HIP(hipMemcpyAsync(d, ci->comps[c].coefs, n, hipMemcpyHostToDevice, ctx->stream));
+        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+        int16_t* d = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
+        RCG_HIP(hipMemcpyAsync(d, ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice, ctx->stream));
         d_co[c] = d;
+        planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
     }
-    const size_t out_bytes = size_t(ci->width) * ci->height * ci->ncomp;
     uint8_t* d_out = static_cast<uint8_t*>(ctx->arena.take(out_bytes));
-    RCG_TRY(reconstruct_on_device(ctx, ci, d_co, d_out, hmax, vmax, rgb_direct));
+    RCG_TRY(reconstruct_on_device(ctx, ci, d_co, planes, d_out, hmax, vmax, rgb_direct));
     RCG_HIP(hipMemcpyAsync(pixels, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
     RCG_HIP(hipStreamSynchronize(ctx->stream));
     return RCG_OK;
@@ -782,19 +792,20 @@ int rcg_reconstruct_dev(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pix
     uint32_t hmax, vmax;
     int rgb_direct;
     RCG_TRY(check_coef_image(d_ci, hmax, vmax, rgb_direct));
-    size_t need = 0;
-    const int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+    const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
     for (uint32_t c = 0; c < d_ci->ncomp; c++) {
         if (reinterpret_cast<uintptr_t>(d_ci->comps[c].coefs) & 15) {
             set_error("coefficient buffers must be 16-byte aligned device pointers");
             return RCG_INVALID;
         }
-        need += Arena::align(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64);
         d_co[c] = d_ci->comps[c].coefs;
     }
     Call call(ctx);
-    RCG_TRY(call.begin(need));
-    RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, d_pixels, hmax, vmax, rgb_direct));
+    RCG_TRY(call.begin(sample_plane_bytes(d_ci)));
+    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c < d_ci->ncomp; c++)
+        planes[c] = static_cast<uint8_t*>(ctx->arena.take(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64));
+    RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
     RCG_HIP(hipStreamSynchronize(ctx->stream));
     return RCG_OK;
 }
@@ -1185,10 +1196,13 @@ namespace {
 
 // Device bytes of the requantisation search over the block grid L: per component the source and candidate
 // coefficients and sample planes, the decoded reference image, and the FP32 and FP64 SSIMULACRA2 state.
+// Device bytes of one requantisation search: the decoded source, the FP32 and FP64 scoring state, per component the
+// source and trellis coefficients and two sets of sample planes, and the RGB8 candidate of the pixel route (counted
+// always: an upper bound).
 size_t requant_need(rcg_ctx* ctx, const Layout& L) {
     const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
     const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
-    size_t need = Arena::align(size_t(L.w) * L.h * L.channels) + ssim_bytes(P32, RCG_PREC_FP32) +
+    size_t need = 2 * Arena::align(size_t(L.w) * L.h * L.channels) + ssim_bytes(P32, RCG_PREC_FP32) +
                   ssim_bytes(P64, RCG_PREC_FP64);
     for (uint32_t c = 0; c < L.ncomp; c++) {
         const size_t nb = size_t(L.bw[c]) * L.bh[c];
@@ -1197,15 +1211,29 @@ size_t requant_need(rcg_ctx* ctx, const Layout& L) {
     return need;
 }
 
+// Where a requantisation search reads its candidates from.
+//   fused: the SSIMULACRA2 loader upsamples and colour-converts the IDCT's sample planes as it reads them (luma at
+//          full resolution, chroma planes of one geometry, ratios 1 or 2, YCbCr or gray);
+//   pixel: any other layout reconstruct accepts (four components, RGB-coded components, other sampling ratios):
+//          the IDCT's planes go through the reconstruct colour stage to an RGB8 candidate, which is scored as pixels.
+struct RequantRoute {
+    bool fused;
+    uint8_t* d_cand;  // pixel route: w*h*3 bytes
+    PlaneView views[4];
+    int rgb_direct;
+    int adobe_transform;
+};
+
 // One requantised candidate scored against the decoded source: its sample planes (requantised on the device inside
 // the IDCT, or uploaded after the host trellis), then SSIMULACRA2 with the FP64 re-score inside the guard band.
-int score_requant(rcg_ctx* ctx, Call& call, const rcg_coef_image* ci, const Layout& L, const int16_t* const d_src[3],
-                  const uint8_t* d_ref, uint32_t ref_ch, uint8_t* const planes[3], int16_t* const d_tr[3],
-                  const std::vector<std::vector<int16_t>>& host_src, int quality, bool trellis, const SsimBuffers& B32,
-                  const SsimBuffers& B64, bool& a32_ready, bool& a64_ready, double target, double& score) {
+int score_requant(rcg_ctx* ctx, Call& call, const rcg_coef_image* ci, const Layout& L, const int16_t* const d_src[4],
+                  const uint8_t* d_ref, uint32_t ref_ch, uint8_t* const planes[4], int16_t* const d_tr[4],
+                  const std::vector<std::vector<int16_t>>& host_src, int quality, bool trellis,
+                  const RequantRoute& route, const SsimBuffers& B32, const SsimBuffers& B64, bool& a32_ready,
+                  bool& a64_ready, double target, double& score) {
     RequantPlan P;
     RCG_TRY(requant_plan(ci, quality, P));
-    IdctPlane ip[3] = {};
+    IdctPlane ip[4] = {};
     for (uint32_t c = 0; c < ci->ncomp; c++) {
         const rcg_coef_component& k = ci->comps[c];
         ip[c].blocks_w = k.blocks_w;
@@ -1238,13 +1266,16 @@ int score_requant(rcg_ctx* ctx, Call& call, const rcg_coef_image* ci, const Layo
         }
     }
     uint8_t* rec[3] = {planes[0], planes[1], planes[2]};
+    const SsimSrc cand = route.fused ? recon_src(L, rec) : bytes_src(route.d_cand, 3);
     const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
     call.D.off = 0;
-    StagedSsim st = stage_ssim(call.D, P32, bytes_src(d_ref, ref_ch), recon_src(L, rec), B32, a32_ready,
-                               RCG_PREC_FP32);
+    StagedSsim st = stage_ssim(call.D, P32, bytes_src(d_ref, ref_ch), cand, B32, a32_ready, RCG_PREC_FP32);
     RCG_TRY(call.D.upload(ctx->stream));
     if (st.build_ref) RCG_TRY(fork_side(ctx));  // the reference state beside the IDCT, on the first step
     RCG_TRY(launch_idct(ctx, ip, int(ci->ncomp)));
+    if (!route.fused)
+        RCG_TRY(launch_upsample_color(ctx, route.views, int(ci->ncomp), route.rgb_direct, L.w, L.h, route.d_cand,
+                                      route.adobe_transform));
     RCG_TRY(launch_staged_refs(ctx, st));
     RCG_TRY(launch_staged_ssim(ctx, st));
     double s = 0.0;
@@ -1254,8 +1285,7 @@ int score_requant(rcg_ctx* ctx, Call& call, const rcg_coef_image* ci, const Layo
     if (std::isfinite(s) && std::fabs(s - target) < kGuard) {
         call.D.off = 0;
         const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
-        StagedSsim s64 = stage_ssim(call.D, P64, bytes_src(d_ref, ref_ch), recon_src(L, rec), B64, a64_ready,
-                                    RCG_PREC_FP64);
+        StagedSsim s64 = stage_ssim(call.D, P64, bytes_src(d_ref, ref_ch), cand, B64, a64_ready, RCG_PREC_FP64);
         RCG_TRY(call.D.upload(ctx->stream));
         RCG_TRY(launch_staged_ssim(ctx, s64));
         RCG_HIP(hipMemcpyAsync(&s, B64.d_score, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1274,26 +1304,25 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
     uint32_t hmax, vmax;
     int rgb_direct;
     RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
-    // The scorer reads the reconstruction through the fused upsampling loader: luma at full resolution, chroma
-    // planes of one geometry, ratios 1 or 2, YCbCr colour.
-    const rcg_coef_component* k = ci->comps;
-    bool ok = !rgb_direct && k[0].h == hmax && k[0].v == vmax && hmax <= 2 && vmax <= 2;
-    if (ci->ncomp == 3)
-        ok = ok && k[1].h == 1 && k[1].v == 1 && k[2].h == 1 && k[2].v == 1 && k[1].blocks_w == k[2].blocks_w &&
-             k[1].blocks_h == k[2].blocks_h;
-    if (!ok) {
-        set_error("requantisation search: this sampling layout is not supported");
-        return RCG_UNSUPPORTED;
-    }
     if (ci->width < 8 || ci->height < 8) {
         set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
         return RCG_UNSUPPORTED;
     }
+    const rcg_coef_component* k = ci->comps;
+    RequantRoute route;
+    std::memset(&route, 0, sizeof(route));
+    route.fused = (ci->ncomp == 1 || ci->ncomp == 3) && !rgb_direct && k[0].h == hmax && k[0].v == vmax &&
+                  hmax <= 2 && vmax <= 2;
+    if (ci->ncomp == 3)
+        route.fused = route.fused && k[1].h == 1 && k[1].v == 1 && k[2].h == 1 && k[2].v == 1 &&
+                      k[1].blocks_w == k[2].blocks_w && k[1].blocks_h == k[2].blocks_h;
+    route.rgb_direct = rgb_direct;
+    route.adobe_transform = ci->adobe_transform < 0 ? -1 : ci->adobe_transform;
     Layout L;
     std::memset(&L, 0, sizeof(L));
     L.w = ci->width;
     L.h = ci->height;
-    L.channels = ci->ncomp == 1 ? 1 : 3;
+    L.channels = out_channels(ci);
     L.hmax = hmax;
     L.vmax = vmax;
     L.ncomp = ci->ncomp;
@@ -1312,10 +1341,10 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
     }
     Call call(ctx);
     RCG_TRY(call.begin(need));
-    int16_t* d_src[3] = {nullptr, nullptr, nullptr};
-    int16_t* d_tr[3] = {nullptr, nullptr, nullptr};
-    uint8_t* planes[3] = {nullptr, nullptr, nullptr};
-    uint8_t* src_planes[3] = {nullptr, nullptr, nullptr};
+    int16_t* d_src[4] = {nullptr, nullptr, nullptr, nullptr};
+    int16_t* d_tr[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t* src_planes[4] = {nullptr, nullptr, nullptr, nullptr};
     for (uint32_t c = 0; c < ci->ncomp; c++) {
         const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
         d_src[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
@@ -1324,38 +1353,27 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
         src_planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
         RCG_HIP(hipMemcpyAsync(d_src[c], ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice,
                                ctx->stream));
+        const rcg_coef_component& kc = ci->comps[c];
+        route.views[c].p = planes[c];
+        route.views[c].stride = kc.blocks_w * 8;
+        route.views[c].cw = (ci->width * kc.h + hmax - 1) / hmax;
+        route.views[c].ch = (ci->height * kc.v + vmax - 1) / vmax;
+        route.views[c].fx = hmax / kc.h;
+        route.views[c].fy = vmax / kc.v;
     }
     uint8_t* d_ref = static_cast<uint8_t*>(ctx->arena.take(ref_bytes));
+    if (!route.fused) route.d_cand = static_cast<uint8_t*>(ctx->arena.take(size_t(L.w) * L.h * 3));
     const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
     const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
     // reference = reconstruct(&ci) (pipeline.rs:373), decoded once on the device.
-    {
-        IdctPlane ip[3] = {};
-        PlaneView views[3];
-        for (uint32_t c = 0; c < ci->ncomp; c++) {
-            const rcg_coef_component& kc = ci->comps[c];
-            ip[c].coefs = d_src[c];
-            ip[c].blocks_w = kc.blocks_w;
-            ip[c].blocks_h = kc.blocks_h;
-            ip[c].plane = src_planes[c];
-            mult_table(ci->qtables[kc.tq], ip[c].mult);
-            views[c].p = src_planes[c];
-            views[c].stride = kc.blocks_w * 8;
-            views[c].cw = (ci->width * kc.h + hmax - 1) / hmax;
-            views[c].ch = (ci->height * kc.v + vmax - 1) / vmax;
-            views[c].fx = hmax / kc.h;
-            views[c].fy = vmax / kc.v;
-        }
-        RCG_TRY(launch_idct(ctx, ip, int(ci->ncomp)));
-        RCG_TRY(launch_upsample_color(ctx, views, int(ci->ncomp), 0, ci->width, ci->height, d_ref));
-    }
+    RCG_TRY(reconstruct_on_device(ctx, ci, d_src, src_planes, d_ref, hmax, vmax, rgb_direct));
     bool a32 = false, a64 = false;
     // jpeg/src/target.rs:37-55
     int lo = 1, hi = std::max(1, std::min(100, max_quality));
     int steps = 0;
     double s = 0.0;
-    RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, hi, trellis != 0, B32,
-                          B64, a32, a64, target, s));
+    RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, hi, trellis != 0,
+                          route, B32, B64, a32, a64, target, s));
     steps++;
     int best_q = hi;
     double best_s = s;
@@ -1363,7 +1381,7 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
         while (lo < hi) {
             const int mid = lo + (hi - lo) / 2;
             RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, mid,
-                                  trellis != 0, B32, B64, a32, a64, target, s));
+                                  trellis != 0, route, B32, B64, a32, a64, target, s));
             steps++;
             if (s >= target) {
                 hi = mid;
@@ -1384,6 +1402,18 @@ extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, 
 extern "C" int rcg_search_bytes(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling,
                                 int mode, size_t* out) {
     if (!ctx || !out) return RCG_INVALID;
+    if (mode == RCG_SEARCH_REQUANT && channels == 4) {
+        // A four-component source: every component's grid is at most the 4:2:0 luma grid of its size.
+        RCG_TRY(check_image(width, height, 3));
+        Layout L = max_layout(width, height, 3, RCG_AUTO);
+        L.ncomp = 4;
+        for (int c = 1; c < 4; c++) {
+            L.bw[c] = L.bw[0];
+            L.bh[c] = L.bh[0];
+        }
+        *out = requant_need(ctx, L) + kDescBytes + 4096;
+        return RCG_OK;
+    }
     RCG_TRY(check_image(width, height, channels));
     size_t need = 0;
     switch (mode) {

@@ -6,7 +6,7 @@
 //                    without a coefficient round trip through HBM). jpeg/src/encode.rs:426-539, decode.rs:1312-1343.
 //   k_idct           zigzag i16 coefficients of any CoefImage component -> u8 sample plane (decode.rs:1312-1343).
 //   k_upsample_color sample planes -> full-size Gray8 or RGB8 ("fancy" triangle upsampling, fixed-point
-//                    YCbCr->RGB; decode.rs:1404-1445, 1769-1829).
+//                    YCbCr->RGB, or cmyk_to_rgb for four components; decode.rs:1404-1478, 1769-1829).
 //
 // Layout: one encode workgroup owns a 512-pixel-wide, one-MCU-tall strip. Its RGB rows are read as aligned 16-byte
 // chunks into LDS (all of a thread's loads in flight at once), converted there by a per-layout routine, and each of
@@ -399,8 +399,9 @@ __global__ __launch_bounds__(256) void k_idct(IdctArgs args) {
 }
 
 struct UpArgs {
-    PlaneView comp[3];
+    PlaneView comp[4];
     uint32_t ncomp, rgb_direct;
+    int adobe_transform;     // four components: -1 without an Adobe marker
     uint32_t w, h;
     uint8_t* out;
 };
@@ -417,6 +418,17 @@ __global__ __launch_bounds__(256) void k_upsample_color(UpArgs a) {
         return;
     }
     uint8_t* o = a.out + uint64_t(y) * a.w * 3;
+    if (a.ncomp == 4) {
+        for (uint32_t x = xs; x < xe; x++) {
+            int32_t r, g, b;
+            cmyk_to_rgb(upsample_at(a.comp[0], x, y), upsample_at(a.comp[1], x, y), upsample_at(a.comp[2], x, y),
+                        upsample_at(a.comp[3], x, y), a.adobe_transform, r, g, b);
+            o[x * 3] = uint8_t(r);
+            o[x * 3 + 1] = uint8_t(g);
+            o[x * 3 + 2] = uint8_t(b);
+        }
+        return;
+    }
     for (uint32_t x = xs; x < xe; x++) {
         const int32_t p0 = upsample_at(a.comp[0], x, y);
         const int32_t p1 = upsample_at(a.comp[1], x, y);
@@ -445,6 +457,10 @@ int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, 
 }
 
 int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n) {
+    if (n < 1 || n > 4) {
+        set_error("internal: the IDCT launch takes 1 to 4 components");
+        return RCG_DEVICE;
+    }
     IdctArgs a;
     std::memset(&a, 0, sizeof(a));
     uint32_t maxb = 0;
@@ -467,12 +483,17 @@ int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n) {
 }
 
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
-                          uint8_t* out) {
+                          uint8_t* out, int adobe_transform) {
+    if (ncomp != 1 && ncomp != 3 && ncomp != 4) {
+        set_error("internal: pixels are made from 1, 3 or 4 components");
+        return RCG_DEVICE;
+    }
     UpArgs a;
     std::memset(&a, 0, sizeof(a));
     for (int i = 0; i < ncomp; i++) a.comp[i] = comps[i];
     a.ncomp = ncomp;
     a.rgb_direct = rgb_direct;
+    a.adobe_transform = adobe_transform;
     a.w = w;
     a.h = h;
     a.out = out;

@@ -112,7 +112,7 @@ struct Layout {
     uint32_t w, h, channels;
     uint32_t hmax, vmax, pw, ph;
     uint32_t ncomp;
-    uint32_t bw[3], bh[3];
+    uint32_t bw[4], bh[4];  // per component (a fourth only for CMYK/YCCK coefficient images)
     size_t coef_count(int c) const { return size_t(bw[c]) * bh[c] * 64; }
     size_t plane_bytes(int c) const { return size_t(bw[c]) * 8 * bh[c] * 8; }
 };
@@ -147,8 +147,10 @@ struct IdctPlane {
     int16_t* requant_out;
 };
 int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n);
+// Full-size pixels from sample planes: Gray8 for one component, RGB8 for three (YCbCr, or copied when rgb_direct)
+// and for four (cmyk_to_rgb with the Adobe transform, -1 without the marker).
 int launch_upsample_color(rcg_ctx* ctx, const PlaneView* comps, int ncomp, int rgb_direct, uint32_t w, uint32_t h,
-                          uint8_t* out);
+                          uint8_t* out, int adobe_transform = -1);
 int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
                       uint32_t kind_a, uint32_t kind_b, hipStream_t stream = nullptr);  // null: the context's stream
 int launch_ssim_ref(rcg_ctx* ctx, const RefJob* d_jobs, const Unit* d_units, uint32_t nunits, int nt,
@@ -197,6 +199,18 @@ struct RequantPlan {
     bool changed;           // some component's table changed
 };
 int requant_plan(const rcg_coef_image* ci, int quality, RequantPlan& P);
+
+// reconstruct (jpeg/src/decode.rs:334, 1390-1478) of a coefficient image whose coefficients are on the device.
+// check_coef_image validates it (RCG_INVALID / RCG_UNSUPPORTED as the reference's errors) and derives the sampling
+// maxima and the RGB-coded flag; out_channels is 1 for gray, else 3; sample_plane_bytes is what the IDCT's planes
+// take (256-byte aligned per component); reconstruct_on_device enqueues the IDCT into planes_out and the colour stage
+// into d_out on the context's stream.
+int check_coef_image(const rcg_coef_image* ci, uint32_t& hmax, uint32_t& vmax, int& rgb_direct);
+uint32_t out_channels(const rcg_coef_image* ci);
+size_t sample_plane_bytes(const rcg_coef_image* ci);
+void mult_table(const uint8_t q[64], float mult[64]);
+int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4],
+                          uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct);
 
 // Tiles of the SSIMULACRA2 scale kernel for a w x h scale.
 inline uint32_t ssim_tiles_x(uint32_t w, int nt) { return (w + (nt - 12) - 1) / (nt - 12); }

@@ -49,4 +49,23 @@ __device__ __forceinline__ void ycc_to_rgb(int32_t a, int32_t b, int32_t c, int3
     bl = bl < 0 ? 0 : (bl > 255 ? 255 : bl);
 }
 
+// Four components to RGB (cmyk_to_rgb, decode.rs:1454-1478): with an Adobe marker the values are stored inverted
+// (255 = no ink) and transform 2 means the first three are YCbCr-coded; without a marker they are ink amounts. The
+// light an ink lets through is multiplied by the black's, (v*k + 127) / 255.
+__device__ __forceinline__ void cmyk_to_rgb(int32_t c, int32_t m, int32_t y, int32_t k, int adobe_transform,
+                                            int32_t& r, int32_t& g, int32_t& b) {
+    if (adobe_transform == 2) {
+        int32_t rr, gg, bb;
+        ycc_to_rgb(c, m, y, rr, gg, bb);
+        c = 255 - rr;
+        m = 255 - gg;
+        y = 255 - bb;
+    }
+    const bool inverted = adobe_transform >= 0;
+    const int32_t lk = inverted ? k : 255 - k;
+    r = ((inverted ? c : 255 - c) * lk + 127) / 255;
+    g = ((inverted ? m : 255 - m) * lk + 127) / 255;
+    b = ((inverted ? y : 255 - y) * lk + 127) / 255;
+}
+
 }  // namespace rcg

@@ -24,7 +24,9 @@ for name in ("hier_dct_huffman.expected.png", "hier_gray_refine_dct.expected.png
 for name in ("rgb8.png", "gray8.png", "rgb8_420.jpg", "rgb8_444.jpg", "gray8.jpg"):
     shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, name))
 # Four-component baseline files of cli/tests/cmyk.rs:38-54 (requantisation at q100 changes no coefficient).
-for name in ("cmyk.jpg", "ycck.jpg", "ycck_444.jpg"):
+# Pillow's RGB conversions of them, the reference's decode tolerance target (cli/tests/cmyk.rs:14-27,58-72).
+for name in ("cmyk.jpg", "ycck.jpg", "ycck_444.jpg", "cmyk_expected.png", "ycck_expected.png",
+             "ycck_444_expected.png"):
     shutil.copyfile(os.path.join(SRC, name), os.path.join(DST, name))
 # JPEG sources of the reader (jpeg/tests/jpeg_decode.rs:6,77,126; jpeg/tests/arith.rs:168): baseline, progressive
 # (jpegtran-style scan script with successive approximation) and optimised-table files, plus one each of the frame

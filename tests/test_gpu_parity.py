@@ -546,3 +546,55 @@ def test_extreme_aspect_ratios(ctx, rc, w, h):
         rec = ctx.reconstruct(dev)
         np.testing.assert_array_equal(rec, O.reconstruct(ora))
         assert abs(ctx.ssimulacra2(img, rec, rc.FP32) - O.ssimulacra2(img, rec)) <= TOL32
+
+
+# ------------------------------------------------------------------------------------ four-component sources
+
+CMYK = ["cmyk.jpg", "ycck.jpg", "ycck_444.jpg"]
+
+
+def _read_gold(name):
+    with open(os.path.join(GOLD, name), "rb") as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("name", CMYK)
+def test_four_component_reconstruct_is_bit_exact(ctx, rc, name):
+    # reconstruct of CMYK (Adobe transform 0: inverted ink) and YCCK (Adobe 2, 4:2:0-style and 4:4:4) coefficient
+    # images: IDCT of four planes, upsampling, cmyk_to_rgb (jpeg/src/decode.rs:1446-1478).
+    ci = rc.read_jpeg(_read_gold(name))
+    assert len(ci.comps) == 4
+    got = ctx.reconstruct(ci)
+    assert got.shape == (ci.height, ci.width, 3)
+    np.testing.assert_array_equal(got, O.reconstruct(ci))
+    # The same planes with no Adobe marker (plain ink amounts) take the other branch of cmyk_to_rgb.
+    ci.adobe_transform = -1
+    np.testing.assert_array_equal(ctx.reconstruct(ci), O.reconstruct(ci))
+
+
+@pytest.mark.parametrize("name,target,trellis", [("cmyk.jpg", 70.0, False), ("ycck.jpg", 60.0, True),
+                                                 ("ycck_444.jpg", 75.0, False), ("ycck.jpg", 80.0, False)])
+def test_four_component_requant_search_chooses_the_oracles_quality(ctx, rc, name, target, trellis):
+    # cli/src/pipeline.rs:369-377 searches any non-resampled JPEG source, four-component ones included: candidates
+    # are requantised, reconstructed through cmyk_to_rgb and scored as pixels.
+    ci = rc.read_jpeg(_read_gold(name))
+    found = ctx.requant_find_quality(ci, target, 100, trellis)
+    want, scores = _oracle_requant_search(ci, target, trellis)
+    assert found.quality == want, (found.quality, want, sorted(scores.items()))
+    assert abs(found.score - scores[want]) <= TOL32
+    ref = O.requantize(ci, want, trellis)
+    for a, b in zip(found.coefs.comps, ref.comps):
+        np.testing.assert_array_equal(a.coefs, b.coefs)
+
+
+def test_rgb_coded_requant_search_takes_the_pixel_route(ctx, rc):
+    # Three components stored as R, G, B (Adobe transform 0, decode.rs:1423-1426): no colour conversion, which the
+    # fused loader does not do, so the candidates are scored as pixels.
+    img = rc.synth_rgb8(77, 120, 90)
+    ci = O.build_coefs(img, 92, O.S444)
+    src = rc.CoefImage(ci.width, ci.height, [rc.CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, c.coefs)
+                                             for c in ci.comps], list(ci.qtables) + [None] * 2, 0)
+    np.testing.assert_array_equal(ctx.reconstruct(src), O.reconstruct(src))
+    found = ctx.requant_find_quality(src, 70.0, 100, False)
+    want, scores = _oracle_requant_search(src, 70.0, False)
+    assert found.quality == want, (found.quality, want, sorted(scores.items()))

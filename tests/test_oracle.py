@@ -259,6 +259,38 @@ def test_reconstruct_agrees_with_libjpeg_on_reference_files(name):
     assert d.max() <= 3 and d.mean() < 0.35, (d.max(), d.mean())
 
 
+def _adobe_transform(data):
+    """The transform byte of a file's Adobe APP14 segment (T.872 6.5.3), or -1 without one."""
+    pos = 2
+    while pos + 4 <= len(data) and data[pos] == 0xFF and data[pos + 1] != 0xDA:
+        length = data[pos + 2] << 8 | data[pos + 3]
+        body = data[pos + 4: pos + 2 + length]
+        if data[pos + 1] == 0xEE and body[:5] == b"Adobe" and len(body) >= 12:
+            return body[11]
+        pos += 2 + length
+    return -1
+
+
+@pytest.mark.parametrize("name,expected", [("cmyk.jpg", "cmyk_expected.png"), ("ycck.jpg", "ycck_expected.png"),
+                                           ("ycck_444.jpg", "ycck_444_expected.png")])
+def test_four_component_reconstruct_matches_the_reference_fixtures(name, expected):
+    # cli/tests/cmyk.rs:14-27,58-72: the expected PNGs are Pillow's (libjpeg's) RGB conversion of the files, and the
+    # reference's decode must be within a mean of 2 levels of them. cmyk_to_rgb (decode.rs:1454-1478) with the
+    # file's Adobe transform (0: inverted CMYK, 2: YCCK).
+    with open(os.path.join(GOLD, name), "rb") as f:
+        data = f.read()
+    ci = _coefimage_from_frame(jpeg_scan.first_frame(data))
+    ci.adobe_transform = _adobe_transform(data)
+    assert ci.adobe_transform in (0, 2) and len(ci.comps) == 4
+    ours = O.reconstruct(ci).astype(np.int32)
+    want = np.asarray(Image.open(os.path.join(GOLD, expected)).convert("RGB")).astype(np.int32)
+    d = np.abs(ours - want)
+    assert d.mean() < 2.0 and d.max() <= 4, (d.mean(), d.max())
+    # A plausible slip -- the ink amounts read without the inversion an Adobe marker means -- is far off.
+    ci.adobe_transform = -1
+    assert np.abs(O.reconstruct(ci).astype(np.int32) - want).mean() > 20
+
+
 @pytest.mark.parametrize("w,h", [(67, 45), (203, 77), (33, 31), (17, 9)])
 @pytest.mark.parametrize("subs", [O.S444, O.S422, O.S420])
 def test_reconstruct_agrees_with_libjpeg_on_oracle_coefficients(w, h, subs):
