@@ -314,6 +314,20 @@ int rcg_find_quality_trellis(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width
                              int16_t* coefs[3], uint8_t qtables[128]);
 
 /* ------------------------------------------------------------------------------------------------------------
+ * Allocation-free reconstruct. No counterpart in the reference: this is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that forbid hipMalloc). Same arguments, results
+ * and errors as rcg_reconstruct_dev, and the same kernels, but the IDCT's sample planes live in a static device array
+ * of RCG_NOALLOC_SCRATCH_BYTES per GPU (every GPU the library runs on holds its copy until the process ends) instead of
+ * the context's growing workspace, and the call allocates nothing: no hipMalloc/hipFree, no profiling events. The
+ * context must have been created (rcg_create, which allocates, and looks up the array's device address). An image
+ * whose sample planes (per component blocks_w*blocks_h*64 bytes, rounded up to 256) exceed the array is refused with
+ * RCG_UNSUPPORTED before anything is enqueued: about 44 MP at 4:2:0, 22 MP at 4:4:4. Calls on one GPU share its
+ * array, so overlapping calls would race; the library serialises them per GPU. The array keeps the last call's
+ * planes. Synchronous, as rcg_reconstruct_dev. */
+#define RCG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int rcg_reconstruct_dev_noalloc(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pixels);
+
+/* ------------------------------------------------------------------------------------------------------------
  * DCT-domain requantisation (JPEG -> JPEG). */
 typedef struct {
     int16_t* coefs[4];          /* caller buffers, one per component, as large as the source component's */

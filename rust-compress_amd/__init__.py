@@ -140,6 +140,7 @@ def lib():
         L.rcg_build_coefs_dev.argtypes = [P, P, U, U, U, I, I, P]
         L.rcg_reconstruct.argtypes = [P, ctypes.POINTER(_CoefImage), P]
         L.rcg_reconstruct_dev.argtypes = [P, ctypes.POINTER(_CoefImage), P]
+        L.rcg_reconstruct_dev_noalloc.argtypes = [P, ctypes.POINTER(_CoefImage), P]
         L.rcg_ssimulacra2.argtypes = [P, P, P, U, U, U, I, ctypes.POINTER(D), P, ctypes.POINTER(I)]
         L.rcg_ssimulacra2_dev.argtypes = [P, P, P, U, U, U, I, ctypes.POINTER(D), P, ctypes.POINTER(I)]
         L.rcg_find_quality.argtypes = [P, P, U, U, U, D, I, I, ctypes.POINTER(_Found), P, P]

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librcgpu.so")
-SOURCES = ["api.cpp", "batch.cpp", "server.cpp", "jpeg.hip", "ssim2.hip", "synth.cpp", "writer.cpp", "reader.cpp"]
+SOURCES = ["api.cpp", "batch.cpp", "server.cpp", "jpeg.hip", "ssim2.hip", "noalloc.hip", "synth.cpp", "writer.cpp",
+           "reader.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RCG_OFFLOAD_ARCH", "gfx950")
 

@@ -556,6 +556,7 @@ int rcg_create(int device, rcg_ctx** out) {
     c->tune64.ref_slots_wide = c->tune32.ref_slots_wide;
     c->tune64.ref_slots_narrow = c->tune32.ref_slots_narrow;
     rc = upload_weights();
+    if (rc == RCG_OK) rc = noalloc_setup(device);
     if (rc != RCG_OK) {
         rcg_destroy(c);
         return rc;

@@ -53,7 +53,7 @@ void Prof::collect() {
 }
 
 ProfScope::ProfScope(rcg_ctx* c, const char* n, hipStream_t s) : ctx(c), name(n), stream(s ? s : c->stream) {
-    if (!ctx->prof.on) return;
+    if (!ctx->prof.on || ctx->prof_suspended) return;
     hipEvent_t a = ctx->prof.get();
     b = ctx->prof.get();
     if (a) (void)hipEventRecord(a, stream);

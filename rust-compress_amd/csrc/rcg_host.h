@@ -101,6 +101,7 @@ struct rcg_ctx {
     float* d_lut = nullptr;  // sRGB -> linear table, 256 f32
     rcg::SsimTune tune32, tune64;
     rcg::Prof prof;
+    bool prof_suspended = false;  // set during a call that must not create profiling events
     double taps64[6];
     float taps32[6];
 };
@@ -211,6 +212,9 @@ size_t sample_plane_bytes(const rcg_coef_image* ci);
 void mult_table(const uint8_t q[64], float mult[64]);
 int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4],
                           uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct);
+// Set-up of the allocation-free entry points on `device` (current): finds the device address of the static scratch
+// array once per device. Called by rcg_create.
+int noalloc_setup(int device);
 
 // Tiles of the SSIMULACRA2 scale kernel for a w x h scale.
 inline uint32_t ssim_tiles_x(uint32_t w, int nt) { return (w + (nt - 12) - 1) / (nt - 12); }

@@ -69,3 +69,22 @@ def test_synthetic_generator_is_deterministic_and_shaped(rc):
     # the split).
     t = rc.synth_rgb8(42, 300, 17)
     assert t.shape == (17, 300, 3)
+
+
+def test_no_alloc_reconstruct_checks_sizes_on_the_host(rc):
+    # rcg_reconstruct_dev_noalloc refuses an image whose sample planes exceed its static scratch before it touches the
+    # context or the device: no context is passed here, and no GPU call is made.
+    L = rc.lib()
+    s = rc._CoefImage()
+    s.width, s.height, s.ncomp = 16384, 8192, 1
+    c = s.comps[0]
+    c.id, c.h, c.v, c.tq, c.blocks_w, c.blocks_h, c.coefs = 1, 1, 1, 0, 2048, 1024, 0x1000  # never dereferenced
+    q = np.ones(64, np.uint8)
+    s.qtables[0] = q.ctypes.data
+    s.adobe_transform = -1
+    assert L.rcg_reconstruct_dev_noalloc(None, ctypes.byref(s), 0x2000) == rc.UNSUPPORTED
+    assert b"allocation-free scratch" in L.rcg_last_error()
+    # Just under the limit (64 MiB of planes exactly) the size is accepted, and the missing context is the error.
+    c.blocks_w, c.blocks_h = 1024, 1024
+    s.width, s.height = 8192, 8192
+    assert L.rcg_reconstruct_dev_noalloc(None, ctypes.byref(s), 0x2000) == rc.INVALID
