@@ -160,6 +160,11 @@ int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, 
  * first, then bisection over list indices; chosen_* report that choice and sweep_* the exhaustive one (the lowest
  * passing candidate). Candidates whose FP32 score lies within 0.05 of the target are re-scored in FP64 first.
  * Pixels are DEVICE pointers for rcg_score_batch_dev and HOST pointers for rcg_score_batch.
+ * winner (optional, winner[0] NULL when not wanted) receives the chosen candidate's coefficients, Found.coefs of
+ * jpeg/src/target.rs:11-16 (build_coefs at the chosen quality and subsampling, coefficients as rcg_build_coefs):
+ * DEVICE buffers, 16-byte aligned, for rcg_score_batch_dev and plans, HOST buffers for rcg_score_batch. winner[c]
+ * must hold component c's grid for every candidate of the job (rcg_coef_layout); winner[1..2] are unused for Gray.
+ * The chosen candidate is encoded once more after the selection, in one launch for every job that asked.
  * This entry point has no single counterpart in the reference: it replaces the serial find_quality_with loop of
  * cli/src/pipeline.rs:207-247 run once per file by cli/src/main.rs:57-75.
  */
@@ -171,6 +176,7 @@ typedef struct {
     int quality[RCG_MAX_CAND];
     int subsampling[RCG_MAX_CAND];
     double target;
+    int16_t* winner[3];
 } rcg_batch_job;
 typedef struct {
     double scores[RCG_MAX_CAND];    /* per candidate, FP64-settled near the target */
@@ -178,6 +184,7 @@ typedef struct {
     int chosen_index, chosen_quality;
     int sweep_index, sweep_quality;
     int status;                     /* rcg_status of this job */
+    int winner_written;             /* 1 when the job's winner buffers hold the chosen candidate's coefficients */
 } rcg_batch_result;
 int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
 int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results);
@@ -225,9 +232,11 @@ int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out);
  * GPU analogue of cli/src/budget.rs:11-50: a request that alone exceeds the budget runs, but by itself. Every tick
  * scores the next candidate(s) of every admitted request's bisection (jpeg/src/target.rs:37-55) in one batched
  * launch sequence; requests that arrive meanwhile join at the next tick. Sources and reference pyramids stay
- * resident in HBM for the life of a request; a finished request's device blocks are kept for reuse by later ones (up
- * to hbm_budget bytes of them), so a steady stream of requests allocates nothing. Each result equals the one the
- * request gets alone.
+ * resident in HBM for the life of a request; a finished request's device blocks are kept for reuse by later ones
+ * (admitted footprints and kept blocks together stay within hbm_budget), so a steady stream of requests allocates
+ * nothing. Each result equals the one the request gets alone. A tick that fails (a working set that cannot be
+ * allocated, say) is retried request by request, so only the requests that fail alone fail; an exception never leaves
+ * the worker (host allocation failure is RCG_OOM for the requests of that tick).
  * speculate = 1 scores one bisection midpoint per tick; 3 also scores both possible next midpoints, so a request
  * takes about half the ticks for 1.5x the work.
  * This entry point has no single counterpart in the reference (it replaces the per-file rayon tasks of
@@ -247,6 +256,9 @@ typedef struct {
     double target;
     int max_quality;                /* 1..100 (the CLI uses 100) */
     int subsampling;
+    /* Optional HOST buffers (coefs[0] NULL: not wanted) for the chosen quality's coefficients, Found.coefs of
+     * jpeg/src/target.rs:11-16, sized as rcg_find_quality's; they must stay valid until the completion is polled. */
+    int16_t* coefs[3];
 } rcg_request;
 typedef struct {
     uint64_t id;
@@ -256,6 +268,7 @@ typedef struct {
     int steps;                      /* candidates scored */
     int ticks;                      /* batched launches it took part in */
     double submit_s, admit_s, done_s;   /* steady-clock seconds */
+    int coefs_written;              /* 1 when the request's coefs buffers hold the chosen quality's coefficients */
 } rcg_completion;
 typedef struct {
     uint64_t ticks, requests_done;

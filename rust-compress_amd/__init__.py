@@ -77,13 +77,14 @@ class BatchJob(ctypes.Structure):
     """rcg_batch_job: one image and its candidate (quality, subsampling) list."""
     _fields_ = [("pixels", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("channels", ctypes.c_uint32), ("ncand", ctypes.c_uint32), ("quality", ctypes.c_int * MAX_CAND),
-                ("subsampling", ctypes.c_int * MAX_CAND), ("target", ctypes.c_double)]
+                ("subsampling", ctypes.c_int * MAX_CAND), ("target", ctypes.c_double),
+                ("winner", ctypes.c_void_p * 3)]
 
 
 class BatchResult(ctypes.Structure):
     _fields_ = [("scores", ctypes.c_double * MAX_CAND), ("settled", ctypes.c_int * MAX_CAND),
                 ("chosen_index", ctypes.c_int), ("chosen_quality", ctypes.c_int), ("sweep_index", ctypes.c_int),
-                ("sweep_quality", ctypes.c_int), ("status", ctypes.c_int)]
+                ("sweep_quality", ctypes.c_int), ("status", ctypes.c_int), ("winner_written", ctypes.c_int)]
 
 
 class Tune(ctypes.Structure):
@@ -104,13 +105,14 @@ class ServerOpts(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("pixels", ctypes.c_void_p), ("width", ctypes.c_uint32),
                 ("height", ctypes.c_uint32), ("channels", ctypes.c_uint32), ("target", ctypes.c_double),
-                ("max_quality", ctypes.c_int), ("subsampling", ctypes.c_int)]
+                ("max_quality", ctypes.c_int), ("subsampling", ctypes.c_int), ("coefs", ctypes.c_void_p * 3)]
 
 
 class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("quality", ctypes.c_int),
                 ("score", ctypes.c_double), ("steps", ctypes.c_int), ("ticks", ctypes.c_int),
-                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("coefs_written", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -399,10 +401,13 @@ class Context:
 
     # ------------------------------------------------------------------ batched scoring
 
-    def _batch_jobs(self, images, candidates, targets, device_ptrs):
+    def _batch_jobs(self, images, candidates, targets, device_ptrs, winners=False):
+        """The rcg_batch_job array; with winners, also per job the buffers for the winner's coefficients (numpy
+        arrays for host images, DeviceBuffers for device ones), sized for every candidate's grid."""
         n = len(images) if device_ptrs is None else len(device_ptrs)
         jobs = (BatchJob * n)()
         keep = []
+        wbufs = []
         per_image = isinstance(candidates[0], (list, tuple)) and isinstance(candidates[0][0], (list, tuple))
         for i in range(n):
             cand = candidates[i] if per_image else candidates
@@ -421,26 +426,46 @@ class Context:
                 jobs[i].quality[k] = q
                 jobs[i].subsampling[k] = sub
             jobs[i].target = float(targets[i] if isinstance(targets, (list, tuple, np.ndarray)) else targets)
-        return jobs, keep
+            if winners:
+                w, h, ch = jobs[i].width, jobs[i].height, jobs[i].channels
+                sizes = _max_grid(w, h, ch, [q for q, _ in cand], [s for _, s in cand])
+                bufs = []
+                for c, nc in enumerate(sizes):
+                    if device_ptrs is None:
+                        b = np.zeros(nc, np.int16)
+                        jobs[i].winner[c] = b.ctypes.data
+                    else:
+                        b = DeviceBuffer(self, max(nc, 1) * 2)
+                        jobs[i].winner[c] = b.ptr
+                    bufs.append(b)
+                wbufs.append(bufs)
+        return (jobs, keep, wbufs) if winners else (jobs, keep)
 
-    def batch_plan(self, device_ptrs, candidates, targets):
+    def batch_plan(self, device_ptrs, candidates, targets, winners=False):
         """Capture the batched scoring of device-resident images as a hipGraph (see BatchPlan)."""
+        if winners:
+            jobs, _, wbufs = self._batch_jobs(None, candidates, targets, device_ptrs, winners=True)
+            return BatchPlan(self, jobs, wbufs)
         jobs, _ = self._batch_jobs(None, candidates, targets, device_ptrs)
         return BatchPlan(self, jobs)
 
-    def score_batch(self, images, candidates, targets, device_ptrs=None):
+    def score_batch(self, images, candidates, targets, device_ptrs=None, winners=False):
         """Score every image against its candidate list [(quality, subsampling), ...] in one launch sequence.
 
         images: list of numpy arrays (host) or, with device_ptrs, a list of (ptr, width, height, channels).
         candidates: one list for every image, or a list of lists. Returns a list of BatchResult (scores, settled
-        flags, chosen_* = the reference's bisection over the sorted list, sweep_* = exhaustive choice)."""
-        jobs, keep = self._batch_jobs(images, candidates, targets, device_ptrs)
+        flags, chosen_* = the reference's bisection over the sorted list, sweep_* = exhaustive choice); with
+        winners, (results, [CoefImage of the chosen candidate, or None]) -- Found.coefs of jpeg/src/target.rs:11-16."""
+        out = self._batch_jobs(images, candidates, targets, device_ptrs, winners)
+        jobs, keep = out[0], out[1]
         n = len(jobs)
         res = (BatchResult * n)()
         fn = lib().rcg_score_batch if device_ptrs is None else lib().rcg_score_batch_dev
         _check(fn(self._h, jobs, n, res))
         del keep
-        return list(res)
+        if not winners:
+            return list(res)
+        return list(res), [_winner_image(jobs[i], res[i], out[2][i]) for i in range(n)]
 
     # ------------------------------------------------------------------ tile policy
 
@@ -546,13 +571,49 @@ class DeviceBuffer:
             pass
 
 
+def _max_grid(w, h, ch, qualities, subsamplings):
+    """Coefficient count of each component's largest grid over the candidates (rcg_coef_layout)."""
+    sizes = [0, 0, 0]
+    for q, s in zip(qualities, subsamplings):
+        L = coef_layout(w, h, ch, q, s)
+        for c in range(L.ncomp):
+            sizes[c] = max(sizes[c], L.blocks_w[c] * L.blocks_h[c] * 64)
+    return sizes[:1] if ch == 1 else sizes
+
+
+def _coef_image_at(w, h, ch, quality, subsampling, arrays):
+    """A CoefImage over winner buffers: build_coefs's layout and tables at (quality, subsampling)."""
+    L = coef_layout(w, h, ch, quality, subsampling)
+    qy, qc = quant_tables(quality)
+    comps = []
+    for c in range(L.ncomp):
+        n = L.blocks_w[c] * L.blocks_h[c] * 64
+        hh, vv = (L.hmax, L.vmax) if c == 0 else (1, 1)
+        comps.append(CoefComponent(c + 1, hh, vv, min(c, 1), L.blocks_w[c], np.array(arrays[c][:n], np.int16)))
+    return CoefImage(w, h, comps, [qy, qc if L.ncomp == 3 else None, None, None])
+
+
+def _winner_image(job, res, bufs):
+    if res.status != OK or not res.winner_written:
+        return None
+    arrays = []
+    for b in bufs:
+        if isinstance(b, DeviceBuffer):
+            arrays.append(b.download(np.empty(b.nbytes // 2, np.int16)))
+        else:
+            arrays.append(b)
+    k = res.chosen_index
+    return _coef_image_at(job.width, job.height, job.channels, job.quality[k], job.subsampling[k], arrays)
+
+
 class BatchPlan:
     """rcg_batch_plan: the batched launch sequence captured once as a hipGraph, replayed by run()."""
 
-    def __init__(self, ctx, jobs):
+    def __init__(self, ctx, jobs, winner_buffers=None):
         self._ctx = ctx
         self._n = len(jobs)
         self._jobs = jobs
+        self._wbufs = winner_buffers
         self._h = ctypes.c_void_p()
         _check(lib().rcg_batch_plan_create(ctx.handle, jobs, self._n, ctypes.byref(self._h)))
 
@@ -560,6 +621,10 @@ class BatchPlan:
         res = (BatchResult * self._n)()
         _check(lib().rcg_batch_plan_run(self._h, res))
         return list(res)
+
+    def winners(self, results):
+        """The chosen candidates' coefficients of the last run (plans made with winners=True)."""
+        return [_winner_image(self._jobs[i], results[i], self._wbufs[i]) for i in range(self._n)]
 
     def close(self):
         if self._h:
@@ -578,14 +643,35 @@ class Server:
 
     def __init__(self, device=0, hbm_budget=0, max_active=0, speculate=1, tune=None):
         self._h = ctypes.c_void_p()
+        self._bufs = {}
         o = ServerOpts(hbm_budget, max_active, speculate, ctypes.addressof(tune) if tune is not None else None)
         _check(lib().rcg_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
 
-    def submit(self, req_id, img, target, max_quality=100, subsampling=S420):
+    def submit(self, req_id, img, target, max_quality=100, subsampling=S420, coefs=False):
+        """Queue a search. coefs: also return the chosen quality's coefficients (winner(completion))."""
         img = np.ascontiguousarray(img, dtype=np.uint8)
         h, w = img.shape[:2]
-        r = Request(req_id, img.ctypes.data, w, h, _channels(img), float(target), max_quality, subsampling)
-        _check(lib().rcg_server_submit(self._h, ctypes.byref(r)))
+        ch = _channels(img)
+        r = Request(req_id, img.ctypes.data, w, h, ch, float(target), max_quality, subsampling)
+        if coefs:
+            qs = [100, 1] if subsampling == AUTO else [100]
+            bufs = [np.zeros(n, np.int16) for n in _max_grid(w, h, ch, qs, [subsampling] * len(qs))]
+            for c, b in enumerate(bufs):
+                r.coefs[c] = b.ctypes.data
+            self._bufs[req_id] = (bufs, w, h, ch, subsampling)
+        try:
+            _check(lib().rcg_server_submit(self._h, ctypes.byref(r)))
+        except RcgError:
+            self._bufs.pop(req_id, None)
+            raise
+
+    def winner(self, completion):
+        """The CoefImage of a completed request submitted with coefs=True (None if it failed); call once."""
+        bufs = self._bufs.pop(completion.id, None)
+        if bufs is None or completion.status != OK or not completion.coefs_written:
+            return None
+        arrays, w, h, ch, sub = bufs
+        return _coef_image_at(w, h, ch, completion.quality, sub, arrays)
 
     def poll(self, max_items=64, timeout_ms=100):
         buf = (Completion * max_items)()

@@ -402,7 +402,7 @@ class Gpu:
             ev = threading.Event()
             self._waiters[rid] = [ev, None]
         try:
-            self._server.submit(rid, img, target, 100, subsampling)
+            self._server.submit(rid, img, target, 100, subsampling, coefs=True)
         except RcgError:
             with self._lock:
                 self._waiters.pop(rid, None)
@@ -418,7 +418,7 @@ class Gpu:
                     with self._lock:
                         w = self._waiters.get(c.id)
                     if w is not None:
-                        w[1] = (c.status, c.quality, c.score)
+                        w[1] = (c.status, c.quality, c.score, self._server.winner(c))
                         w[0].set()
         except RcgError as e:
             # A failing poll ends the serving loop: every waiting search, and every later one, fails with it.
@@ -426,7 +426,7 @@ class Gpu:
                 self._poll_error = e
                 for w in self._waiters.values():
                     if w[1] is None:
-                        w[1] = (e.status, 0, 0.0)
+                        w[1] = (e.status, 0, 0.0, None)
                     w[0].set()
 
     def close(self):
@@ -473,11 +473,10 @@ class Pipeline:
         if w < 8 or h < 8:
             raise _fail("unsupported", "--target-ssimulacra2 needs an image of at least 8x8 pixels")
         if not self.trellis:
-            st, q, score = self.gpu.search(img, target, self.subsampling)
-            if st != 0:
-                raise _from_rcg(RcgError(st, "search failed"))
-            need = self.gpu.footprint(w, h, ch, self.subsampling, SEARCH_PLAIN)
-            ci = self.gpu.run(need, lambda ctx: ctx.build_coefs(img, q, self.subsampling))
+            # The serving loop returns the chosen quality's coefficients with the completion (Found.coefs).
+            st, q, score, ci = self.gpu.search(img, target, self.subsampling)
+            if st != 0 or ci is None:
+                raise _from_rcg(RcgError(st or 3, "search failed"))
             return ci, f"q{q}, ssimulacra2 {score:.4f}"
         need = self.gpu.footprint(w, h, ch, self.subsampling, SEARCH_TRELLIS)
         found = self.gpu.run(need, lambda ctx: ctx.find_quality(img, target, 100, self.subsampling, trellis=True))

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -14,9 +15,29 @@
 namespace rcg {
 
 static thread_local std::string g_last_error;
-void set_error(const std::string& msg) { g_last_error = msg; }
+static thread_local const char* g_last_fixed = nullptr;  // set when the message itself could not be stored
+void set_error(const std::string& msg) noexcept {
+    try {
+        g_last_error = msg;
+        g_last_fixed = nullptr;
+    } catch (...) {
+        g_last_fixed = "out of host memory (while recording an error)";
+    }
+}
+
+// Fault injection for tests: with RCG_TEST_ARENA_LIMIT=<bytes> in the environment, a call whose working set needs
+// more than that many bytes of workspace fails with RCG_OOM, as if the device could not hold it.
+static size_t test_arena_limit() {
+    const char* v = std::getenv("RCG_TEST_ARENA_LIMIT");
+    return v ? size_t(std::strtoull(v, nullptr, 10)) : 0;
+}
 
 int Arena::ensure(size_t bytes) {
+    const size_t limit = test_arena_limit();
+    if (limit && bytes > limit) {
+        set_error("device workspace of " + std::to_string(bytes) + " bytes: above RCG_TEST_ARENA_LIMIT");
+        return RCG_OOM;
+    }
     if (bytes <= cap) return RCG_OK;
     if (base) (void)hipFree(base);
     base = nullptr;
@@ -509,173 +530,187 @@ int ssim_pair_dev(rcg_ctx* ctx, Call& call, const uint8_t* d_a, const uint8_t* d
 
 extern "C" {
 
-const char* rcg_last_error(void) { return g_last_error.c_str(); }
+const char* rcg_last_error(void) { return g_last_fixed ? g_last_fixed : g_last_error.c_str(); }
 
 const char* rcg_version(void) { return "rcgpu 0.1 gfx950"; }
 
 int rcg_create(int device, rcg_ctx** out) {
-    if (!out) return RCG_INVALID;
-    *out = nullptr;
-    int n = 0;
-    RCG_HIP(hipGetDeviceCount(&n));
-    if (device < 0 || device >= n) {
-        set_error("no such HIP device");
-        return RCG_INVALID;
-    }
-    RCG_HIP(hipSetDevice(device));
-    rcg_ctx* c = new (std::nothrow) rcg_ctx();
-    if (!c) return RCG_OOM;
-    c->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        set_error(std::string("stream/event creation: ") + hipGetErrorString(e));
-        rcg_destroy(c);
-        return RCG_DEVICE;
-    }
-    float lut[256];
-    make_lut(lut);
-    e = hipMalloc(reinterpret_cast<void**>(&c->d_lut), sizeof(lut));
-    if (e == hipSuccess) e = hipMemcpy(c->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        set_error(std::string("lookup table: ") + hipGetErrorString(e));
-        rcg_destroy(c);
-        return RCG_DEVICE;
-    }
-    make_taps(c->taps64);
-    for (int i = 0; i < 6; i++) c->taps32[i] = float(c->taps64[i]);
-    c->tune64.fixed_nt = 128;
-    int rc = ref_kernel_slots(device, &c->tune32.ref_slots_wide, &c->tune32.ref_slots_narrow);
-    if (rc != RCG_OK) {
-        rcg_destroy(c);
-        return rc;
-    }
-    c->tune64.ref_slots_wide = c->tune32.ref_slots_wide;
-    c->tune64.ref_slots_narrow = c->tune32.ref_slots_narrow;
-    rc = upload_weights();
-    if (rc == RCG_OK) rc = noalloc_setup(device);
-    if (rc != RCG_OK) {
-        rcg_destroy(c);
-        return rc;
-    }
-    *out = c;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!out) return RCG_INVALID;
+        *out = nullptr;
+        int n = 0;
+        RCG_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) {
+            set_error("no such HIP device");
+            return RCG_INVALID;
+        }
+        RCG_HIP(hipSetDevice(device));
+        rcg_ctx* c = new (std::nothrow) rcg_ctx();
+        if (!c) return RCG_OOM;
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            set_error(std::string("stream/event creation: ") + hipGetErrorString(e));
+            rcg_destroy(c);
+            return RCG_DEVICE;
+        }
+        float lut[256];
+        make_lut(lut);
+        e = hipMalloc(reinterpret_cast<void**>(&c->d_lut), sizeof(lut));
+        if (e == hipSuccess) e = hipMemcpy(c->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error(std::string("lookup table: ") + hipGetErrorString(e));
+            rcg_destroy(c);
+            return RCG_DEVICE;
+        }
+        make_taps(c->taps64);
+        for (int i = 0; i < 6; i++) c->taps32[i] = float(c->taps64[i]);
+        c->tune64.fixed_nt = 128;
+        int rc = ref_kernel_slots(device, &c->tune32.ref_slots_wide, &c->tune32.ref_slots_narrow);
+        if (rc != RCG_OK) {
+            rcg_destroy(c);
+            return rc;
+        }
+        c->tune64.ref_slots_wide = c->tune32.ref_slots_wide;
+        c->tune64.ref_slots_narrow = c->tune32.ref_slots_narrow;
+        rc = upload_weights();
+        if (rc == RCG_OK) rc = noalloc_setup(device);
+        if (rc != RCG_OK) {
+            rcg_destroy(c);
+            return rc;
+        }
+        *out = c;
+        return RCG_OK;
+    });
 }
 
 void rcg_destroy(rcg_ctx* ctx) {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
-    if (ctx->arena.base) (void)hipFree(ctx->arena.base);
-    if (ctx->pinned.base) (void)hipHostFree(ctx->pinned.base);
-    if (ctx->d_lut) (void)hipFree(ctx->d_lut);
-    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-    if (ctx->ev_mid) (void)hipEventDestroy(ctx->ev_mid);
-    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-    if (ctx->side) (void)hipStreamDestroy(ctx->side);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    rcg::guarded_void([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+        if (ctx->arena.base) (void)hipFree(ctx->arena.base);
+        if (ctx->pinned.base) (void)hipHostFree(ctx->pinned.base);
+        if (ctx->d_lut) (void)hipFree(ctx->d_lut);
+        if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+        if (ctx->ev_mid) (void)hipEventDestroy(ctx->ev_mid);
+        if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
 }
 
 int rcg_coef_layout(uint32_t width, uint32_t height, uint32_t channels, int quality, int subsampling,
                     rcg_layout* out) {
-    if (!out) return RCG_INVALID;
-    RCG_TRY(check_image(width, height, channels));
-    const Layout L = make_layout(width, height, channels, quality, subsampling);
-    out->width = width;
-    out->height = height;
-    out->ncomp = L.ncomp;
-    out->hmax = L.hmax;
-    out->vmax = L.vmax;
-    for (int c = 0; c < 3; c++) {
-        out->blocks_w[c] = L.bw[c];
-        out->blocks_h[c] = L.bh[c];
-    }
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!out) return RCG_INVALID;
+        RCG_TRY(check_image(width, height, channels));
+        const Layout L = make_layout(width, height, channels, quality, subsampling);
+        out->width = width;
+        out->height = height;
+        out->ncomp = L.ncomp;
+        out->hmax = L.hmax;
+        out->vmax = L.vmax;
+        for (int c = 0; c < 3; c++) {
+            out->blocks_w[c] = L.bw[c];
+            out->blocks_h[c] = L.bh[c];
+        }
+        return RCG_OK;
+    });
 }
 
 int rcg_quant_tables(int quality, uint8_t out[128]) {
-    if (!out) return RCG_INVALID;
-    quant_tables(quality, out, out + 64);
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!out) return RCG_INVALID;
+        quant_tables(quality, out, out + 64);
+        return RCG_OK;
+    });
 }
 
 int rcg_build_coefs_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, uint32_t height, uint32_t channels,
                         int quality, int subsampling, int16_t* d_coefs[3]) {
-    if (!ctx || !d_pixels || !d_coefs) return RCG_INVALID;
-    RCG_TRY(check_image(width, height, channels));
-    const Layout L = make_layout(width, height, channels, quality, subsampling);
-    for (uint32_t c = 0; c < L.ncomp; c++)
-        if (!d_coefs[c] || (reinterpret_cast<uintptr_t>(d_coefs[c]) & 15)) {
-            set_error("coefficient buffers must be 16-byte aligned device pointers");
-            return RCG_INVALID;
-        }
-    Call call(ctx);
-    RCG_TRY(call.begin(0));
-    int16_t* co[3] = {d_coefs[0], L.ncomp == 3 ? d_coefs[1] : nullptr, L.ncomp == 3 ? d_coefs[2] : nullptr};
-    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_pixels, quality, co, nullptr));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !d_pixels || !d_coefs) return RCG_INVALID;
+        RCG_TRY(check_image(width, height, channels));
+        const Layout L = make_layout(width, height, channels, quality, subsampling);
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            if (!d_coefs[c] || (reinterpret_cast<uintptr_t>(d_coefs[c]) & 15)) {
+                set_error("coefficient buffers must be 16-byte aligned device pointers");
+                return RCG_INVALID;
+            }
+        Call call(ctx);
+        RCG_TRY(call.begin(0));
+        int16_t* co[3] = {d_coefs[0], L.ncomp == 3 ? d_coefs[1] : nullptr, L.ncomp == 3 ? d_coefs[2] : nullptr};
+        RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_pixels, quality, co, nullptr));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }
 
 int rcg_build_coefs(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
                     int quality, int subsampling, int16_t* coefs[3], uint8_t qtables[128]) {
-    if (!ctx || !pixels || !coefs) return RCG_INVALID;
-    RCG_TRY(check_image(width, height, channels));
-    const Layout L = make_layout(width, height, channels, quality, subsampling);
-    for (uint32_t c = 0; c < L.ncomp; c++)
-        if (!coefs[c]) return RCG_INVALID;
-    const size_t src_bytes = size_t(width) * height * channels;
-    Call call(ctx);
-    RCG_TRY(call.begin(Arena::align(src_bytes) + coef_bytes(L)));
-    uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
-    int16_t* d_co[3] = {nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < L.ncomp; c++)
-        d_co[c] = static_cast<int16_t*>(ctx->arena.take(L.coef_count(c) * sizeof(int16_t)));
-    RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
-    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, d_co, nullptr));
-    for (uint32_t c = 0; c < L.ncomp; c++)
-        RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    if (qtables) quant_tables(quality, qtables, qtables + 64);
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !pixels || !coefs) return RCG_INVALID;
+        RCG_TRY(check_image(width, height, channels));
+        const Layout L = make_layout(width, height, channels, quality, subsampling);
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            if (!coefs[c]) return RCG_INVALID;
+        const size_t src_bytes = size_t(width) * height * channels;
+        Call call(ctx);
+        RCG_TRY(call.begin(Arena::align(src_bytes) + coef_bytes(L)));
+        uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+        int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            d_co[c] = static_cast<int16_t*>(ctx->arena.take(L.coef_count(c) * sizeof(int16_t)));
+        RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+        RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, d_co, nullptr));
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        if (qtables) quant_tables(quality, qtables, qtables + 64);
+        return RCG_OK;
+    });
 }
 
 int rcg_build_scaled(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
                      int quality, int subsampling, int16_t* coefs[3], float* scaled[3], uint8_t qtables[128]) {
-    if (!ctx || !pixels || !coefs || !scaled) return RCG_INVALID;
-    RCG_TRY(check_image(width, height, channels));
-    const Layout L = make_layout(width, height, channels, quality, subsampling);
-    for (uint32_t c = 0; c < L.ncomp; c++)
-        if (!coefs[c] || !scaled[c]) return RCG_INVALID;
-    const size_t src_bytes = size_t(width) * height * channels;
-    size_t sc_bytes = 0;
-    for (uint32_t c = 0; c < L.ncomp; c++) sc_bytes += Arena::align(L.coef_count(c) * sizeof(float));
-    Call call(ctx);
-    RCG_TRY(call.begin(Arena::align(src_bytes) + coef_bytes(L) + sc_bytes));
-    uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
-    int16_t* d_co[3] = {nullptr, nullptr, nullptr};
-    float* d_sc[3] = {nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < L.ncomp; c++) {
-        d_co[c] = static_cast<int16_t*>(ctx->arena.take(L.coef_count(c) * sizeof(int16_t)));
-        d_sc[c] = static_cast<float*>(ctx->arena.take(L.coef_count(c) * sizeof(float)));
-    }
-    RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
-    RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, d_co, nullptr, d_sc));
-    for (uint32_t c = 0; c < L.ncomp; c++) {
-        RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        RCG_HIP(hipMemcpyAsync(scaled[c], d_sc[c], L.coef_count(c) * sizeof(float), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    }
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    if (qtables) quant_tables(quality, qtables, qtables + 64);
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !pixels || !coefs || !scaled) return RCG_INVALID;
+        RCG_TRY(check_image(width, height, channels));
+        const Layout L = make_layout(width, height, channels, quality, subsampling);
+        for (uint32_t c = 0; c < L.ncomp; c++)
+            if (!coefs[c] || !scaled[c]) return RCG_INVALID;
+        const size_t src_bytes = size_t(width) * height * channels;
+        size_t sc_bytes = 0;
+        for (uint32_t c = 0; c < L.ncomp; c++) sc_bytes += Arena::align(L.coef_count(c) * sizeof(float));
+        Call call(ctx);
+        RCG_TRY(call.begin(Arena::align(src_bytes) + coef_bytes(L) + sc_bytes));
+        uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+        int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+        float* d_sc[3] = {nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            d_co[c] = static_cast<int16_t*>(ctx->arena.take(L.coef_count(c) * sizeof(int16_t)));
+            d_sc[c] = static_cast<float*>(ctx->arena.take(L.coef_count(c) * sizeof(float)));
+        }
+        RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+        RCG_TRY(stage_and_launch_encode(ctx, call.D, L, d_src, quality, d_co, nullptr, d_sc));
+        for (uint32_t c = 0; c < L.ncomp; c++) {
+            RCG_HIP(hipMemcpyAsync(coefs[c], d_co[c], L.coef_count(c) * sizeof(int16_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            RCG_HIP(hipMemcpyAsync(scaled[c], d_sc[c], L.coef_count(c) * sizeof(float), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        }
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        if (qtables) quant_tables(quality, qtables, qtables + 64);
+        return RCG_OK;
+    });
 }
 
 }  // extern "C"
@@ -762,91 +797,99 @@ size_t rcg::sample_plane_bytes(const rcg_coef_image* ci) {
 extern "C" {
 
 int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels) {
-    if (!ctx || !pixels) return RCG_INVALID;
-    uint32_t hmax, vmax;
-    int rgb_direct;
-    RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
-    const size_t out_bytes = size_t(ci->width) * ci->height * out_channels(ci);
-    size_t need = Arena::align(out_bytes) + sample_plane_bytes(ci);
-    for (uint32_t c = 0; c < ci->ncomp; c++)
-        need += Arena::align(size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64 * sizeof(int16_t));
-    Call call(ctx);
-    RCG_TRY(call.begin(need));
-    const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < ci->ncomp; c++) {
-        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
-        int16_t* d = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
-        RCG_HIP(hipMemcpyAsync(d, ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice, ctx->stream));
-        d_co[c] = d;
-        planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
-    }
-    uint8_t* d_out = static_cast<uint8_t*>(ctx->arena.take(out_bytes));
-    RCG_TRY(reconstruct_on_device(ctx, ci, d_co, planes, d_out, hmax, vmax, rgb_direct));
-    RCG_HIP(hipMemcpyAsync(pixels, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !pixels) return RCG_INVALID;
+        uint32_t hmax, vmax;
+        int rgb_direct;
+        RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
+        const size_t out_bytes = size_t(ci->width) * ci->height * out_channels(ci);
+        size_t need = Arena::align(out_bytes) + sample_plane_bytes(ci);
+        for (uint32_t c = 0; c < ci->ncomp; c++)
+            need += Arena::align(size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64 * sizeof(int16_t));
+        Call call(ctx);
+        RCG_TRY(call.begin(need));
+        const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+            int16_t* d = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
+            RCG_HIP(hipMemcpyAsync(d, ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice, ctx->stream));
+            d_co[c] = d;
+            planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
+        }
+        uint8_t* d_out = static_cast<uint8_t*>(ctx->arena.take(out_bytes));
+        RCG_TRY(reconstruct_on_device(ctx, ci, d_co, planes, d_out, hmax, vmax, rgb_direct));
+        RCG_HIP(hipMemcpyAsync(pixels, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }
 
 int rcg_reconstruct_dev(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pixels) {
-    if (!ctx || !d_pixels) return RCG_INVALID;
-    uint32_t hmax, vmax;
-    int rgb_direct;
-    RCG_TRY(check_coef_image(d_ci, hmax, vmax, rgb_direct));
-    const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < d_ci->ncomp; c++) {
-        if (reinterpret_cast<uintptr_t>(d_ci->comps[c].coefs) & 15) {
-            set_error("coefficient buffers must be 16-byte aligned device pointers");
-            return RCG_INVALID;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !d_pixels) return RCG_INVALID;
+        uint32_t hmax, vmax;
+        int rgb_direct;
+        RCG_TRY(check_coef_image(d_ci, hmax, vmax, rgb_direct));
+        const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < d_ci->ncomp; c++) {
+            if (reinterpret_cast<uintptr_t>(d_ci->comps[c].coefs) & 15) {
+                set_error("coefficient buffers must be 16-byte aligned device pointers");
+                return RCG_INVALID;
+            }
+            d_co[c] = d_ci->comps[c].coefs;
         }
-        d_co[c] = d_ci->comps[c].coefs;
-    }
-    Call call(ctx);
-    RCG_TRY(call.begin(sample_plane_bytes(d_ci)));
-    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < d_ci->ncomp; c++)
-        planes[c] = static_cast<uint8_t*>(ctx->arena.take(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64));
-    RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+        Call call(ctx);
+        RCG_TRY(call.begin(sample_plane_bytes(d_ci)));
+        uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < d_ci->ncomp; c++)
+            planes[c] = static_cast<uint8_t*>(ctx->arena.take(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64));
+        RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }
 
 int rcg_ssimulacra2_dev(rcg_ctx* ctx, const uint8_t* d_reference, const uint8_t* d_distorted, uint32_t width,
                         uint32_t height, uint32_t channels, int precision, double* score, double* per_scale,
                         int* nscales) {
-    if (!ctx || !d_reference || !d_distorted || !score) return RCG_INVALID;
-    if (!valid_channels(channels)) return RCG_UNSUPPORTED;
-    if (width < 8 || height < 8) {
-        set_error("SSIMULACRA 2 needs at least 8x8 pixels");
-        return RCG_INVALID;
-    }
-    const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
-    const PairPlan P = plan_pair(width, height, T);
-    Call call(ctx);
-    RCG_TRY(call.begin(ssim_bytes(P, precision)));
-    return ssim_pair_dev(ctx, call, d_reference, d_distorted, width, height, channels, precision, score, per_scale,
-                         nscales);
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !d_reference || !d_distorted || !score) return RCG_INVALID;
+        if (!valid_channels(channels)) return RCG_UNSUPPORTED;
+        if (width < 8 || height < 8) {
+            set_error("SSIMULACRA 2 needs at least 8x8 pixels");
+            return RCG_INVALID;
+        }
+        const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
+        const PairPlan P = plan_pair(width, height, T);
+        Call call(ctx);
+        RCG_TRY(call.begin(ssim_bytes(P, precision)));
+        return ssim_pair_dev(ctx, call, d_reference, d_distorted, width, height, channels, precision, score, per_scale,
+                             nscales);
+    });
 }
 
 int rcg_ssimulacra2(rcg_ctx* ctx, const uint8_t* reference, const uint8_t* distorted, uint32_t width,
                     uint32_t height, uint32_t channels, int precision, double* score, double* per_scale,
                     int* nscales) {
-    if (!ctx || !reference || !distorted || !score) return RCG_INVALID;
-    if (!valid_channels(channels)) return RCG_UNSUPPORTED;
-    if (width < 8 || height < 8) {
-        set_error("SSIMULACRA 2 needs at least 8x8 pixels");
-        return RCG_INVALID;
-    }
-    const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
-    const PairPlan P = plan_pair(width, height, T);
-    const size_t n = size_t(width) * height * channels;
-    Call call(ctx);
-    RCG_TRY(call.begin(ssim_bytes(P, precision) + 2 * Arena::align(n)));
-    uint8_t* d_a = static_cast<uint8_t*>(ctx->arena.take(n));
-    uint8_t* d_b = static_cast<uint8_t*>(ctx->arena.take(n));
-    RCG_HIP(hipMemcpyAsync(d_a, reference, n, hipMemcpyHostToDevice, ctx->stream));
-    RCG_HIP(hipMemcpyAsync(d_b, distorted, n, hipMemcpyHostToDevice, ctx->stream));
-    return ssim_pair_dev(ctx, call, d_a, d_b, width, height, channels, precision, score, per_scale, nscales);
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !reference || !distorted || !score) return RCG_INVALID;
+        if (!valid_channels(channels)) return RCG_UNSUPPORTED;
+        if (width < 8 || height < 8) {
+            set_error("SSIMULACRA 2 needs at least 8x8 pixels");
+            return RCG_INVALID;
+        }
+        const SsimTune& T = precision == RCG_PREC_FP64 ? ctx->tune64 : ctx->tune32;
+        const PairPlan P = plan_pair(width, height, T);
+        const size_t n = size_t(width) * height * channels;
+        Call call(ctx);
+        RCG_TRY(call.begin(ssim_bytes(P, precision) + 2 * Arena::align(n)));
+        uint8_t* d_a = static_cast<uint8_t*>(ctx->arena.take(n));
+        uint8_t* d_b = static_cast<uint8_t*>(ctx->arena.take(n));
+        RCG_HIP(hipMemcpyAsync(d_a, reference, n, hipMemcpyHostToDevice, ctx->stream));
+        RCG_HIP(hipMemcpyAsync(d_b, distorted, n, hipMemcpyHostToDevice, ctx->stream));
+        return ssim_pair_dev(ctx, call, d_a, d_b, width, height, channels, precision, score, per_scale, nscales);
+    });
 }
 
 }  // extern "C"
@@ -988,14 +1031,18 @@ extern "C" {
 int rcg_find_quality(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height, uint32_t channels,
                      double target, int max_quality, int subsampling, rcg_found* out, int16_t* coefs[3],
                      uint8_t qtables[128]) {
-    return find_quality_impl(ctx, pixels, true, width, height, channels, target, max_quality, subsampling, out,
-                             coefs, qtables);
+    return rcg::guarded([&]() -> int {
+        return find_quality_impl(ctx, pixels, true, width, height, channels, target, max_quality, subsampling, out,
+                                 coefs, qtables);
+    });
 }
 
 int rcg_find_quality_dev(rcg_ctx* ctx, const uint8_t* d_pixels, uint32_t width, uint32_t height,
                          uint32_t channels, double target, int max_quality, int subsampling, rcg_found* out) {
-    return find_quality_impl(ctx, d_pixels, false, width, height, channels, target, max_quality, subsampling, out,
-                             nullptr, nullptr);
+    return rcg::guarded([&]() -> int {
+        return find_quality_impl(ctx, d_pixels, false, width, height, channels, target, max_quality, subsampling, out,
+                                 nullptr, nullptr);
+    });
 }
 
 }  // extern "C"
@@ -1117,77 +1164,79 @@ int score_trellis(rcg_ctx* ctx, Call& call, const Layout& L, const uint8_t* d_sr
 extern "C" int rcg_find_quality_trellis(rcg_ctx* ctx, const uint8_t* pixels, uint32_t width, uint32_t height,
                                         uint32_t channels, double target, int max_quality, int subsampling,
                                         rcg_found* out, int16_t* coefs[3], uint8_t qtables[128]) {
-    if (!ctx || !pixels || !out) return RCG_INVALID;
-    RCG_TRY(check_image(width, height, channels));
-    if (width < 8 || height < 8) {
-        set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
-        return RCG_UNSUPPORTED;
-    }
-    const Layout Lmax = max_layout(width, height, channels, subsampling);
-    const PairPlan P32 = plan_pair(width, height, ctx->tune32);
-    const PairPlan P64 = plan_pair(width, height, ctx->tune64);
-    const size_t src_bytes = size_t(width) * height * channels;
-    const size_t need = search_need(ctx, width, height, channels, subsampling, true);
-    try {
-        Call call(ctx);
-        RCG_TRY(call.begin(need));
-        uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
-        RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
-        uint8_t* rec[3] = {nullptr, nullptr, nullptr};
-        int16_t* d_co[3] = {nullptr, nullptr, nullptr};
-        float* d_sc[3] = {nullptr, nullptr, nullptr};
-        for (uint32_t c = 0; c < Lmax.ncomp; c++) {
-            rec[c] = static_cast<uint8_t*>(ctx->arena.take(Lmax.plane_bytes(c)));
-            d_co[c] = static_cast<int16_t*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(int16_t)));
-            d_sc[c] = static_cast<float*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(float)));
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !pixels || !out) return RCG_INVALID;
+        RCG_TRY(check_image(width, height, channels));
+        if (width < 8 || height < 8) {
+            set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
+            return RCG_UNSUPPORTED;
         }
-        const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
-        const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
-        bool a32 = false, a64 = false;
-        TrellisHost H;
-        auto keep = [&](const Layout& Lq) {
-            for (uint32_t c = 0; c < Lq.ncomp; c++) H.best[c] = H.tr[c];
-        };
-        // jpeg/src/target.rs:37-55
-        int lo = 1, hi = std::max(1, std::min(100, max_quality));
-        int steps = 0;
-        double s = 0.0;
-        Layout L = make_layout(width, height, channels, hi, subsampling);
-        RCG_TRY(score_trellis(ctx, call, L, d_src, hi, d_co, d_sc, rec, H, B32, B64, a32, a64, target, s));
-        steps++;
-        keep(L);
-        int best_q = hi;
-        double best_s = s;
-        if (s >= target) {
-            while (lo < hi) {
-                const int mid = lo + (hi - lo) / 2;
-                L = make_layout(width, height, channels, mid, subsampling);
-                RCG_TRY(score_trellis(ctx, call, L, d_src, mid, d_co, d_sc, rec, H, B32, B64, a32, a64, target, s));
-                steps++;
-                if (s >= target) {
-                    hi = mid;
-                    best_q = mid;
-                    best_s = s;
-                    keep(L);
-                } else {
-                    lo = mid + 1;
+        const Layout Lmax = max_layout(width, height, channels, subsampling);
+        const PairPlan P32 = plan_pair(width, height, ctx->tune32);
+        const PairPlan P64 = plan_pair(width, height, ctx->tune64);
+        const size_t src_bytes = size_t(width) * height * channels;
+        const size_t need = search_need(ctx, width, height, channels, subsampling, true);
+        try {
+            Call call(ctx);
+            RCG_TRY(call.begin(need));
+            uint8_t* d_src = static_cast<uint8_t*>(ctx->arena.take(src_bytes));
+            RCG_HIP(hipMemcpyAsync(d_src, pixels, src_bytes, hipMemcpyHostToDevice, ctx->stream));
+            uint8_t* rec[3] = {nullptr, nullptr, nullptr};
+            int16_t* d_co[3] = {nullptr, nullptr, nullptr};
+            float* d_sc[3] = {nullptr, nullptr, nullptr};
+            for (uint32_t c = 0; c < Lmax.ncomp; c++) {
+                rec[c] = static_cast<uint8_t*>(ctx->arena.take(Lmax.plane_bytes(c)));
+                d_co[c] = static_cast<int16_t*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(int16_t)));
+                d_sc[c] = static_cast<float*>(ctx->arena.take(Lmax.coef_count(c) * sizeof(float)));
+            }
+            const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
+            const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
+            bool a32 = false, a64 = false;
+            TrellisHost H;
+            auto keep = [&](const Layout& Lq) {
+                for (uint32_t c = 0; c < Lq.ncomp; c++) H.best[c] = H.tr[c];
+            };
+            // jpeg/src/target.rs:37-55
+            int lo = 1, hi = std::max(1, std::min(100, max_quality));
+            int steps = 0;
+            double s = 0.0;
+            Layout L = make_layout(width, height, channels, hi, subsampling);
+            RCG_TRY(score_trellis(ctx, call, L, d_src, hi, d_co, d_sc, rec, H, B32, B64, a32, a64, target, s));
+            steps++;
+            keep(L);
+            int best_q = hi;
+            double best_s = s;
+            if (s >= target) {
+                while (lo < hi) {
+                    const int mid = lo + (hi - lo) / 2;
+                    L = make_layout(width, height, channels, mid, subsampling);
+                    RCG_TRY(score_trellis(ctx, call, L, d_src, mid, d_co, d_sc, rec, H, B32, B64, a32, a64, target, s));
+                    steps++;
+                    if (s >= target) {
+                        hi = mid;
+                        best_q = mid;
+                        best_s = s;
+                        keep(L);
+                    } else {
+                        lo = mid + 1;
+                    }
                 }
             }
+            out->quality = best_q;
+            out->score = best_s;
+            out->steps = steps;
+            if (coefs) {
+                const Layout Lb = make_layout(width, height, channels, best_q, subsampling);
+                for (uint32_t c = 0; c < Lb.ncomp; c++)
+                    if (coefs[c]) std::memcpy(coefs[c], H.best[c].data(), Lb.coef_count(c) * sizeof(int16_t));
+            }
+            if (qtables) quant_tables(best_q, qtables, qtables + 64);
+        } catch (const std::bad_alloc&) {
+            set_error("out of host memory");
+            return RCG_OOM;
         }
-        out->quality = best_q;
-        out->score = best_s;
-        out->steps = steps;
-        if (coefs) {
-            const Layout Lb = make_layout(width, height, channels, best_q, subsampling);
-            for (uint32_t c = 0; c < Lb.ncomp; c++)
-                if (coefs[c]) std::memcpy(coefs[c], H.best[c].data(), Lb.coef_count(c) * sizeof(int16_t));
-        }
-        if (qtables) quant_tables(best_q, qtables, qtables + 64);
-    } catch (const std::bad_alloc&) {
-        set_error("out of host memory");
-        return RCG_OOM;
-    }
-    return RCG_OK;
+        return RCG_OK;
+    });
 }
 
 // ================================================================================================================
@@ -1301,138 +1350,142 @@ int score_requant(rcg_ctx* ctx, Call& call, const rcg_coef_image* ci, const Layo
 
 extern "C" int rcg_requant_find_quality(rcg_ctx* ctx, const rcg_coef_image* ci, double target, int max_quality,
                                         int trellis, rcg_found* out, rcg_requantized* winner) {
-    if (!ctx || !ci || !out) return RCG_INVALID;
-    uint32_t hmax, vmax;
-    int rgb_direct;
-    RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
-    if (ci->width < 8 || ci->height < 8) {
-        set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
-        return RCG_UNSUPPORTED;
-    }
-    const rcg_coef_component* k = ci->comps;
-    RequantRoute route;
-    std::memset(&route, 0, sizeof(route));
-    route.fused = (ci->ncomp == 1 || ci->ncomp == 3) && !rgb_direct && k[0].h == hmax && k[0].v == vmax &&
-                  hmax <= 2 && vmax <= 2;
-    if (ci->ncomp == 3)
-        route.fused = route.fused && k[1].h == 1 && k[1].v == 1 && k[2].h == 1 && k[2].v == 1 &&
-                      k[1].blocks_w == k[2].blocks_w && k[1].blocks_h == k[2].blocks_h;
-    route.rgb_direct = rgb_direct;
-    route.adobe_transform = ci->adobe_transform < 0 ? -1 : ci->adobe_transform;
-    Layout L;
-    std::memset(&L, 0, sizeof(L));
-    L.w = ci->width;
-    L.h = ci->height;
-    L.channels = out_channels(ci);
-    L.hmax = hmax;
-    L.vmax = vmax;
-    L.ncomp = ci->ncomp;
-    for (uint32_t c = 0; c < ci->ncomp; c++) {
-        L.bw[c] = ci->comps[c].blocks_w;
-        L.bh[c] = ci->comps[c].blocks_h;
-    }
-    const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
-    const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
-    const size_t ref_bytes = size_t(L.w) * L.h * L.channels;
-    const size_t need = requant_need(ctx, L);
-    std::vector<std::vector<int16_t>> host_src(ci->ncomp);
-    for (uint32_t c = 0; c < ci->ncomp; c++) {
-        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
-        if (trellis) host_src[c].assign(ci->comps[c].coefs, ci->comps[c].coefs + nb * 64);
-    }
-    Call call(ctx);
-    RCG_TRY(call.begin(need));
-    int16_t* d_src[4] = {nullptr, nullptr, nullptr, nullptr};
-    int16_t* d_tr[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint8_t* src_planes[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < ci->ncomp; c++) {
-        const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
-        d_src[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
-        d_tr[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
-        planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
-        src_planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
-        RCG_HIP(hipMemcpyAsync(d_src[c], ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice,
-                               ctx->stream));
-        const rcg_coef_component& kc = ci->comps[c];
-        route.views[c].p = planes[c];
-        route.views[c].stride = kc.blocks_w * 8;
-        route.views[c].cw = (ci->width * kc.h + hmax - 1) / hmax;
-        route.views[c].ch = (ci->height * kc.v + vmax - 1) / vmax;
-        route.views[c].fx = hmax / kc.h;
-        route.views[c].fy = vmax / kc.v;
-    }
-    uint8_t* d_ref = static_cast<uint8_t*>(ctx->arena.take(ref_bytes));
-    if (!route.fused) route.d_cand = static_cast<uint8_t*>(ctx->arena.take(size_t(L.w) * L.h * 3));
-    const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
-    const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
-    // reference = reconstruct(&ci) (pipeline.rs:373), decoded once on the device.
-    RCG_TRY(reconstruct_on_device(ctx, ci, d_src, src_planes, d_ref, hmax, vmax, rgb_direct));
-    bool a32 = false, a64 = false;
-    // jpeg/src/target.rs:37-55
-    int lo = 1, hi = std::max(1, std::min(100, max_quality));
-    int steps = 0;
-    double s = 0.0;
-    RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, hi, trellis != 0,
-                          route, B32, B64, a32, a64, target, s));
-    steps++;
-    int best_q = hi;
-    double best_s = s;
-    if (s >= target) {
-        while (lo < hi) {
-            const int mid = lo + (hi - lo) / 2;
-            RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, mid,
-                                  trellis != 0, route, B32, B64, a32, a64, target, s));
-            steps++;
-            if (s >= target) {
-                hi = mid;
-                best_q = mid;
-                best_s = s;
-            } else {
-                lo = mid + 1;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !ci || !out) return RCG_INVALID;
+        uint32_t hmax, vmax;
+        int rgb_direct;
+        RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
+        if (ci->width < 8 || ci->height < 8) {
+            set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
+            return RCG_UNSUPPORTED;
+        }
+        const rcg_coef_component* k = ci->comps;
+        RequantRoute route;
+        std::memset(&route, 0, sizeof(route));
+        route.fused = (ci->ncomp == 1 || ci->ncomp == 3) && !rgb_direct && k[0].h == hmax && k[0].v == vmax &&
+                      hmax <= 2 && vmax <= 2;
+        if (ci->ncomp == 3)
+            route.fused = route.fused && k[1].h == 1 && k[1].v == 1 && k[2].h == 1 && k[2].v == 1 &&
+                          k[1].blocks_w == k[2].blocks_w && k[1].blocks_h == k[2].blocks_h;
+        route.rgb_direct = rgb_direct;
+        route.adobe_transform = ci->adobe_transform < 0 ? -1 : ci->adobe_transform;
+        Layout L;
+        std::memset(&L, 0, sizeof(L));
+        L.w = ci->width;
+        L.h = ci->height;
+        L.channels = out_channels(ci);
+        L.hmax = hmax;
+        L.vmax = vmax;
+        L.ncomp = ci->ncomp;
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            L.bw[c] = ci->comps[c].blocks_w;
+            L.bh[c] = ci->comps[c].blocks_h;
+        }
+        const PairPlan P32 = plan_pair(L.w, L.h, ctx->tune32);
+        const PairPlan P64 = plan_pair(L.w, L.h, ctx->tune64);
+        const size_t ref_bytes = size_t(L.w) * L.h * L.channels;
+        const size_t need = requant_need(ctx, L);
+        std::vector<std::vector<int16_t>> host_src(ci->ncomp);
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+            if (trellis) host_src[c].assign(ci->comps[c].coefs, ci->comps[c].coefs + nb * 64);
+        }
+        Call call(ctx);
+        RCG_TRY(call.begin(need));
+        int16_t* d_src[4] = {nullptr, nullptr, nullptr, nullptr};
+        int16_t* d_tr[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint8_t* src_planes[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < ci->ncomp; c++) {
+            const size_t nb = size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h;
+            d_src[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
+            d_tr[c] = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
+            planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
+            src_planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
+            RCG_HIP(hipMemcpyAsync(d_src[c], ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            const rcg_coef_component& kc = ci->comps[c];
+            route.views[c].p = planes[c];
+            route.views[c].stride = kc.blocks_w * 8;
+            route.views[c].cw = (ci->width * kc.h + hmax - 1) / hmax;
+            route.views[c].ch = (ci->height * kc.v + vmax - 1) / vmax;
+            route.views[c].fx = hmax / kc.h;
+            route.views[c].fy = vmax / kc.v;
+        }
+        uint8_t* d_ref = static_cast<uint8_t*>(ctx->arena.take(ref_bytes));
+        if (!route.fused) route.d_cand = static_cast<uint8_t*>(ctx->arena.take(size_t(L.w) * L.h * 3));
+        const SsimBuffers B32 = take_ssim(ctx->arena, P32, RCG_PREC_FP32);
+        const SsimBuffers B64 = take_ssim(ctx->arena, P64, RCG_PREC_FP64);
+        // reference = reconstruct(&ci) (pipeline.rs:373), decoded once on the device.
+        RCG_TRY(reconstruct_on_device(ctx, ci, d_src, src_planes, d_ref, hmax, vmax, rgb_direct));
+        bool a32 = false, a64 = false;
+        // jpeg/src/target.rs:37-55
+        int lo = 1, hi = std::max(1, std::min(100, max_quality));
+        int steps = 0;
+        double s = 0.0;
+        RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, hi, trellis != 0,
+                              route, B32, B64, a32, a64, target, s));
+        steps++;
+        int best_q = hi;
+        double best_s = s;
+        if (s >= target) {
+            while (lo < hi) {
+                const int mid = lo + (hi - lo) / 2;
+                RCG_TRY(score_requant(ctx, call, ci, L, d_src, d_ref, L.channels, planes, d_tr, host_src, mid,
+                                      trellis != 0, route, B32, B64, a32, a64, target, s));
+                steps++;
+                if (s >= target) {
+                    hi = mid;
+                    best_q = mid;
+                    best_s = s;
+                } else {
+                    lo = mid + 1;
+                }
             }
         }
-    }
-    out->quality = best_q;
-    out->score = best_s;
-    out->steps = steps;
-    if (winner) RCG_TRY(rcg_requantize(ci, best_q, trellis, winner));
-    return RCG_OK;
+        out->quality = best_q;
+        out->score = best_s;
+        out->steps = steps;
+        if (winner) RCG_TRY(rcg_requantize(ci, best_q, trellis, winner));
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_search_bytes(rcg_ctx* ctx, uint32_t width, uint32_t height, uint32_t channels, int subsampling,
                                 int mode, size_t* out) {
-    if (!ctx || !out) return RCG_INVALID;
-    if (mode == RCG_SEARCH_REQUANT && channels == 4) {
-        // A four-component source: every component's grid is at most the 4:2:0 luma grid of its size.
-        RCG_TRY(check_image(width, height, 3));
-        Layout L = max_layout(width, height, 3, RCG_AUTO);
-        L.ncomp = 4;
-        for (int c = 1; c < 4; c++) {
-            L.bw[c] = L.bw[0];
-            L.bh[c] = L.bh[0];
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !out) return RCG_INVALID;
+        if (mode == RCG_SEARCH_REQUANT && channels == 4) {
+            // A four-component source: every component's grid is at most the 4:2:0 luma grid of its size.
+            RCG_TRY(check_image(width, height, 3));
+            Layout L = max_layout(width, height, 3, RCG_AUTO);
+            L.ncomp = 4;
+            for (int c = 1; c < 4; c++) {
+                L.bw[c] = L.bw[0];
+                L.bh[c] = L.bh[0];
+            }
+            *out = requant_need(ctx, L) + kDescBytes + 4096;
+            return RCG_OK;
         }
-        *out = requant_need(ctx, L) + kDescBytes + 4096;
+        RCG_TRY(check_image(width, height, channels));
+        size_t need = 0;
+        switch (mode) {
+            case RCG_SEARCH_PLAIN: need = search_need(ctx, width, height, channels, subsampling, false); break;
+            case RCG_SEARCH_TRELLIS: need = search_need(ctx, width, height, channels, subsampling, true); break;
+            case RCG_SEARCH_REQUANT: {
+                // The search takes sources with sampling ratios 1 or 2: their grids are at most the larger of the
+                // 4:4:4 and 4:2:0 ones of that size, per component.
+                const Layout L = max_layout(width, height, channels, RCG_AUTO);
+                need = requant_need(ctx, L);
+                break;
+            }
+            default:
+                set_error("unknown search mode");
+                return RCG_INVALID;
+        }
+        *out = need + kDescBytes + 4096;
         return RCG_OK;
-    }
-    RCG_TRY(check_image(width, height, channels));
-    size_t need = 0;
-    switch (mode) {
-        case RCG_SEARCH_PLAIN: need = search_need(ctx, width, height, channels, subsampling, false); break;
-        case RCG_SEARCH_TRELLIS: need = search_need(ctx, width, height, channels, subsampling, true); break;
-        case RCG_SEARCH_REQUANT: {
-            // The search takes sources with sampling ratios 1 or 2: their grids are at most the larger of the
-            // 4:4:4 and 4:2:0 ones of that size, per component.
-            const Layout L = max_layout(width, height, channels, RCG_AUTO);
-            need = requant_need(ctx, L);
-            break;
-        }
-        default:
-            set_error("unknown search mode");
-            return RCG_INVALID;
-    }
-    *out = need + kDescBytes + 4096;
-    return RCG_OK;
+    });
 }
 
 // ================================================================================================================
@@ -1470,221 +1523,239 @@ int tune_from_c(const rcg_tune* in, SsimTune& T) {
 }  // namespace
 
 extern "C" int rcg_get_tune(rcg_ctx* ctx, rcg_tune* out) {
-    if (!ctx || !out) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    tune_to_c(ctx->tune32, out);
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !out) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        tune_to_c(ctx->tune32, out);
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune) {
-    if (!ctx) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    SsimTune T = ctx->tune32;
-    const int rc = tune_from_c(tune, T);
-    if (rc != RCG_OK) {
-        set_error("invalid tile policy");
-        return rc;
-    }
-    ctx->tune32 = T;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        SsimTune T = ctx->tune32;
+        const int rc = tune_from_c(tune, T);
+        if (rc != RCG_OK) {
+            set_error("invalid tile policy");
+            return rc;
+        }
+        ctx->tune32 = T;
+        return RCG_OK;
+    });
 }
 
 // Times k_ssim_scale (f32, the reference cache and f32 planes in, no pyramid out) on `jobs` pairs of S x S synthetic
 // planes in one launch for every tile height, for S = 4096 ... 64, and installs the fastest tile height of each size
 // class.
 extern "C" int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out) {
-    // jobs = 0: throughput mode, for launches of very many pairs (a batch of many images). Each timing launch then
-    // holds groups of 8 pairs, as many as make about 8 rounds of the GPU's workgroup slots, and the time per pair is
-    // compared: a partly idle last round, which a launch of a few pairs pays and a large batch does not, no longer
-    // favours short tiles.
-    const bool throughput = jobs == 0;
-    const int candidates = jobs;
-    if (!ctx || candidates < 0 || candidates > 64) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
-    // Tile heights: a fine grid (a launch's last round of workgroups is the less idle the closer its tile count comes
-    // to a multiple of the GPU's workgroup slots, and one step of the grid moves the count by a few percent). Small scales are latency-bound
-    // (one workgroup walks TH + 10 rows one after another), so very short tiles are tried too; a launch of many
-    // rounds (throughput) is best served by tall ones, whose 10-row halo costs least.
-    const int ths[] = {2,  4,  6,  8,   12,  16,  24,  32,  40,  48,  56,  64,  72,  80,  88,  96,
-                       104, 112, 128, 144, 160, 176, 208, 256, 320, 384, 512};
-    const int nsz = 7;
-    const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
-    ctx->arena.reset();
-    RCG_TRY(ctx->arena.ensure(5 * Arena::align(plane_bytes) + (size_t(32) << 20) + (size_t(4) << 20)));
-    RCG_TRY(ctx->pinned.ensure(size_t(4) << 20));
-    float* a = static_cast<float*>(ctx->arena.take(plane_bytes));
-    float* b = static_cast<float*>(ctx->arena.take(plane_bytes));
-    float* cache = static_cast<float*>(ctx->arena.take(3 * plane_bytes));
-    double* partials = static_cast<double*>(ctx->arena.take(size_t(32) << 20));
-    char* ddesc = static_cast<char*>(ctx->arena.take(size_t(4) << 20));
-    // Synthetic linear-light planes: a smooth pattern and a perturbed copy (the timing does not depend on values).
-    {
-        std::vector<float> h(size_t(4096) * 4096 * 3);
-        uint64_t z = 0x9E3779B97F4A7C15ull;
-        for (size_t i = 0; i < h.size(); i++) {
-            z = z * 6364136223846793005ull + 1442695040888963407ull;
-            h[i] = float((z >> 40) & 0xFFFF) / 65535.0f;
-        }
-        RCG_HIP(hipMemcpy(a, h.data(), plane_bytes, hipMemcpyHostToDevice));
-        for (size_t i = 0; i < h.size(); i++) h[i] = std::min(1.0f, h[i] * 0.97f + 0.01f);
-        RCG_HIP(hipMemcpy(b, h.data(), plane_bytes, hipMemcpyHostToDevice));
-    }
-    hipEvent_t e0, e1;
-    RCG_HIP(hipEventCreate(&e0));
-    RCG_HIP(hipEventCreate(&e1));
-    SsimTune T = ctx->tune32;
-    int best_th[nsz];
-    const bool prof = ctx->prof.on;
-    ctx->prof.on = false;
-    for (int si = 0; si < nsz; si++) {
-        const uint32_t S = sizes[si];
-        double best = 1e30;
-        best_th[si] = 8;
-        for (int th : ths) {
-            if (uint32_t(th) > S) continue;
-            const int nt = S <= T.narrow_w ? 128 : T.wide_nt;
-            const uint32_t tx = ssim_tiles_x(S, nt), ty = ssim_tiles_y(S, th);
-            const uint32_t tiles = tx * ty;
-            const int group = throughput ? 8 : candidates;
-            const int pairs = throughput ? int(std::min<uint32_t>(4096u, ((8192u + tiles - 1) / tiles + 7) / 8 * 8))
-                                         : candidates;
-            if (size_t(tiles) * 18 * 8 * pairs > (size_t(32) << 20)) continue;
-            if ((sizeof(SsimJob) + 64) * size_t(pairs) + sizeof(Unit) * (size_t(tiles) * pairs + 64 * kXcds) + 4096 >
-                (size_t(4) << 20))
-                continue;
-            // The reference cache of the S x S plane `a` (the same for every tile height; rebuilt per size).
-            RefJob rj;
-            std::memset(&rj, 0, sizeof(rj));
-            rj.w = S;
-            rj.h = S;
-            rj.th = uint32_t(th);
-            rj.tiles_x = tx;
-            rj.tiles_y = ty;
-            rj.a = planes_src(a);
-            rj.cache = cache;
-            std::vector<SsimJob> jobs(pairs);
-            for (int c = 0; c < pairs; c++) {
-                SsimJob& j = jobs[c];
-                std::memset(&j, 0, sizeof(j));
-                j.w = S;
-                j.h = S;
-                j.th = uint32_t(th);
-                j.tiles_x = tx;
-                j.tiles_y = ty;
-                j.a = cache_src(cache);
-                j.b = planes_src(b);
-                j.partials = partials + size_t(c) * tx * ty * 18;
+    return rcg::guarded([&]() -> int {
+        // jobs = 0: throughput mode, for launches of very many pairs (a batch of many images). Each timing launch then
+        // holds groups of 8 pairs, as many as make about 8 rounds of the GPU's workgroup slots, and the time per pair is
+        // compared: a partly idle last round, which a launch of a few pairs pays and a large batch does not, no longer
+        // favours short tiles.
+        const bool throughput = jobs == 0;
+        const int candidates = jobs;
+        if (!ctx || candidates < 0 || candidates > 64) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64};
+        // Tile heights: a fine grid (a launch's last round of workgroups is the less idle the closer its tile count comes
+        // to a multiple of the GPU's workgroup slots, and one step of the grid moves the count by a few percent). Small scales are latency-bound
+        // (one workgroup walks TH + 10 rows one after another), so very short tiles are tried too; a launch of many
+        // rounds (throughput) is best served by tall ones, whose 10-row halo costs least.
+        const int ths[] = {2,  4,  6,  8,   12,  16,  24,  32,  40,  48,  56,  64,  72,  80,  88,  96,
+                           104, 112, 128, 144, 160, 176, 208, 256, 320, 384, 512};
+        const int nsz = 7;
+        const size_t plane_bytes = size_t(4096) * 4096 * 3 * sizeof(float);
+        ctx->arena.reset();
+        RCG_TRY(ctx->arena.ensure(5 * Arena::align(plane_bytes) + (size_t(32) << 20) + (size_t(4) << 20)));
+        RCG_TRY(ctx->pinned.ensure(size_t(4) << 20));
+        float* a = static_cast<float*>(ctx->arena.take(plane_bytes));
+        float* b = static_cast<float*>(ctx->arena.take(plane_bytes));
+        float* cache = static_cast<float*>(ctx->arena.take(3 * plane_bytes));
+        double* partials = static_cast<double*>(ctx->arena.take(size_t(32) << 20));
+        char* ddesc = static_cast<char*>(ctx->arena.take(size_t(4) << 20));
+        // Synthetic linear-light planes: a smooth pattern and a perturbed copy (the timing does not depend on values).
+        {
+            std::vector<float> h(size_t(4096) * 4096 * 3);
+            uint64_t z = 0x9E3779B97F4A7C15ull;
+            for (size_t i = 0; i < h.size(); i++) {
+                z = z * 6364136223846793005ull + 1442695040888963407ull;
+                h[i] = float((z >> 40) & 0xFFFF) / 65535.0f;
             }
-            // The candidates of a tile side by side on one XCD, as the scoring passes launch them.
-            std::vector<UnitGroup> groups;
-            for (int g = 0; g < pairs; g += group) groups.push_back(UnitGroup{uint32_t(g), uint32_t(group), tiles});
-            const std::vector<Unit> units = xcd_units(groups);
-            Desc D;
-            D.host = ctx->pinned.base;
-            D.dev = ddesc;
-            D.cap = size_t(4) << 20;
-            const SsimJob* dj = D.put(jobs.data(), jobs.size());
-            const Unit* du = D.put(units.data(), units.size());
-            const RefJob* drj = D.put(&rj, 1);
-            RCG_TRY(D.upload(ctx->stream));
-            RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes, ctx->stream));
-            const uint32_t nu = uint32_t(units.size());
-            RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
-            const size_t work = size_t(S) * S * size_t(pairs);
-            const int reps = work >= (size_t(64) << 20) ? 2 : (work >= (size_t(16) << 20) ? 3 : 10);
-            RCG_HIP(hipEventRecord(e0, ctx->stream));
-            for (int r = 0; r < reps; r++)
+            RCG_HIP(hipMemcpy(a, h.data(), plane_bytes, hipMemcpyHostToDevice));
+            for (size_t i = 0; i < h.size(); i++) h[i] = std::min(1.0f, h[i] * 0.97f + 0.01f);
+            RCG_HIP(hipMemcpy(b, h.data(), plane_bytes, hipMemcpyHostToDevice));
+        }
+        hipEvent_t e0, e1;
+        RCG_HIP(hipEventCreate(&e0));
+        RCG_HIP(hipEventCreate(&e1));
+        SsimTune T = ctx->tune32;
+        int best_th[nsz];
+        const bool prof = ctx->prof.on;
+        ctx->prof.on = false;
+        for (int si = 0; si < nsz; si++) {
+            const uint32_t S = sizes[si];
+            double best = 1e30;
+            best_th[si] = 8;
+            for (int th : ths) {
+                if (uint32_t(th) > S) continue;
+                const int nt = S <= T.narrow_w ? 128 : T.wide_nt;
+                const uint32_t tx = ssim_tiles_x(S, nt), ty = ssim_tiles_y(S, th);
+                const uint32_t tiles = tx * ty;
+                const int group = throughput ? 8 : candidates;
+                const int pairs = throughput ? int(std::min<uint32_t>(4096u, ((8192u + tiles - 1) / tiles + 7) / 8 * 8))
+                                             : candidates;
+                if (size_t(tiles) * 18 * 8 * pairs > (size_t(32) << 20)) continue;
+                if ((sizeof(SsimJob) + 64) * size_t(pairs) + sizeof(Unit) * (size_t(tiles) * pairs + 64 * kXcds) + 4096 >
+                    (size_t(4) << 20))
+                    continue;
+                // The reference cache of the S x S plane `a` (the same for every tile height; rebuilt per size).
+                RefJob rj;
+                std::memset(&rj, 0, sizeof(rj));
+                rj.w = S;
+                rj.h = S;
+                rj.th = uint32_t(th);
+                rj.tiles_x = tx;
+                rj.tiles_y = ty;
+                rj.a = planes_src(a);
+                rj.cache = cache;
+                std::vector<SsimJob> jobs(pairs);
+                for (int c = 0; c < pairs; c++) {
+                    SsimJob& j = jobs[c];
+                    std::memset(&j, 0, sizeof(j));
+                    j.w = S;
+                    j.h = S;
+                    j.th = uint32_t(th);
+                    j.tiles_x = tx;
+                    j.tiles_y = ty;
+                    j.a = cache_src(cache);
+                    j.b = planes_src(b);
+                    j.partials = partials + size_t(c) * tx * ty * 18;
+                }
+                // The candidates of a tile side by side on one XCD, as the scoring passes launch them.
+                std::vector<UnitGroup> groups;
+                for (int g = 0; g < pairs; g += group) groups.push_back(UnitGroup{uint32_t(g), uint32_t(group), tiles});
+                const std::vector<Unit> units = xcd_units(groups);
+                Desc D;
+                D.host = ctx->pinned.base;
+                D.dev = ddesc;
+                D.cap = size_t(4) << 20;
+                const SsimJob* dj = D.put(jobs.data(), jobs.size());
+                const Unit* du = D.put(units.data(), units.size());
+                const RefJob* drj = D.put(&rj, 1);
+                RCG_TRY(D.upload(ctx->stream));
+                RCG_TRY(launch_ssim_ref(ctx, drj, nullptr, tx * ty, nt, kSrcPlanes, ctx->stream));
+                const uint32_t nu = uint32_t(units.size());
                 RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
-            RCG_HIP(hipEventRecord(e1, ctx->stream));
-            RCG_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            RCG_HIP(hipEventElapsedTime(&ms, e0, e1));
-            const double per = double(ms) / reps / pairs;  // per pair; the pair count is fixed unless throughput
-            if (per < best) {
-                best = per;
-                best_th[si] = th;
+                const size_t work = size_t(S) * S * size_t(pairs);
+                const int reps = work >= (size_t(64) << 20) ? 2 : (work >= (size_t(16) << 20) ? 3 : 10);
+                RCG_HIP(hipEventRecord(e0, ctx->stream));
+                for (int r = 0; r < reps; r++)
+                    RCG_TRY(launch_ssim_scale(ctx, RCG_PREC_FP32, dj, du, nu, nt, kSrcRefCache, kSrcPlanes));
+                RCG_HIP(hipEventRecord(e1, ctx->stream));
+                RCG_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                RCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                const double per = double(ms) / reps / pairs;  // per pair; the pair count is fixed unless throughput
+                if (per < best) {
+                    best = per;
+                    best_th[si] = th;
+                }
             }
         }
-    }
-    ctx->prof.on = prof;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    // Size classes: class i covers heights in [sizes[i], sizes[i-1]); adjacent classes with the same choice merge.
-    SsimTune N = T;
-    N.nclass = 0;
-    for (int si = 0; si < nsz; si++) {
-        const uint32_t lo = si == nsz - 1 ? 0 : sizes[si];
-        if (N.nclass > 0 && N.th[N.nclass - 1] == best_th[si]) {
-            N.th_min_h[N.nclass - 1] = lo;
-            continue;
+        ctx->prof.on = prof;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        // Size classes: class i covers heights in [sizes[i], sizes[i-1]); adjacent classes with the same choice merge.
+        SsimTune N = T;
+        N.nclass = 0;
+        for (int si = 0; si < nsz; si++) {
+            const uint32_t lo = si == nsz - 1 ? 0 : sizes[si];
+            if (N.nclass > 0 && N.th[N.nclass - 1] == best_th[si]) {
+                N.th_min_h[N.nclass - 1] = lo;
+                continue;
+            }
+            N.th[N.nclass] = best_th[si];
+            N.th_min_h[N.nclass] = lo;
+            N.nclass++;
         }
-        N.th[N.nclass] = best_th[si];
-        N.th_min_h[N.nclass] = lo;
-        N.nclass++;
-    }
-    for (int i = N.nclass; i < 8; i++) {
-        N.th[i] = N.th[N.nclass - 1];
-        N.th_min_h[i] = 0;
-    }
-    ctx->tune32 = N;
-    if (out) tune_to_c(N, out);
-    return RCG_OK;
+        for (int i = N.nclass; i < 8; i++) {
+            N.th[i] = N.th[N.nclass - 1];
+            N.th_min_h[i] = 0;
+        }
+        ctx->tune32 = N;
+        if (out) tune_to_c(N, out);
+        return RCG_OK;
+    });
 }
 
 // ================================================================================================================
 // Device buffers
 
 extern "C" int rcg_device_alloc(rcg_ctx* ctx, size_t bytes, void** d_ptr) {
-    if (!ctx || !d_ptr) return RCG_INVALID;
-    *d_ptr = nullptr;
-    RCG_HIP(hipSetDevice(ctx->device));
-    RCG_HIP(hipMalloc(d_ptr, bytes ? bytes : 1));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !d_ptr) return RCG_INVALID;
+        *d_ptr = nullptr;
+        RCG_HIP(hipSetDevice(ctx->device));
+        RCG_HIP(hipMalloc(d_ptr, bytes ? bytes : 1));
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_device_free(rcg_ctx* ctx, void* d_ptr) {
-    if (!ctx) return RCG_INVALID;
-    if (!d_ptr) return RCG_OK;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    RCG_HIP(hipFree(d_ptr));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx) return RCG_INVALID;
+        if (!d_ptr) return RCG_OK;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        RCG_HIP(hipFree(d_ptr));
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_memcpy_h2d(rcg_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
-    if (!ctx || (!d_dst && bytes) || (!src && bytes)) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    RCG_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || (!d_dst && bytes) || (!src && bytes)) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        RCG_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_memcpy_d2h(rcg_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
-    if (!ctx || (!dst && bytes) || (!d_src && bytes)) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    RCG_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || (!dst && bytes) || (!d_src && bytes)) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        RCG_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_synchronize(rcg_ctx* ctx) {
-    if (!ctx) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_device_copy(rcg_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
-    if (!ctx || (bytes && (!d_dst || !d_src))) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    RCG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || (bytes && (!d_dst || !d_src))) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        RCG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }

@@ -13,6 +13,8 @@
 // (reading the already-built pyramids), then the selection of jpeg/src/target.rs:37-55 is replayed per image.
 #include <hip/hip_runtime.h>
 
+#include <memory>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -91,24 +93,28 @@ int wait_side_mark(rcg_ctx* ctx) {
 }  // namespace rcg
 
 extern "C" int rcg_profile(rcg_ctx* ctx, int enable) {
-    if (!ctx) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->prof.collect();
-    if (enable) ctx->prof.totals.clear();
-    ctx->prof.on = enable != 0;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->prof.collect();
+        if (enable) ctx->prof.totals.clear();
+        ctx->prof.on = enable != 0;
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_ms, int* launches) {
-    if (!ctx || !kernel) return RCG_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->prof.collect();
-    auto it = ctx->prof.totals.find(kernel);
-    if (total_ms) *total_ms = it == ctx->prof.totals.end() ? 0.0 : it->second.first;
-    if (launches) *launches = it == ctx->prof.totals.end() ? 0 : it->second.second;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !kernel) return RCG_INVALID;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->prof.collect();
+        auto it = ctx->prof.totals.find(kernel);
+        if (total_ms) *total_ms = it == ctx->prof.totals.end() ? 0.0 : it->second.first;
+        if (launches) *launches = it == ctx->prof.totals.end() ? 0 : it->second.second;
+        return RCG_OK;
+    });
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -688,12 +694,12 @@ int finish_items(rcg_ctx* ctx, ScoreState& S, Item* items, int n) {
 // settling of the scores inside the guard band of each item's target. The arena must already hold
 // score_items_bytes() beyond what the caller took.
 //
-// The encode is launched as soon as its own descriptors are uploaded; the host stages the SSIMULACRA2 descriptors
-// (tens of microseconds for one large image) while it runs. They go up on the side stream ahead of the reference
-// launches, which every candidate launch waits for (the scale-0 ones through the side stream's mark, the rest
-// through the join), or, when no reference state is built, on the context's stream behind the encode. The side
-// stream is forked before the encode launch, so the reference launches do not wait for the encode.
-int score_items(rcg_ctx* ctx, Item* items, int n) {
+// Calls of 2^25 candidate-pixels or more upload their descriptors in two parts. The encode and reference-launch
+// descriptors go first, and the encode is launched (and the reference launches enqueued on the side stream, forked
+// before the encode so that they do not wait for it) as soon as they are up. The host then stages the candidate
+// launches' descriptors (tens of microseconds for one large image) while those run, and uploads them on the
+// context's stream behind the encode, which the candidate launches follow.
+static int score_items_impl(rcg_ctx* ctx, Item* items, int n) {
     ScoreState S;
     // Below ~2^25 candidate-pixels the staging is short and the second upload costs more than it hides (a plain
     // search over images up to 3000x3000, one candidate per step, measured 3% slower with it): one upload.
@@ -713,24 +719,120 @@ int score_items(rcg_ctx* ctx, Item* items, int n) {
     const bool refs = !refs_sp.refs.empty();
     if (refs) RCG_TRY(fork_side(ctx));
     if (S.nej) RCG_TRY(launch_encode(ctx, S.d_ej, S.d_eb, S.nej, S.etiles));
-    // From here on work is in flight: a failure waits for both streams before it returns, so that no launch of this
-    // call still reads the caller's buffers afterwards.
-    auto rest = [&]() -> int {
-        if (refs) RCG_TRY(launch_refs_side(ctx, refs_sp));
-        S.fp32 = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32, kStageScales);
-        S.fp32.refs = refs_sp.refs;
-        S.fp32_desc_end = S.D.off;
-        RCG_TRY(S.D.upload_range(first_end, S.D.off, ctx->stream));
-        return enqueue_after_encode(ctx, S, refs, false);
-    };
-    const int rc = rest();
+    if (refs) RCG_TRY(launch_refs_side(ctx, refs_sp));
+    S.fp32 = stage_pairs(S.D, items, n, S.bufs, S.pairs, RCG_PREC_FP32, S.d_s32, kStageScales);
+    S.fp32.refs = refs_sp.refs;
+    S.fp32_desc_end = S.D.off;
+    RCG_TRY(S.D.upload_range(first_end, S.D.off, ctx->stream));
+    RCG_TRY(enqueue_after_encode(ctx, S, refs, false));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    return finish_items(ctx, S, items, n);
+}
+
+// A failure part way leaves work in flight on either stream: it is waited for before the error returns, so that no
+// launch of this call still reads or writes the caller's buffers (a server reuses a failed request's blocks).
+int score_items(rcg_ctx* ctx, Item* items, int n) {
+    int rc;
+    try {
+        rc = score_items_impl(ctx, items, n);
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamSynchronize(ctx->stream);
+        throw;
+    }
     if (rc != RCG_OK) {
         (void)hipStreamSynchronize(ctx->side);
         (void)hipStreamSynchronize(ctx->stream);
-        return rc;
     }
+    return rc;
+}
+
+// Coefficient bytes of each component of the largest grid among an item's candidates.
+static void winner_grid_bytes(const Item& it, size_t out[3]) {
+    out[0] = out[1] = out[2] = 0;
+    for (uint32_t c = 0; c < it.ncand; c++) {
+        const Layout L = make_layout(it.w, it.h, it.ch, it.quality[c], it.subsampling[c]);
+        for (uint32_t k = 0; k < L.ncomp; k++) out[k] = std::max(out[k], L.coef_count(int(k)) * sizeof(int16_t));
+    }
+}
+
+size_t winner_pinned(int n) {
+    return size_t(n) * (sizeof(EncCand) + sizeof(EncJob) + sizeof(uint32_t) + 3 * 64) + 4096;
+}
+
+size_t winner_bytes(const Item* items, const int* which, int n, bool host_dest) {
+    size_t b = Arena::align(winner_pinned(n));
+    if (host_dest)
+        for (int k = 0; k < n; k++) {
+            size_t g[3];
+            winner_grid_bytes(items[which[k]], g);
+            for (int c = 0; c < 3; c++) b += Arena::align(g[c]);
+        }
+    return b;
+}
+
+int encode_winners(rcg_ctx* ctx, Arena& A, Pinned& pin, const Item* items, const int* which, const int* chosen,
+                   const WinnerDest* dest, int n, bool host_dest) {
+    if (n <= 0) return RCG_OK;
+    const size_t dbytes = winner_pinned(n);
+    RCG_TRY(pin.ensure(dbytes));
+    Desc D;
+    D.host = pin.base;
+    D.dev = static_cast<char*>(A.take(dbytes));
+    D.cap = dbytes;
+    D.off = 0;
+    std::vector<EncJob> ej(static_cast<size_t>(n));
+    std::vector<uint32_t> begins(static_cast<size_t>(n));
+    std::vector<Layout> lay(static_cast<size_t>(n));
+    std::vector<WinnerDest> dev(static_cast<size_t>(n));
+    uint32_t tiles = 0;
+    for (int k = 0; k < n; k++) {
+        const Item& it = items[which[k]];
+        const int q = it.quality[chosen[k]];
+        lay[k] = make_layout(it.w, it.h, it.ch, q, it.subsampling[chosen[k]]);
+        const Layout& L = lay[k];
+        for (int c = 0; c < 3; c++) {
+            dev[k].coefs[c] = nullptr;
+            if (uint32_t(c) >= L.ncomp) continue;
+            dev[k].coefs[c] = host_dest ? static_cast<int16_t*>(A.take(L.coef_count(c) * sizeof(int16_t)))
+                                        : dest[k].coefs[c];
+        }
+        EncCand cand;
+        std::memset(&cand, 0, sizeof(cand));
+        uint8_t qy[64], qc[64];
+        quant_tables(q, qy, qc);
+        fill_cand_tables(qy, qc, cand);
+        for (int c = 0; c < 3; c++) cand.coef[c] = dev[k].coefs[c];
+        EncJob& e = ej[k];
+        std::memset(&e, 0, sizeof(e));
+        e.src = it.d_src;
+        e.src_bytes = uint64_t(it.w) * it.h * it.ch;
+        e.w = it.w;
+        e.h = it.h;
+        e.channels = it.ch;
+        e.hmax = L.hmax;
+        e.vmax = L.vmax;
+        e.pw = L.pw;
+        e.ph = L.ph;
+        e.tiles_x = (e.pw + kEncTileW - 1) / kEncTileW;
+        e.tiles_y = e.ph / (8 * e.vmax);
+        e.ncand = 1;
+        e.cand = D.put(&cand, 1);
+        e.tile_begin = tiles;
+        begins[k] = tiles;
+        tiles += e.tiles_x * e.tiles_y;
+    }
+    const EncJob* d_ej = D.put(ej.data(), ej.size());
+    const uint32_t* d_eb = D.put(begins.data(), begins.size());
+    RCG_TRY(D.upload(ctx->stream));
+    RCG_TRY(launch_encode(ctx, d_ej, d_eb, n, tiles));
+    if (host_dest)
+        for (int k = 0; k < n; k++)
+            for (uint32_t c = 0; c < lay[k].ncomp; c++)
+                RCG_HIP(hipMemcpyAsync(dest[k].coefs[c], dev[k].coefs[c], lay[k].coef_count(int(c)) * sizeof(int16_t),
+                                       hipMemcpyDeviceToHost, ctx->stream));
     RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return finish_items(ctx, S, items, n);
+    return RCG_OK;
 }
 
 // Selection of jpeg/src/target.rs:37-55 over a candidate list sorted by quality: the highest first, then bisection
@@ -768,6 +870,73 @@ int select_candidate(const int* quality, const double* scores, int n, double tar
 
 namespace {
 
+// Validate one batch job into an Item; returns its status.
+int job_item(const rcg_batch_job& j, bool device_winner, Item& it) {
+    int st = check_image(j.width, j.height, j.channels);
+    if (st == RCG_OK && (j.width < 8 || j.height < 8)) st = RCG_UNSUPPORTED;
+    if (st == RCG_OK && (j.ncand == 0 || j.ncand > RCG_MAX_CAND || !j.pixels)) st = RCG_INVALID;
+    if (st == RCG_OK && j.winner[0]) {
+        const uint32_t ncomp = j.channels == 1 ? 1 : 3;
+        for (uint32_t c = 0; c < ncomp; c++)
+            if (!j.winner[c] || (device_winner && (reinterpret_cast<uintptr_t>(j.winner[c]) & 15))) {
+                set_error("winner buffers: one per component, 16-byte aligned on the device");
+                st = RCG_INVALID;
+            }
+    }
+    if (st != RCG_OK) return st;
+    std::memset(&it, 0, sizeof(it));
+    it.d_src = j.pixels;
+    it.w = j.width;
+    it.h = j.height;
+    it.ch = j.channels;
+    it.ncand = j.ncand;
+    for (uint32_t c = 0; c < j.ncand; c++) {
+        it.quality[c] = j.quality[c];
+        it.subsampling[c] = j.subsampling[c];
+    }
+    it.target = j.target;
+    return RCG_OK;
+}
+
+// The jobs (by item index) that asked for their winner's coefficients.
+void winner_items(const rcg_batch_job* jobs, const std::vector<int>& job_of, std::vector<int>& which,
+                  std::vector<WinnerDest>& dest) {
+    which.clear();
+    dest.clear();
+    for (size_t k = 0; k < job_of.size(); k++) {
+        const rcg_batch_job& j = jobs[job_of[k]];
+        if (!j.winner[0]) continue;
+        which.push_back(int(k));
+        WinnerDest d;
+        for (int c = 0; c < 3; c++) d.coefs[c] = j.winner[c];
+        dest.push_back(d);
+    }
+}
+
+// Selection per item into the results, then the winners' coefficients for the jobs that asked.
+int report_items(rcg_ctx* ctx, Arena& A, Pinned& pin, const Item* items, const std::vector<int>& job_of,
+                 const std::vector<int>& which, const std::vector<WinnerDest>& dest, bool host_dest,
+                 rcg_batch_result* results) {
+    for (size_t k = 0; k < job_of.size(); k++) {
+        const Item& it = items[k];
+        rcg_batch_result& r = results[job_of[k]];
+        for (uint32_t c = 0; c < it.ncand; c++) {
+            r.scores[c] = it.scores[c];
+            r.settled[c] = it.settled[c];
+        }
+        r.chosen_index = select_candidate(it.quality, it.scores, int(it.ncand), it.target, &r.sweep_index);
+        r.chosen_quality = it.quality[r.chosen_index];
+        r.sweep_quality = it.quality[r.sweep_index];
+    }
+    if (which.empty()) return RCG_OK;
+    std::vector<int> chosen(which.size());
+    for (size_t k = 0; k < which.size(); k++) chosen[k] = results[job_of[which[k]]].chosen_index;
+    RCG_TRY(encode_winners(ctx, A, pin, items, which.data(), chosen.data(), dest.data(), int(which.size()),
+                           host_dest));
+    for (int k : which) results[job_of[k]].winner_written = 1;
+    return RCG_OK;
+}
+
 int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results, bool host_src) {
     if (!ctx || (!jobs && njobs) || (!results && njobs) || njobs < 0) return RCG_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -780,29 +949,19 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
         rcg_batch_result& r = results[i];
         std::memset(&r, 0, sizeof(r));
         r.chosen_index = r.sweep_index = -1;
-        int st = check_image(j.width, j.height, j.channels);
-        if (st == RCG_OK && (j.width < 8 || j.height < 8)) st = RCG_UNSUPPORTED;
-        if (st == RCG_OK && (j.ncand == 0 || j.ncand > RCG_MAX_CAND || !j.pixels)) st = RCG_INVALID;
-        r.status = st;
-        if (st != RCG_OK) continue;
         Item it;
-        std::memset(&it, 0, sizeof(it));
-        it.d_src = j.pixels;
-        it.w = j.width;
-        it.h = j.height;
-        it.ch = j.channels;
-        it.ncand = j.ncand;
-        for (uint32_t c = 0; c < j.ncand; c++) {
-            it.quality[c] = j.quality[c];
-            it.subsampling[c] = j.subsampling[c];
-        }
-        it.target = j.target;
+        r.status = job_item(j, !host_src, it);
+        if (r.status != RCG_OK) continue;
         items.push_back(it);
         job_of.push_back(i);
         if (host_src) own += Arena::align(size_t(j.width) * j.height * j.channels);
         own += ref_state_bytes(plan_pair(j.width, j.height, ctx->tune32));
     }
-    const size_t need = own + score_items_bytes(ctx, items.data(), int(items.size()));
+    std::vector<int> which;
+    std::vector<WinnerDest> dest;
+    winner_items(jobs, job_of, which, dest);
+    const size_t need = own + score_items_bytes(ctx, items.data(), int(items.size())) +
+                        winner_bytes(items.data(), which.data(), int(which.size()), host_src);
     ctx->arena.reset();
     RCG_TRY(ctx->arena.ensure(need));
     for (size_t k = 0; k < items.size(); k++) {
@@ -817,28 +976,21 @@ int score_batch_impl(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_bat
         it.ref_ready = false;
     }
     RCG_TRY(score_items(ctx, items.data(), int(items.size())));
-    for (size_t k = 0; k < items.size(); k++) {
-        const Item& it = items[k];
-        rcg_batch_result& r = results[job_of[k]];
-        for (uint32_t c = 0; c < it.ncand; c++) {
-            r.scores[c] = it.scores[c];
-            r.settled[c] = it.settled[c];
-        }
-        r.chosen_index = select_candidate(it.quality, it.scores, int(it.ncand), it.target, &r.sweep_index);
-        r.chosen_quality = it.quality[r.chosen_index];
-        r.sweep_quality = it.quality[r.sweep_index];
-    }
-    return RCG_OK;
+    return report_items(ctx, ctx->arena, ctx->pinned, items.data(), job_of, which, dest, host_src, results);
 }
 
 }  // namespace
 
 extern "C" int rcg_score_batch_dev(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results) {
-    return score_batch_impl(ctx, jobs, njobs, results, false);
+    return rcg::guarded([&]() -> int {
+        return score_batch_impl(ctx, jobs, njobs, results, false);
+    });
 }
 
 extern "C" int rcg_score_batch(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_batch_result* results) {
-    return score_batch_impl(ctx, jobs, njobs, results, true);
+    return rcg::guarded([&]() -> int {
+        return score_batch_impl(ctx, jobs, njobs, results, true);
+    });
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -848,80 +1000,84 @@ struct rcg_plan {
     rcg_ctx* ctx = nullptr;
     Arena arena;
     Pinned pinned;
+    Pinned winner_pinned;       // staging of the winners' encode descriptors (the pass's own stays as uploaded)
+    size_t winner_mark = 0;     // arena offset where each run's winner encode takes its descriptors
     std::vector<rcg_batch_job> jobs;
     std::vector<Item> items;
     std::vector<int> job_of;
     std::vector<int> status;
+    std::vector<int> which;
+    std::vector<WinnerDest> dest;
     ScoreState S;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // The caller has waited for every use of the plan's memory on the device.
+    ~rcg_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (arena.base) (void)hipFree(arena.base);
+        if (pinned.base) (void)hipHostFree(pinned.base);
+        if (winner_pinned.base) (void)hipHostFree(winner_pinned.base);
+    }
 };
 
 extern "C" void rcg_batch_plan_destroy(rcg_plan* p) {
-    if (!p) return;
-    if (p->ctx) {
-        std::lock_guard<std::mutex> lock(p->ctx->mu);
-        (void)hipSetDevice(p->ctx->device);
-        (void)hipStreamSynchronize(p->ctx->stream);
-    }
-    if (p->exec) (void)hipGraphExecDestroy(p->exec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
-    if (p->arena.base) (void)hipFree(p->arena.base);
-    if (p->pinned.base) (void)hipHostFree(p->pinned.base);
-    delete p;
+    rcg::guarded_void([&] {
+        if (!p) return;
+        {
+            std::lock_guard<std::mutex> lock(p->ctx->mu);
+            (void)hipSetDevice(p->ctx->device);
+            (void)hipStreamSynchronize(p->ctx->stream);
+            (void)hipStreamSynchronize(p->ctx->side);
+        }
+        delete p;
+    });
 }
 
 extern "C" int rcg_batch_plan_create(rcg_ctx* ctx, const rcg_batch_job* jobs, int njobs, rcg_plan** out) {
-    if (!ctx || !out || njobs < 0 || (!jobs && njobs)) return RCG_INVALID;
-    *out = nullptr;
-    rcg_plan* p = new (std::nothrow) rcg_plan();
-    if (!p) return RCG_OOM;
-    p->ctx = ctx;
-    p->jobs.assign(jobs, jobs + njobs);
-    p->status.assign(njobs, RCG_OK);
-    size_t own = 0;
-    for (int i = 0; i < njobs; i++) {
-        const rcg_batch_job& j = jobs[i];
-        int st = check_image(j.width, j.height, j.channels);
-        if (st == RCG_OK && (j.width < 8 || j.height < 8)) st = RCG_UNSUPPORTED;
-        if (st == RCG_OK && (j.ncand == 0 || j.ncand > RCG_MAX_CAND || !j.pixels)) st = RCG_INVALID;
-        p->status[i] = st;
-        if (st != RCG_OK) continue;
-        Item it;
-        std::memset(&it, 0, sizeof(it));
-        it.d_src = j.pixels;
-        it.w = j.width;
-        it.h = j.height;
-        it.ch = j.channels;
-        it.ncand = j.ncand;
-        for (uint32_t c = 0; c < j.ncand; c++) {
-            it.quality[c] = j.quality[c];
-            it.subsampling[c] = j.subsampling[c];
+    return rcg::guarded([&]() -> int {
+        if (!ctx || !out || njobs < 0 || (!jobs && njobs)) return RCG_INVALID;
+        *out = nullptr;
+        std::unique_ptr<rcg_plan> p(new rcg_plan());
+        p->ctx = ctx;
+        p->jobs.assign(jobs, jobs + njobs);
+        p->status.assign(njobs, RCG_OK);
+        size_t own = 0;
+        for (int i = 0; i < njobs; i++) {
+            Item it;
+            p->status[i] = job_item(jobs[i], true, it);
+            if (p->status[i] != RCG_OK) continue;
+            p->items.push_back(it);
+            p->job_of.push_back(i);
+            own += ref_state_bytes(plan_pair(jobs[i].width, jobs[i].height, ctx->tune32));
         }
-        it.target = j.target;
-        p->items.push_back(it);
-        p->job_of.push_back(i);
-        own += ref_state_bytes(plan_pair(j.width, j.height, ctx->tune32));
-    }
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    int rc = RCG_OK;
-    {
-        hipError_t e = hipSetDevice(ctx->device);
-        if (e != hipSuccess) rc = RCG_DEVICE;
-    }
-    if (rc == RCG_OK) rc = p->arena.ensure(own + score_items_bytes(ctx, p->items.data(), int(p->items.size())));
-    if (rc == RCG_OK) {
+        winner_items(jobs, p->job_of, p->which, p->dest);
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        // Whatever fails below, nothing of this plan is left running on the device when it is freed.
+        struct Drain {
+            rcg_ctx* c;
+            bool armed = true;
+            ~Drain() {
+                if (armed) {
+                    (void)hipStreamSynchronize(c->stream);
+                    (void)hipStreamSynchronize(c->side);
+                }
+            }
+        } drain{ctx};
+        RCG_TRY(p->arena.ensure(own + score_items_bytes(ctx, p->items.data(), int(p->items.size())) +
+                                winner_bytes(p->items.data(), p->which.data(), int(p->which.size()), false)));
         for (Item& it : p->items) {
             take_ref_state(p->arena, plan_pair(it.w, it.h, ctx->tune32), it.ref);
             it.ref_ready = false;
         }
-        rc = prepare_items(ctx, p->arena, p->pinned, p->items.data(), int(p->items.size()), p->S);
-    }
-    if (rc == RCG_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RCG_DEVICE;
-    if (rc == RCG_OK) {
+        RCG_TRY(prepare_items(ctx, p->arena, p->pinned, p->items.data(), int(p->items.size()), p->S));
+        p->winner_mark = p->arena.used;
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
         // Capture the FP32 launch sequence (kernels and one device-to-host copy; no memsets, no allocation).
         const bool prof = ctx->prof.on;
         ctx->prof.on = false;
+        int rc = RCG_OK;
         hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             rc = enqueue_items(ctx, p->S);
@@ -935,57 +1091,33 @@ extern "C" int rcg_batch_plan_create(rcg_ctx* ctx, const rcg_batch_job* jobs, in
             rc = RCG_DEVICE;
         }
         ctx->prof.on = prof;
-        if (rc == RCG_OK) {
-            e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-                rc = RCG_DEVICE;
-            }
-        }
-    }
-    if (rc != RCG_OK) {
-        p->ctx = nullptr;  // the lock is held: destroy without re-locking
-        if (p->exec) (void)hipGraphExecDestroy(p->exec);
-        if (p->graph) (void)hipGraphDestroy(p->graph);
-        p->exec = nullptr;
-        p->graph = nullptr;
-        if (p->arena.base) (void)hipFree(p->arena.base);
-        if (p->pinned.base) (void)hipHostFree(p->pinned.base);
-        p->arena.base = nullptr;
-        p->pinned.base = nullptr;
-        delete p;
-        return rc;
-    }
-    *out = p;
-    return RCG_OK;
+        RCG_TRY(rc);
+        RCG_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+        drain.armed = false;
+        *out = p.release();
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_batch_plan_run(rcg_plan* p, rcg_batch_result* results) {
-    if (!p || (!results && !p->jobs.empty())) return RCG_INVALID;
-    rcg_ctx* ctx = p->ctx;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    RCG_HIP(hipSetDevice(ctx->device));
-    if (p->exec) {
-        ProfScope ps(ctx, "batch_graph");
-        RCG_HIP(hipGraphLaunch(p->exec, ctx->stream));
-    }
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    RCG_TRY(finish_items(ctx, p->S, p->items.data(), int(p->items.size())));
-    for (size_t i = 0; i < p->jobs.size(); i++) {
-        std::memset(&results[i], 0, sizeof(rcg_batch_result));
-        results[i].status = p->status[i];
-        results[i].chosen_index = results[i].sweep_index = -1;
-    }
-    for (size_t k = 0; k < p->items.size(); k++) {
-        const Item& it = p->items[k];
-        rcg_batch_result& r = results[p->job_of[k]];
-        for (uint32_t c = 0; c < it.ncand; c++) {
-            r.scores[c] = it.scores[c];
-            r.settled[c] = it.settled[c];
+    return rcg::guarded([&]() -> int {
+        if (!p || (!results && !p->jobs.empty())) return RCG_INVALID;
+        rcg_ctx* ctx = p->ctx;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        RCG_HIP(hipSetDevice(ctx->device));
+        if (p->exec) {
+            ProfScope ps(ctx, "batch_graph");
+            RCG_HIP(hipGraphLaunch(p->exec, ctx->stream));
         }
-        r.chosen_index = select_candidate(it.quality, it.scores, int(it.ncand), it.target, &r.sweep_index);
-        r.chosen_quality = it.quality[r.chosen_index];
-        r.sweep_quality = it.quality[r.sweep_index];
-    }
-    return RCG_OK;
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        RCG_TRY(finish_items(ctx, p->S, p->items.data(), int(p->items.size())));
+        for (size_t i = 0; i < p->jobs.size(); i++) {
+            std::memset(&results[i], 0, sizeof(rcg_batch_result));
+            results[i].status = p->status[i];
+            results[i].chosen_index = results[i].sweep_index = -1;
+        }
+        p->arena.used = p->winner_mark;
+        return report_items(ctx, p->arena, p->winner_pinned, p->items.data(), p->job_of, p->which, p->dest, false,
+                            results);
+    });
 }

@@ -43,46 +43,48 @@ int noalloc_setup(int device) {
 using namespace rcg;
 
 extern "C" int rcg_reconstruct_dev_noalloc(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pixels) {
-    if (!d_pixels) return RCG_INVALID;
-    uint32_t hmax, vmax;
-    int rgb_direct;
-    RCG_TRY(check_coef_image(d_ci, hmax, vmax, rgb_direct));
-    const size_t need = sample_plane_bytes(d_ci);
-    if (need > RCG_NOALLOC_SCRATCH_BYTES) {
-        set_error("the image's sample planes take " + std::to_string(need) + " bytes, more than the " +
-                  std::to_string(RCG_NOALLOC_SCRATCH_BYTES) + "-byte allocation-free scratch; use rcg_reconstruct_dev");
-        return RCG_UNSUPPORTED;
-    }
-    const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (uint32_t c = 0; c < d_ci->ncomp; c++) {
-        if (reinterpret_cast<uintptr_t>(d_ci->comps[c].coefs) & 15) {
-            set_error("coefficient buffers must be 16-byte aligned device pointers");
-            return RCG_INVALID;
+    return rcg::guarded([&]() -> int {
+        if (!d_pixels) return RCG_INVALID;
+        uint32_t hmax, vmax;
+        int rgb_direct;
+        RCG_TRY(check_coef_image(d_ci, hmax, vmax, rgb_direct));
+        const size_t need = sample_plane_bytes(d_ci);
+        if (need > RCG_NOALLOC_SCRATCH_BYTES) {
+            set_error("the image's sample planes take " + std::to_string(need) + " bytes, more than the " +
+                      std::to_string(RCG_NOALLOC_SCRATCH_BYTES) + "-byte allocation-free scratch; use rcg_reconstruct_dev");
+            return RCG_UNSUPPORTED;
         }
-        d_co[c] = d_ci->comps[c].coefs;
-    }
-    if (!ctx) return RCG_INVALID;
-    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch[ctx->device]) {
-        set_error("allocation-free scratch is not set up on this device");
-        return RCG_DEVICE;
-    }
-    std::lock_guard<std::mutex> ctx_lock(ctx->mu);
-    std::lock_guard<std::mutex> dev_lock(g_device_mu[ctx->device]);
-    RCG_HIP(hipSetDevice(ctx->device));
-    uint8_t* base = static_cast<uint8_t*>(g_scratch[ctx->device]);
-    uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t off = 0;
-    for (uint32_t c = 0; c < d_ci->ncomp; c++) {
-        planes[c] = base + off;
-        off += Arena::align(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64);
-    }
-    // Profiling would create HIP events; this call records none.
-    struct Suspend {
-        rcg_ctx* c;
-        explicit Suspend(rcg_ctx* x) : c(x) { c->prof_suspended = true; }
-        ~Suspend() { c->prof_suspended = false; }
-    } suspend(ctx);
-    RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
-    RCG_HIP(hipStreamSynchronize(ctx->stream));
-    return RCG_OK;
+        const int16_t* d_co[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t c = 0; c < d_ci->ncomp; c++) {
+            if (reinterpret_cast<uintptr_t>(d_ci->comps[c].coefs) & 15) {
+                set_error("coefficient buffers must be 16-byte aligned device pointers");
+                return RCG_INVALID;
+            }
+            d_co[c] = d_ci->comps[c].coefs;
+        }
+        if (!ctx) return RCG_INVALID;
+        if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch[ctx->device]) {
+            set_error("allocation-free scratch is not set up on this device");
+            return RCG_DEVICE;
+        }
+        std::lock_guard<std::mutex> ctx_lock(ctx->mu);
+        std::lock_guard<std::mutex> dev_lock(g_device_mu[ctx->device]);
+        RCG_HIP(hipSetDevice(ctx->device));
+        uint8_t* base = static_cast<uint8_t*>(g_scratch[ctx->device]);
+        uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
+        size_t off = 0;
+        for (uint32_t c = 0; c < d_ci->ncomp; c++) {
+            planes[c] = base + off;
+            off += Arena::align(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64);
+        }
+        // Profiling would create HIP events; this call records none.
+        struct Suspend {
+            rcg_ctx* c;
+            explicit Suspend(rcg_ctx* x) : c(x) { c->prof_suspended = true; }
+            ~Suspend() { c->prof_suspended = false; }
+        } suspend(ctx);
+        RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        return RCG_OK;
+    });
 }

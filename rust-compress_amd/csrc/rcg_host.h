@@ -5,7 +5,9 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <exception>
 #include <map>
+#include <new>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -15,7 +17,34 @@
 
 namespace rcg {
 
-void set_error(const std::string& msg);
+// Message of the last failure on the calling thread (rcg_last_error). Never throws: without memory for the message
+// it keeps a fixed one.
+void set_error(const std::string& msg) noexcept;
+
+// The exception barrier of every exported entry point: the ABI never unwinds. Host allocation failure becomes
+// RCG_OOM, any other exception RCG_DEVICE, each with a message.
+template <class F>
+int guarded(F&& body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return RCG_OOM;
+    } catch (const std::exception& e) {
+        set_error(std::string("internal error: ") + e.what());
+        return RCG_DEVICE;
+    } catch (...) {
+        set_error("internal error");
+        return RCG_DEVICE;
+    }
+}
+template <class F>
+void guarded_void(F&& body) noexcept {
+    (void)guarded([&]() -> int {
+        body();
+        return RCG_OK;
+    });
+}
 
 #define RCG_HIP(expr)                                                                                  \
     do {                                                                                               \
@@ -296,6 +325,19 @@ struct Item {
 };
 size_t score_items_bytes(rcg_ctx* ctx, const Item* items, int n);
 int score_items(rcg_ctx* ctx, Item* items, int n);
+
+// The chosen candidate's coefficients of some items (Found.coefs, jpeg/src/target.rs:11-16): build_coefs at that
+// candidate's quality and subsampling, encoded once more in one launch over the items. dest[k] receives item
+// which[k]'s candidate chosen[k]: device pointers written by the kernel (16-byte aligned) when host_dest is false,
+// else host pointers filled by copies from arena buffers. winner_bytes() is the arena space encode_winners() takes
+// and winner_pinned() the pinned staging it needs. Synchronous.
+struct WinnerDest {
+    int16_t* coefs[3];
+};
+size_t winner_bytes(const Item* items, const int* which, int n, bool host_dest);
+size_t winner_pinned(int n);
+int encode_winners(rcg_ctx* ctx, Arena& A, Pinned& pin, const Item* items, const int* which, const int* chosen,
+                   const WinnerDest* dest, int n, bool host_dest);
 int select_candidate(const int* quality, const double* scores, int n, double target, int* sweep);
 
 #define RCG_TRY(expr)                    \

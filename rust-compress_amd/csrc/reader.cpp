@@ -700,11 +700,15 @@ int parse(const uint8_t* data, size_t len, bool decode, int16_t* const* out, rcg
 }  // namespace
 
 extern "C" int rcg_jpeg_parse(const uint8_t* data, size_t len, rcg_jpeg_info* info) {
-    if (!info) return RCG_INVALID;
-    return parse(data, len, false, nullptr, info);
+    return rcg::guarded([&]() -> int {
+        if (!info) return RCG_INVALID;
+        return parse(data, len, false, nullptr, info);
+    });
 }
 
 extern "C" int rcg_jpeg_read_coefs(const uint8_t* data, size_t len, int16_t* coefs[4]) {
-    if (!coefs) return RCG_INVALID;
-    return parse(data, len, true, coefs, nullptr);
+    return rcg::guarded([&]() -> int {
+        if (!coefs) return RCG_INVALID;
+        return parse(data, len, true, coefs, nullptr);
+    });
 }

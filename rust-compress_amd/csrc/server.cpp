@@ -17,6 +17,8 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -102,7 +104,15 @@ struct Request {
     Search search;
     int steps = 0, ticks = 0;
     int status = RCG_OK;
+    int coefs_written = 0;
 };
+
+// Fault injection for tests: with RCG_TEST_THROW_TICK=<n> in the environment, the worker's n-th tick (counting from
+// 1) throws std::bad_alloc once, as a host allocation inside the tick would.
+int test_throw_tick() {
+    const char* v = std::getenv("RCG_TEST_THROW_TICK");
+    return v ? std::atoi(v) : 0;
+}
 
 }  // namespace
 
@@ -117,17 +127,29 @@ struct rcg_server {
     size_t used = 0;
     // Device blocks of finished requests, kept for reuse by size class: hipMalloc and hipFree of blocks of many
     // megabytes take milliseconds each, and on the worker thread they stalled the ticks (a sustained 200 requests/s
-    // spent two thirds of its time there). Touched by the worker thread only; at most hbm_budget bytes are kept.
+    // spent two thirds of its time there). Touched by the worker thread only (spare_bytes also read under mu by
+    // admit()). Admitted footprints plus kept blocks stay within hbm_budget.
     std::multimap<size_t, void*> spare;
     size_t spare_bytes = 0;
     bool stop = false;
     std::thread worker;
     rcg_server_stats stats{};
     uint64_t batch_sum = 0;
+    int throw_tick = 0;
 
+    // Eighth-of-an-octave size classes (at most 12.5% slack), 1 MiB at least.
+    static size_t size_class(size_t n) {
+        if (n <= (size_t(1) << 20)) return size_t(1) << 20;
+        const int b = 63 - __builtin_clzll(uint64_t(n));
+        const size_t step = size_t(1) << (b - 3);
+        return (n + step - 1) / step * step;
+    }
+
+    // Device bytes a request holds while admitted: its resident blocks as allocated (size classes) and the largest
+    // per-tick working set of its candidates.
     size_t footprint_of(const rcg_request& r) const {
         const PairPlan P = plan_pair(r.width, r.height, ctx->tune32);
-        size_t b = Arena::align(size_t(r.width) * r.height * r.channels) + ref_state_bytes(P);
+        size_t b = size_class(size_t(r.width) * r.height * r.channels) + size_class(ref_state_bytes(P));
         // Under Auto a tick's candidates are 4:4:4 (q >= 90) or 4:2:0 (q < 90): charge the larger working set.
         size_t work = 0;
         for (int q : {100, 1}) {
@@ -146,12 +168,11 @@ struct rcg_server {
         return b + work;
     }
 
-    // Eighth-of-an-octave size classes (at most 12.5% slack), 1 MiB at least.
-    static size_t size_class(size_t n) {
-        if (n <= (size_t(1) << 20)) return size_t(1) << 20;
-        const int b = 63 - __builtin_clzll(uint64_t(n));
-        const size_t step = size_t(1) << (b - 3);
-        return (n + step - 1) / step * step;
+    void free_spares() {
+        for (auto& kv : spare) (void)hipFree(kv.second);
+        spare.clear();
+        std::lock_guard<std::mutex> lk(mu);
+        spare_bytes = 0;
     }
     int device_alloc(size_t n, void** p) {
         const size_t c = size_class(n);
@@ -159,16 +180,22 @@ struct rcg_server {
         if (it != spare.end()) {
             *p = it->second;
             spare.erase(it);
+            std::lock_guard<std::mutex> lk(mu);
             spare_bytes -= c;
             return RCG_OK;
         }
+        if (hipMalloc(p, c) == hipSuccess) return RCG_OK;
+        (void)hipGetLastError();
+        // The kept blocks of other sizes may be what stands in the way: give them back and try once more.
+        free_spares();
         RCG_HIP(hipMalloc(p, c));
         return RCG_OK;
     }
+    // A finished request's block: kept for reuse while admitted footprints and kept blocks fit the budget (mu held).
     void device_release(void* p, size_t n) {
         if (!p) return;
         const size_t c = size_class(n);
-        if (spare_bytes + c > opts.hbm_budget) {
+        if (used + spare_bytes + c > opts.hbm_budget) {
             (void)hipFree(p);
             return;
         }
@@ -176,6 +203,7 @@ struct rcg_server {
         spare_bytes += c;
     }
 
+    // mu held.
     void finish(Request* r, double t) {
         rcg_completion c{};
         c.id = r->req.id;
@@ -187,22 +215,26 @@ struct rcg_server {
         c.submit_s = r->submit_s;
         c.admit_s = r->admit_s;
         c.done_s = t;
+        c.coefs_written = r->coefs_written;
+        used -= std::min(used, r->footprint);
         device_release(r->d_src, r->src_bytes);
         device_release(r->d_ref, r->ref_bytes);
-        used -= std::min(used, r->footprint);
         done.push_back(c);
         stats.requests_done++;
         delete r;
     }
 
-    // Admit queued requests in order while they fit (mu held).
-    void admit() {
+    // Admit queued requests in order while they fit beside the kept blocks (mu held); kept blocks give way to a
+    // request that would otherwise wait (free_spares() at the next tick).
+    bool admit() {
+        bool want_room = false;
         while (!pending.empty()) {
             Request* r = pending.front();
             const size_t budget = opts.hbm_budget;
             const size_t need = std::min(r->footprint, budget);
             if (opts.max_active > 0 && int(active.size()) >= opts.max_active) break;
             if (!active.empty() && used + need > budget) break;
+            if (used + spare_bytes + need > budget && spare_bytes) want_room = true;
             pending.pop_front();
             r->admit_s = now_s();
             used += need;
@@ -210,6 +242,7 @@ struct rcg_server {
             stats.peak_footprint = std::max(stats.peak_footprint, used);
             active.push_back(r);
         }
+        return want_room;
     }
 
     // Take resident buffers and enqueue the source upload of a newly admitted request (worker thread, mu not held).
@@ -229,97 +262,189 @@ struct rcg_server {
         return RCG_OK;
     }
 
+    // One scoring pass over `items` on the worker's context.
+    int score(std::vector<Item>& items) {
+        std::lock_guard<std::mutex> clk(ctx->mu);
+        ctx->arena.reset();
+        RCG_TRY(ctx->arena.ensure(score_items_bytes(ctx, items.data(), int(items.size()))));
+        return score_items(ctx, items.data(), int(items.size()));
+    }
+
+    // The chosen quality's coefficients of the requests in `fin` that asked for them, in one encode launch.
+    int write_winners(const std::vector<Request*>& fin) {
+        std::vector<Item> items;
+        std::vector<int> which, chosen;
+        std::vector<WinnerDest> dest;
+        for (Request* r : fin) {
+            if (!r->req.coefs[0]) continue;
+            Item it;
+            std::memset(&it, 0, sizeof(it));
+            it.d_src = r->d_src;
+            it.w = r->req.width;
+            it.h = r->req.height;
+            it.ch = r->req.channels;
+            it.ncand = 1;
+            it.quality[0] = r->search.best_q;
+            it.subsampling[0] = r->req.subsampling;
+            which.push_back(int(items.size()));
+            chosen.push_back(0);
+            WinnerDest d;
+            for (int c = 0; c < 3; c++) d.coefs[c] = r->req.coefs[c];
+            dest.push_back(d);
+            items.push_back(it);
+        }
+        if (items.empty()) return RCG_OK;
+        std::lock_guard<std::mutex> clk(ctx->mu);
+        ctx->arena.reset();
+        RCG_TRY(ctx->arena.ensure(winner_bytes(items.data(), which.data(), int(which.size()), true)));
+        return encode_winners(ctx, ctx->arena, ctx->pinned, items.data(), which.data(), chosen.data(), dest.data(),
+                              int(which.size()), true);
+    }
+
+    // A tick: score the wanted candidates of every request in `batch` (the failing ones are marked), apply the
+    // scores, and finish the requests that are done. Returns the number of pairs scored.
+    size_t tick(const std::vector<Request*>& batch, const std::vector<Request*>& fresh, double& t0, double& t1) {
+        std::vector<Item> items;
+        std::vector<std::vector<int>> want(batch.size());
+        std::vector<int> item_of(batch.size(), -1);
+        for (size_t i = 0; i < batch.size(); i++) {
+            Request* r = batch[i];
+            if (r->status != RCG_OK) continue;
+            r->search.wanted(std::max(1, opts.speculate), want[i]);
+            Item it;
+            std::memset(&it, 0, sizeof(it));
+            it.d_src = r->d_src;
+            it.w = r->req.width;
+            it.h = r->req.height;
+            it.ch = r->req.channels;
+            it.ncand = uint32_t(want[i].size());
+            for (uint32_t c = 0; c < it.ncand; c++) {
+                it.quality[c] = want[i][c];
+                it.subsampling[c] = r->req.subsampling;
+            }
+            it.target = r->req.target;
+            it.ref = r->ref;
+            it.ref_ready = r->ref_ready;
+            item_of[i] = int(items.size());
+            items.push_back(it);
+        }
+        t0 = now_s();
+        std::vector<int> status(items.size(), RCG_OK);
+        if (!items.empty()) {
+            if (throw_tick > 0 && stats.ticks + 1 == uint64_t(throw_tick)) {
+                throw_tick = 0;
+                throw std::bad_alloc();
+            }
+            int rc = score(items);
+            if (rc != RCG_OK && items.size() > 1) {
+                // Isolate the failure: each request alone, so that only those that fail alone fail.
+                for (size_t k = 0; k < items.size(); k++) {
+                    std::vector<Item> one(1, items[k]);
+                    status[k] = score(one);
+                    items[k] = one[0];
+                }
+            } else if (rc != RCG_OK) {
+                status[0] = rc;
+            }
+        }
+        t1 = now_s();
+        if (!fresh.empty()) {
+            // A tick that succeeded read its results back, so the uploads ahead of it have completed; a failed one
+            // waited for its streams before returning. A tick that scored nothing may not have waited.
+            (void)hipStreamSynchronize(ctx->stream);
+            for (Request* r : fresh) std::vector<uint8_t>().swap(r->pixels);
+        }
+        std::vector<Request*> fin;
+        for (size_t i = 0; i < batch.size(); i++) {
+            Request* r = batch[i];
+            if (r->status == RCG_OK && status[item_of[i]] != RCG_OK) r->status = status[item_of[i]];
+            if (r->status == RCG_OK) {
+                const Item& it = items[item_of[i]];
+                r->ref_ready = true;
+                r->ticks++;
+                r->steps += int(it.ncand);
+                // Apply the scores in decision order: the current candidate, then whichever speculative one
+                // became current.
+                bool progressed = true;
+                while (progressed && !r->search.done) {
+                    progressed = false;
+                    for (uint32_t c = 0; c < it.ncand; c++)
+                        if (r->search.apply(it.quality[c], it.scores[c])) {
+                            progressed = true;
+                            break;
+                        }
+                }
+            }
+            if (r->status != RCG_OK || r->search.done) fin.push_back(r);
+        }
+        std::vector<Request*> ok;
+        for (Request* r : fin)
+            if (r->status == RCG_OK && r->req.coefs[0]) ok.push_back(r);
+        const int wrc = write_winners(ok);
+        for (Request* r : ok) {
+            if (wrc == RCG_OK)
+                r->coefs_written = 1;
+            else
+                r->status = wrc;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        stats.ticks++;
+        stats.busy_s += t1 - t0;
+        batch_sum += items.size();
+        for (Request* r : fin) {
+            active.erase(std::find(active.begin(), active.end(), r));
+            finish(r, t1);
+        }
+        if (stats.ticks) stats.mean_batch = double(batch_sum) / double(stats.ticks);
+        return items.size();
+    }
+
     void run() {
         RCG_HIP_VOID(hipSetDevice(ctx->device));
-        std::vector<Item> items;
-        std::vector<std::vector<int>> want;
         for (;;) {
-            std::vector<Request*> fresh;
+            std::vector<Request*> fresh, batch;
+            bool want_room = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
                 if (stop) return;
                 const size_t before = active.size();
-                admit();
+                want_room = admit();
                 for (size_t i = before; i < active.size(); i++) fresh.push_back(active[i]);
-            }
-            for (Request* r : fresh)
-                if (r->d_src == nullptr && materialise(r) != RCG_OK) r->status = RCG_DEVICE;
-            // One tick over every active request (the worker is the only writer of `active`'s entries).
-            std::vector<Request*> batch;
-            {
-                std::lock_guard<std::mutex> lk(mu);
                 batch = active;
             }
-            items.clear();
-            want.assign(batch.size(), {});
-            std::vector<int> item_of(batch.size(), -1);
-            for (size_t i = 0; i < batch.size(); i++) {
-                Request* r = batch[i];
-                if (r->status != RCG_OK) continue;
-                r->search.wanted(std::max(1, opts.speculate), want[i]);
-                Item it;
-                std::memset(&it, 0, sizeof(it));
-                it.d_src = r->d_src;
-                it.w = r->req.width;
-                it.h = r->req.height;
-                it.ch = r->req.channels;
-                it.ncand = uint32_t(want[i].size());
-                for (uint32_t c = 0; c < it.ncand; c++) {
-                    it.quality[c] = want[i][c];
-                    it.subsampling[c] = r->req.subsampling;
-                }
-                it.target = r->req.target;
-                it.ref = r->ref;
-                it.ref_ready = r->ref_ready;
-                item_of[i] = int(items.size());
-                items.push_back(it);
-            }
-            const double t0 = now_s();
-            int rc = RCG_OK;
-            if (!items.empty()) {
-                std::lock_guard<std::mutex> clk(ctx->mu);
-                ctx->arena.reset();
-                rc = ctx->arena.ensure(score_items_bytes(ctx, items.data(), int(items.size())));
-                if (rc == RCG_OK) rc = score_items(ctx, items.data(), int(items.size()));
-            }
-            const double t1 = now_s();
-            if (!fresh.empty()) {
-                // A tick that succeeded read its results back, so the uploads ahead of it have completed. A tick that
-                // scored nothing (every fresh request failed to materialise) or failed part way may not have waited.
-                if (items.empty() || rc != RCG_OK) (void)hipStreamSynchronize(ctx->stream);
-                for (Request* r : fresh) std::vector<uint8_t>().swap(r->pixels);
-            }
-            std::lock_guard<std::mutex> lk(mu);
-            stats.ticks++;
-            stats.busy_s += t1 - t0;
-            batch_sum += items.size();
-            for (size_t i = 0; i < batch.size(); i++) {
-                Request* r = batch[i];
-                if (r->status == RCG_OK && rc != RCG_OK) r->status = rc;
-                if (r->status == RCG_OK) {
-                    const Item& it = items[item_of[i]];
-                    r->ref_ready = true;
-                    r->ticks++;
-                    r->steps += int(it.ncand);
-                    // Apply the scores in decision order: the current candidate, then whichever speculative one
-                    // became current.
-                    bool progressed = true;
-                    while (progressed && !r->search.done) {
-                        progressed = false;
-                        for (uint32_t c = 0; c < it.ncand; c++)
-                            if (r->search.apply(it.quality[c], it.scores[c])) {
-                                progressed = true;
-                                break;
-                            }
+            if (want_room) free_spares();
+            double t0 = now_s(), t1 = t0;
+            try {
+                for (Request* r : fresh)
+                    if (r->d_src == nullptr) {
+                        const int rc = materialise(r);
+                        if (rc != RCG_OK) r->status = rc;
                     }
-                }
-                if (r->status != RCG_OK || r->search.done) {
+                tick(batch, fresh, t0, t1);
+            } catch (...) {
+                // Nothing may leave the worker: the requests of this tick fail, and the loop goes on.
+                const int st = [] {
+                    try {
+                        throw;
+                    } catch (const std::bad_alloc&) {
+                        return int(RCG_OOM);
+                    } catch (...) {
+                        return int(RCG_DEVICE);
+                    }
+                }();
+                set_error(st == RCG_OOM ? "out of host memory in a serving tick" : "internal error in a serving tick");
+                (void)hipStreamSynchronize(ctx->side);
+                (void)hipStreamSynchronize(ctx->stream);
+                for (Request* r : fresh) std::vector<uint8_t>().swap(r->pixels);
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request* r : batch) {
+                    if (std::find(active.begin(), active.end(), r) == active.end()) continue;
+                    if (r->status == RCG_OK) r->status = st;
                     active.erase(std::find(active.begin(), active.end(), r));
-                    finish(r, t1);
+                    finish(r, now_s());
                 }
             }
-            if (stats.ticks) stats.mean_batch = double(batch_sum) / double(stats.ticks);
             cv_done.notify_all();
         }
     }
@@ -328,97 +453,110 @@ struct rcg_server {
 extern "C" {
 
 int rcg_server_create(int device, const rcg_server_opts* opts, rcg_server** out) {
-    if (!out) return RCG_INVALID;
-    *out = nullptr;
-    rcg_server* s = new (std::nothrow) rcg_server();
-    if (!s) return RCG_OOM;
-    if (opts) s->opts = *opts;
-    if (s->opts.hbm_budget == 0) s->opts.hbm_budget = size_t(16) << 30;
-    if (s->opts.speculate != 3) s->opts.speculate = 1;
-    int rc = rcg_create(device, &s->ctx);
-    if (rc != RCG_OK) {
-        delete s;
-        return rc;
-    }
-    if (s->opts.tune) {
-        rc = rcg_set_tune(s->ctx, s->opts.tune);
-        if (rc != RCG_OK) {
-            rcg_destroy(s->ctx);
-            delete s;
-            return rc;
+    return rcg::guarded([&]() -> int {
+        if (!out) return RCG_INVALID;
+        *out = nullptr;
+        std::unique_ptr<rcg_server> s(new rcg_server());
+        if (opts) s->opts = *opts;
+        if (s->opts.hbm_budget == 0) s->opts.hbm_budget = size_t(16) << 30;
+        if (s->opts.speculate != 3) s->opts.speculate = 1;
+        s->throw_tick = test_throw_tick();
+        RCG_TRY(rcg_create(device, &s->ctx));
+        struct Owner {  // destroys the context unless the worker took it over
+            rcg_ctx* c;
+            bool armed = true;
+            ~Owner() {
+                if (armed) rcg_destroy(c);
+            }
+        } owner{s->ctx};
+        if (s->opts.tune) {
+            RCG_TRY(rcg_set_tune(s->ctx, s->opts.tune));
+            s->opts.tune = nullptr;
         }
-        s->opts.tune = nullptr;
-    }
-    s->worker = std::thread([s] { s->run(); });
-    *out = s;
-    return RCG_OK;
+        rcg_server* raw = s.get();
+        s->worker = std::thread([raw] { raw->run(); });
+        owner.armed = false;
+        *out = s.release();
+        return RCG_OK;
+    });
 }
 
 int rcg_server_submit(rcg_server* s, const rcg_request* req) {
-    if (!s || !req || !req->pixels) return RCG_INVALID;
-    RCG_TRY(check_image(req->width, req->height, req->channels));
-    if (req->width < 8 || req->height < 8) {
-        set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
-        return RCG_UNSUPPORTED;
-    }
-    Request* r = new (std::nothrow) Request();
-    if (!r) return RCG_OOM;
-    r->req = *req;
-    r->pixels.assign(req->pixels, req->pixels + size_t(req->width) * req->height * req->channels);
-    r->req.pixels = nullptr;
-    r->submit_s = now_s();
-    r->search.max_q = std::max(1, std::min(100, req->max_quality));
-    r->search.target = req->target;
-    r->search.hi = r->search.max_q;
-    r->footprint = s->footprint_of(*req);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->pending.push_back(r);
-    }
-    s->cv_work.notify_one();
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!s || !req || !req->pixels) return RCG_INVALID;
+        RCG_TRY(check_image(req->width, req->height, req->channels));
+        if (req->width < 8 || req->height < 8) {
+            set_error("--target-ssimulacra2 needs an image of at least 8x8 pixels");
+            return RCG_UNSUPPORTED;
+        }
+        std::unique_ptr<Request> r(new Request());
+        r->req = *req;
+        r->pixels.assign(req->pixels, req->pixels + size_t(req->width) * req->height * req->channels);
+        r->req.pixels = nullptr;
+        r->submit_s = now_s();
+        r->search.max_q = std::max(1, std::min(100, req->max_quality));
+        r->search.target = req->target;
+        r->search.hi = r->search.max_q;
+        const uint32_t ncomp = req->channels == 1 ? 1 : 3;
+        for (uint32_t c = 0; req->coefs[0] && c < ncomp; c++)
+            if (!req->coefs[c]) return RCG_INVALID;
+        r->footprint = s->footprint_of(*req);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->pending.push_back(r.get());
+        }
+        r.release();
+        s->cv_work.notify_one();
+        return RCG_OK;
+    });
 }
 
 int rcg_server_poll(rcg_server* s, rcg_completion* out, int max, int timeout_ms, int* n) {
-    if (!s || (!out && max > 0) || !n) return RCG_INVALID;
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (s->done.empty() && timeout_ms > 0)
-        s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->done.empty(); });
-    int k = 0;
-    while (k < max && !s->done.empty()) {
-        out[k++] = s->done.front();
-        s->done.pop_front();
-    }
-    *n = k;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!s || (!out && max > 0) || !n) return RCG_INVALID;
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->done.empty() && timeout_ms > 0)
+            s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->done.empty(); });
+        int k = 0;
+        while (k < max && !s->done.empty()) {
+            out[k++] = s->done.front();
+            s->done.pop_front();
+        }
+        *n = k;
+        return RCG_OK;
+    });
 }
 
 int rcg_server_get_stats(rcg_server* s, rcg_server_stats* out) {
-    if (!s || !out) return RCG_INVALID;
-    std::lock_guard<std::mutex> lk(s->mu);
-    *out = s->stats;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!s || !out) return RCG_INVALID;
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->stats;
+        return RCG_OK;
+    });
 }
 
 void rcg_server_destroy(rcg_server* s) {
-    if (!s) return;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
-    for (Request* r : s->active) {
-        if (r->d_src) (void)hipFree(r->d_src);
-        if (r->d_ref) (void)hipFree(r->d_ref);
-        delete r;
-    }
-    for (auto& b : s->spare) (void)hipFree(b.second);
-    for (Request* r : s->pending) delete r;
-    rcg_destroy(s->ctx);
-    delete s;
+    rcg::guarded_void([&] {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        if (s->worker.joinable()) s->worker.join();
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
+        for (Request* r : s->active) {
+            if (r->d_src) (void)hipFree(r->d_src);
+            if (r->d_ref) (void)hipFree(r->d_ref);
+            delete r;
+        }
+        for (auto& b : s->spare) (void)hipFree(b.second);
+        for (Request* r : s->pending) delete r;
+        rcg_destroy(s->ctx);
+        delete s;
+    });
 }
 
 }  // extern "C"

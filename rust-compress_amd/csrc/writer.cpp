@@ -908,73 +908,85 @@ int hand_over(const Bytes& b, uint8_t** out, size_t* len) {
 }  // namespace
 
 extern "C" int rcg_write_jpeg(const rcg_coef_image* ci, int mode, uint8_t** out, size_t* len) {
-    if (!out || !len) return RCG_INVALID;
-    *out = nullptr;
-    *len = 0;
-    Coefs c;
-    RCG_TRY(view(ci, c));
-    try {
-        switch (mode) {
-            case RCG_WRITE_BASELINE:
-                return hand_over(write_baseline(c, false), out, len);
-            case RCG_WRITE_PROGRESSIVE:
-                return hand_over(emit_progressive(c, plan(c).second, false), out, len);
-            case RCG_WRITE_BEST: {
-                // encode.rs:543-547: both sizes are estimated first; only the winner is written.
-                const auto p = plan(c);
-                if (baseline_estimate(c) <= p.first) return hand_over(write_baseline(c, false), out, len);
-                return hand_over(emit_progressive(c, p.second, false), out, len);
+    return rcg::guarded([&]() -> int {
+        if (!out || !len) return RCG_INVALID;
+        *out = nullptr;
+        *len = 0;
+        Coefs c;
+        RCG_TRY(view(ci, c));
+        try {
+            switch (mode) {
+                case RCG_WRITE_BASELINE:
+                    return hand_over(write_baseline(c, false), out, len);
+                case RCG_WRITE_PROGRESSIVE:
+                    return hand_over(emit_progressive(c, plan(c).second, false), out, len);
+                case RCG_WRITE_BEST: {
+                    // encode.rs:543-547: both sizes are estimated first; only the winner is written.
+                    const auto p = plan(c);
+                    if (baseline_estimate(c) <= p.first) return hand_over(write_baseline(c, false), out, len);
+                    return hand_over(emit_progressive(c, p.second, false), out, len);
+                }
+                default:
+                    rcg::set_error("unknown write mode");
+                    return RCG_INVALID;
             }
-            default:
-                rcg::set_error("unknown write mode");
-                return RCG_INVALID;
+        } catch (const std::bad_alloc&) {
+            rcg::set_error("out of host memory");
+            return RCG_OOM;
         }
-    } catch (const std::bad_alloc&) {
-        rcg::set_error("out of host memory");
-        return RCG_OOM;
-    }
+    });
 }
 
-extern "C" void rcg_free(void* p) { std::free(p); }
+extern "C" void rcg_free(void* p) {
+    rcg::guarded_void([&] {
+        std::free(p);
+    });
+}
 
 extern "C" int rcg_huffman_lengths(const uint32_t* freqs, size_t n, int max_len, uint8_t* lengths) {
-    if ((!freqs && n) || (!lengths && n) || max_len < 1 || max_len > 32) return RCG_INVALID;
-    try {
-        const std::vector<uint8_t> l = build_lengths(freqs, n, max_len);
-        if (n) std::memcpy(lengths, l.data(), n);
-    } catch (const std::bad_alloc&) {
-        return RCG_OOM;
-    }
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if ((!freqs && n) || (!lengths && n) || max_len < 1 || max_len > 32) return RCG_INVALID;
+        try {
+            const std::vector<uint8_t> l = build_lengths(freqs, n, max_len);
+            if (n) std::memcpy(lengths, l.data(), n);
+        } catch (const std::bad_alloc&) {
+            return RCG_OOM;
+        }
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_trellis(const rcg_coef_image* plain, const float* const scaled[4], int16_t* out[4]) {
-    if (!plain || !scaled || !out) return RCG_INVALID;
-    Coefs c;
-    RCG_TRY(view(plain, c));
-    if (c.comps.size() > 3 || !c.qtables[0]) {
-        rcg::set_error("trellis: 1 or 3 components with a luma table in slot 0");
-        return RCG_INVALID;
-    }
-    for (size_t k = 0; k < c.comps.size(); k++)
-        if (!scaled[k] || !out[k]) return RCG_INVALID;
-    try {
-        trellis(c, scaled, out);
-    } catch (const std::bad_alloc&) {
-        return RCG_OOM;
-    } catch (const std::system_error&) {
-        rcg::set_error("trellis: could not start host threads");
-        return RCG_DEVICE;
-    }
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!plain || !scaled || !out) return RCG_INVALID;
+        Coefs c;
+        RCG_TRY(view(plain, c));
+        if (c.comps.size() > 3 || !c.qtables[0]) {
+            rcg::set_error("trellis: 1 or 3 components with a luma table in slot 0");
+            return RCG_INVALID;
+        }
+        for (size_t k = 0; k < c.comps.size(); k++)
+            if (!scaled[k] || !out[k]) return RCG_INVALID;
+        try {
+            trellis(c, scaled, out);
+        } catch (const std::bad_alloc&) {
+            return RCG_OOM;
+        } catch (const std::system_error&) {
+            rcg::set_error("trellis: could not start host threads");
+            return RCG_DEVICE;
+        }
+        return RCG_OK;
+    });
 }
 
 extern "C" int rcg_rdo_quantize_ac(const uint8_t table[64], float kappa, const uint8_t sizes[256], const float a[64],
                                    int16_t out[64]) {
-    if (!table || !sizes || !a || !out) return RCG_INVALID;
-    const Rdo r(table, kappa, sizes);
-    r.quantize_ac(a, out);
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!table || !sizes || !a || !out) return RCG_INVALID;
+        const Rdo r(table, kappa, sizes);
+        r.quantize_ac(a, out);
+        return RCG_OK;
+    });
 }
 
 namespace rcg {
@@ -1026,34 +1038,36 @@ int requant_plan(const rcg_coef_image* ci, int quality, RequantPlan& P) {
 }  // namespace rcg
 
 extern "C" int rcg_requantize(const rcg_coef_image* ci, int quality, int trellis_on, rcg_requantized* out) {
-    if (!ci || !out) return RCG_INVALID;
-    rcg::RequantPlan P;
-    RCG_TRY(rcg::requant_plan(ci, quality, P));
-    // The coefficient grid and sampling checks of the writer's view, without requiring the tables it will replace.
-    rcg_coef_image probe = *ci;
-    uint8_t ones[64];
-    std::memset(ones, 1, 64);
-    for (int s = 0; s < 4; s++)
-        if (!probe.qtables[s]) probe.qtables[s] = ones;
-    Coefs c;
-    RCG_TRY(view(&probe, c));
-    int16_t* outs[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (uint32_t k = 0; k < ci->ncomp; k++) {
-        if (!out->coefs[k]) return RCG_INVALID;
-        outs[k] = out->coefs[k];
-    }
-    try {
-        requantize_host(c, P, trellis_on != 0, outs);
-    } catch (const std::bad_alloc&) {
-        return RCG_OOM;
-    } catch (const std::system_error&) {
-        rcg::set_error("requantize: could not start host threads");
-        return RCG_DEVICE;
-    }
-    for (int s = 0; s < 4; s++) {
-        out->present[s] = P.present[s];
-        std::memcpy(out->qtables[s], P.slots[s], 64);
-    }
-    for (uint32_t k = 0; k < 4; k++) out->tq[k] = k < ci->ncomp ? P.tq[k] : 0;
-    return RCG_OK;
+    return rcg::guarded([&]() -> int {
+        if (!ci || !out) return RCG_INVALID;
+        rcg::RequantPlan P;
+        RCG_TRY(rcg::requant_plan(ci, quality, P));
+        // The coefficient grid and sampling checks of the writer's view, without requiring the tables it will replace.
+        rcg_coef_image probe = *ci;
+        uint8_t ones[64];
+        std::memset(ones, 1, 64);
+        for (int s = 0; s < 4; s++)
+            if (!probe.qtables[s]) probe.qtables[s] = ones;
+        Coefs c;
+        RCG_TRY(view(&probe, c));
+        int16_t* outs[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (uint32_t k = 0; k < ci->ncomp; k++) {
+            if (!out->coefs[k]) return RCG_INVALID;
+            outs[k] = out->coefs[k];
+        }
+        try {
+            requantize_host(c, P, trellis_on != 0, outs);
+        } catch (const std::bad_alloc&) {
+            return RCG_OOM;
+        } catch (const std::system_error&) {
+            rcg::set_error("requantize: could not start host threads");
+            return RCG_DEVICE;
+        }
+        for (int s = 0; s < 4; s++) {
+            out->present[s] = P.present[s];
+            std::memcpy(out->qtables[s], P.slots[s], 64);
+        }
+        for (uint32_t k = 0; k < 4; k++) out->tq[k] = k < ci->ncomp ? P.tq[k] : 0;
+        return RCG_OK;
+    });
 }

@@ -598,3 +598,39 @@ def test_rgb_coded_requant_search_takes_the_pixel_route(ctx, rc):
     found = ctx.requant_find_quality(src, 70.0, 100, False)
     want, scores = _oracle_requant_search(src, 70.0, False)
     assert found.quality == want, (found.quality, want, sorted(scores.items()))
+
+
+# ------------------------------------------------------------------------------------- winners' coefficients
+
+def _assert_winner(img, ci, quality, sub):
+    # Found.coefs (jpeg/src/target.rs:11-16): build_coefs at the chosen quality and subsampling.
+    want = O.build_coefs(img, quality, sub)
+    assert ci is not None
+    _assert_coefs_equal(ci, want)
+
+
+def test_batch_returns_the_winners_coefficients(ctx, rc):
+    cands = [(q, rc.AUTO) for q in (40, 60, 80, 85, 90, 95)]   # Auto: 4:2:0 below q90, 4:4:4 above
+    imgs = [rc.synth_rgb8(800 + i, w, h) for i, (w, h) in enumerate([(301, 203), (97, 131), (640, 480), (64, 64)])]
+    imgs.append(np.ascontiguousarray(rc.synth_rgb8(805, 150, 90)[:, :, 0]))
+    for target in (60.0, 85.0):
+        res, win = ctx.score_batch(imgs, cands, target, winners=True)
+        for img, r, ci in zip(imgs, res, win):
+            assert r.status == 0 and r.winner_written == 1
+            _assert_winner(img, ci, r.chosen_quality, rc.AUTO)
+    # Device-resident images: the winners land in device buffers, eagerly and from a replayed plan.
+    dev = [ctx.upload(im) for im in imgs]
+    ptrs = [(d.ptr, im.shape[1], im.shape[0], 1 if im.ndim == 2 else 3) for d, im in zip(dev, imgs)]
+    res, win = ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs, winners=True)
+    for img, r, ci in zip(imgs, res, win):
+        _assert_winner(img, ci, r.chosen_quality, rc.AUTO)
+    plan = ctx.batch_plan(ptrs, cands, 70.0, winners=True)
+    for rep in range(2):
+        got = plan.run()
+        for img, r, ci in zip(imgs, got, plan.winners(got)):
+            assert r.winner_written == 1
+            _assert_winner(img, ci, r.chosen_quality, rc.AUTO)
+    plan.close()
+    # Jobs that do not ask get none.
+    res = ctx.score_batch(imgs[:2], cands, 70.0)
+    assert all(r.winner_written == 0 for r in res)

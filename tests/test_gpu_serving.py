@@ -88,3 +88,85 @@ def test_submit_rejects_small(rc):
         srv.submit(1, np.zeros((7, 40, 3), np.uint8), 70.0)
     assert e.value.status == rc.UNSUPPORTED
     srv.close()
+
+
+def test_completions_carry_the_winners_coefficients(rc, ctx):
+    # Found.coefs (jpeg/src/target.rs:11-16): each completion's coefficients are build_coefs at its chosen quality.
+    from oracle import oracle as O
+    srv = rc.Server(0, speculate=3)
+    reqs = [(i, rc.synth_rgb8(900 + i, w, h), t, sub)
+            for i, (w, h, t, sub) in enumerate([(333, 250, 70.0, rc.S420), (128, 96, 80.0, rc.AUTO),
+                                                (500, 211, 60.0, rc.S444), (97, 64, 75.0, rc.AUTO)])]
+    for i, img, t, sub in reqs:
+        srv.submit(i, img, t, 100, sub, coefs=True)
+    got = _drain(srv, len(reqs))
+    for i, img, t, sub in reqs:
+        c = got[i]
+        assert c.status == 0 and c.coefs_written == 1
+        ci = srv.winner(c)
+        want = O.build_coefs(img, c.quality, sub)
+        for a, b in zip(ci.comps, want.comps):
+            assert (a.h, a.v, a.blocks_w) == (b.h, b.v, b.blocks_w)
+            np.testing.assert_array_equal(a.coefs, b.coefs)
+        f = ctx.find_quality(img, t, 100, sub)
+        assert (c.quality, c.score) == (f.quality, f.score)
+    srv.close()
+
+
+def test_a_request_whose_working_set_cannot_be_allocated_fails_alone(rc, ctx, monkeypatch):
+    # Forced from the host: with RCG_TEST_ARENA_LIMIT the workspace refuses working sets above 8 MiB, as a full
+    # device would. The 2000x1500 request's working set exceeds it, the small ones' do not: the shared tick fails,
+    # is retried request by request, and only the large request fails (RCG_OOM); the others get their alone results.
+    small = [(i, rc.synth_rgb8(950 + i, w, h), t) for i, (w, h, t) in
+             enumerate([(96, 80, 70.0), (128, 128, 60.0), (200, 150, 80.0)])]
+    alone = {i: ctx.find_quality(img, t, 100, rc.S420) for i, img, t in small}
+    big = rc.synth_rgb8(960, 2000, 1500)
+    monkeypatch.setenv("RCG_TEST_ARENA_LIMIT", str(8 << 20))
+    srv = rc.Server(0)
+    for i, img, t in small[:2]:
+        srv.submit(i, img, t)
+    srv.submit(99, big, 70.0)
+    srv.submit(*small[2])
+    got = _drain(srv, 4)
+    srv.close()
+    monkeypatch.delenv("RCG_TEST_ARENA_LIMIT")
+    assert got[99].status == rc.OOM
+    for i, img, t in small:
+        assert got[i].status == 0, i
+        assert (got[i].quality, got[i].score) == (alone[i].quality, alone[i].score), i
+
+
+def test_an_exception_in_a_tick_fails_that_tick_and_the_worker_goes_on(rc, ctx, monkeypatch):
+    # Forced from the host: RCG_TEST_THROW_TICK=1 makes the worker's first tick throw std::bad_alloc, as a host
+    # allocation in the tick would. Its request fails with RCG_OOM; the worker survives and later requests complete
+    # with their alone results.
+    monkeypatch.setenv("RCG_TEST_THROW_TICK", "1")
+    srv = rc.Server(0)
+    monkeypatch.delenv("RCG_TEST_THROW_TICK")
+    a = rc.synth_rgb8(970, 160, 120)
+    srv.submit(1, a, 70.0)
+    first = _drain(srv, 1)
+    assert first[1].status == rc.OOM
+    b = rc.synth_rgb8(971, 200, 100)
+    srv.submit(2, b, 70.0)
+    srv.submit(3, a, 70.0)
+    got = _drain(srv, 2)
+    srv.close()
+    for i, img in ((2, b), (3, a)):
+        f = ctx.find_quality(img, 70.0, 100, rc.S420)
+        assert got[i].status == 0 and (got[i].quality, got[i].score) == (f.quality, f.score)
+
+
+def test_kept_blocks_stay_within_the_budget(rc):
+    # Finished requests' device blocks are kept for reuse only while admitted footprints and kept blocks fit the
+    # budget: with a budget of about two requests' footprints, a stream of mixed sizes still completes.
+    srv = rc.Server(0, hbm_budget=96 << 20)
+    imgs = [rc.synth_rgb8(980 + i, w, h) for i, (w, h) in enumerate([(700, 500), (300, 200), (800, 600),
+                                                                       (128, 128), (640, 480), (1000, 700)])]
+    for i, img in enumerate(imgs):
+        srv.submit(i, img, 70.0)
+    got = _drain(srv, len(imgs))
+    st = srv.stats()
+    srv.close()
+    assert all(c.status == 0 for c in got.values())
+    assert st.peak_footprint <= 96 << 20 or st.mean_batch == 1.0
