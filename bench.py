@@ -1,14 +1,19 @@
-"""Benchmark of the JPEG target-quality inner loop on one MI355X.
+"""Benchmark of the JPEG target-quality search on one MI355X.
 
-Workload (the plain-drop-in configuration): one 4096x4096 RGB8 synthetic image (seeded generator of
-rcg_synth_rgb8), candidates q{50,75,90} x {4:2:0, 4:4:4}, trellis off. One step scores all six candidates: encode
-(colour, downsample, FDCT, quantisation) + dequantise/IDCT reconstruction + SSIMULACRA2 (FP32 maps, FP64 pooling)
-against the source, in one launch sequence, with the source resident in HBM before the timed region.
+Headline workload (the batching configuration): 256 synthetic RGB8 images (seeded generator of rcg_synth_rgb8), width
+and height each uniform in [64, 3000], each swept over 8 candidate qualities {40,50,60,70,80,85,90,95} at 4:2:0 with
+trellis off; the choice is the lowest quality with SSIMULACRA2 >= 70 by the reference's bisection over the sorted
+list. One step is one replay of the whole pass captured as a hipGraph (encode, reconstruction planes, reference
+states, six SSIMULACRA2 scale launches, pooling), followed on the host by the FP64 settling of scores near the target
+and the selection. The images are resident in HBM before the timed region.
 
-value = candidate-evaluation throughput in MP/s (image megapixels x candidates / second). The same run also reports
-the plain per-image GPU search (serial bisection over q in 1..100, rcg_find_quality_dev) in search-MP/s, the roofline
-of the dominant kernel (k_ssim_scale) from HIP events recorded on the library's own stream around every launch in
-the timed region, and the CPU oracle (C restatement of the reference CPU path, OpenMP threads) on a bounded sample.
+value = search throughput in MP/s (image megapixels searched per second). Beside it, as the metric asks: the plain
+per-image GPU search (serial bisection over q in 1..100, rcg_find_quality_dev) over the same images, the batched /
+plain ratio, and the serving configuration's p50/p99 latency (Poisson arrivals at 200 requests/s). The roofline is
+that of the pass's dominant kernel (k_ssim_scale), from HIP events recorded by the library around every launch of the
+same pass run eagerly, on the stream each launch runs on; HBM traffic and VALU busy come from rocprofv3 counter
+passes over one eager pass in child processes. The CPU baseline is the oracle (C restatement of the reference CPU
+path, OpenMP threads) on a bounded sample of the same images: warm-up, then the median of 5 runs.
 
 All device memory and synchronisation go through librcgpu (rcg_device_alloc, rcg_synchronize on the stream every
 kernel runs on); PyTorch is not loaded, so the process holds a single HIP runtime (torch bundles its own).
@@ -32,14 +37,46 @@ HBM_PEAK_GBS = 8000.0
 SSIM_BYTES_PER_PX = 18.0
 # Whole candidate evaluation: + read RGB8 (3), write/read coefficients (3 + 3, or 6 + 6 at 4:4:4), write RGB8 (3).
 EVAL_BYTES_PER_PX = {3: 30.0, 1: 36.0}
+# Per image, once: build the reference pyramid (read 3 + write 4).
+REF_BYTES_PER_PX = 7.0
+CANDS_B = (40, 50, 60, 70, 80, 85, 90, 95)
+TARGET_B = 70.0
+SCALE0 = "k_ssim_scale<float, 256, 3u, 1u>"
 
 
 def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def batch_sizes(n):
+    """The batching configuration's image sizes: W and H each uniform in [64, 3000] (seed 2024)."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    return [(int(rng.integers(64, 3001)), int(rng.integers(64, 3001))) for _ in range(n)]
+
+
+def upload_batch(rc, ctx, n):
+    sizes = batch_sizes(n)
+    dev = []
+    for i, (w, h) in enumerate(sizes):
+        dev.append(ctx.upload(rc.synth_rgb8(1000 + i, w, h)))
+        if i % 64 == 63:
+            progress(f"generated {i + 1} batch images")
+    ctx.synchronize()
+    return sizes, dev
+
+
+def install_policy(rc, ctx, policy):
+    tune = rc.Tune()
+    tune.wide_nt, tune.narrow_w, tune.nclass = policy["wide_nt"], policy["narrow_w"], len(policy["classes"])
+    for i, (min_h, th) in enumerate(policy["classes"]):
+        tune.min_h[i], tune.th[i] = min_h, th
+    ctx.set_tune(tune)
+    return ctx.get_tune()
+
+
 def _pmc_run(counter, bench_args, outdir, name):
-    """One rocprofv3 counter pass over a child run of this script; returns [(kernel, dispatch, KiB)] in dispatch
+    """One rocprofv3 counter pass over a child run of this script; returns [(kernel, dispatch, value)] in dispatch
     order, or None if the pass failed."""
     import shutil
     import sqlite3
@@ -61,14 +98,14 @@ def _pmc_run(counter, bench_args, outdir, name):
     rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
                        "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,)).fetchall()
     con.close()
-    return [(k.split("(")[0], i, v) for i, k, v in rows]
+    return [(k.split("(")[0].replace("void rcg::", "").replace("rcg::", ""), i, v) for i, k, v in rows]
 
 
-def tuned_policy(ncand):
-    """The headline's tile policy, autotuned in a child process (so that this process has not touched the GPU
-    when the counter passes run); None if the child failed."""
+def tuned_policy(jobs):
+    """A tile policy autotuned in a child process (so that this process has not touched the GPU when the counter
+    passes run); None if the child failed."""
     import subprocess
-    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--autotune-only", str(ncand)],
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--autotune-only", str(jobs)],
                        stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, cwd=ROOT, text=True)
     lines = r.stdout.strip().splitlines()
     if r.returncode != 0 or not lines:
@@ -76,68 +113,78 @@ def tuned_policy(ncand):
     return json.loads(lines[-1])
 
 
-def measure_traffic(size, policy):
-    """HBM bytes per launch of the dominant kernel (k_ssim_scale) in one headline step, from FETCH_SIZE and
-    WRITE_SIZE (one rocprofv3 pass each over a one-step child run under the same tile policy as the timed steps),
-    each scaled by what the counter reads for a 512 MiB device-to-device copy (the same counter's calibration
-    factor). Runs before this process touches the GPU: the passes are child processes."""
+def counter_passes(images, policy):
+    """rocprofv3 counter passes over one eager pass of the batched configuration in child processes (each generates
+    the images, installs `policy`, runs the pass once), before this process touches the GPU:
+      traffic: FETCH_SIZE and WRITE_SIZE of every k_ssim_scale FP32 launch, each scaled by what the same counter reads
+               for a 512 MiB device-to-device copy (that counter's calibration factor);
+      VALU:    rocprofv3's VALUBusy (100 * SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE) of each launch."""
     import tempfile
-    out = tempfile.mkdtemp(prefix="rcg_traffic_")
-    step = ["--steps", "1", "--warmup", "0", "--size", str(size), "--no-cpu", "--no-plain", "--no-batch",
-            "--no-serve", "--no-cli", "--no-traffic", "--tile-policy", json.dumps(policy)]
+    out = tempfile.mkdtemp(prefix="rcg_counters_")
+    child = ["--batch-pass-child", json.dumps(policy), "--batch-images", str(images)]
     copy = ["--copy-calibration", "512"]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        progress(f"traffic: {counter} passes")
+        progress(f"counters: {counter} passes")
         cal = _pmc_run(counter, copy, out, counter.lower() + "_copy")
-        run = _pmc_run(counter, step, out, counter.lower() + "_step")
+        run = _pmc_run(counter, child, out, counter.lower() + "_pass")
         if not cal or not run:
-            return None
-        # Calibration: the copy's largest dispatch moved 512 MiB (read for FETCH_SIZE, written for WRITE_SIZE).
+            return None, None
         kib = max(v for _, _, v in cal)
         factor = (512 * 1024) / kib if kib > 0 else None
-        # The last headline step: the dispatches after the last reference-cache launch from RGB8 bytes (the autotuner
-        # launches it only from planes).
-        last_ref = max((i for i, (k, _, _) in enumerate(run) if k.startswith("void rcg::k_ssim_ref<") and
-                        k.endswith("0u>")), default=None)
-        if last_ref is None or factor is None:
-            return None
-        vals = [v for k, _, v in run[last_ref:] if "k_ssim_scale<float" in k]
-        if not vals:
-            return None
-        res[counter] = {"kib_per_launch": sum(vals) / len(vals), "factor": factor, "launches": len(vals)}
-    fetch = res["FETCH_SIZE"]["kib_per_launch"] * res["FETCH_SIZE"]["factor"] * 1024
-    write = res["WRITE_SIZE"]["kib_per_launch"] * res["WRITE_SIZE"]["factor"] * 1024
-    return {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
-            "fetch_factor": round(res["FETCH_SIZE"]["factor"], 3), "write_factor": round(res["WRITE_SIZE"]["factor"], 3),
-            "launches": res["FETCH_SIZE"]["launches"], "tile_policy": policy}
+        vals = [v for k, _, v in run if k.startswith("k_ssim_scale<float")]
+        s0 = [v for k, _, v in run if k.startswith(SCALE0)]
+        if not vals or factor is None:
+            return None, None
+        res[counter] = {"kib_per_launch": sum(vals) / len(vals), "scale0_kib": sum(s0), "factor": factor,
+                        "launches": len(vals)}
+    f, w = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    fetch = f["kib_per_launch"] * f["factor"] * 1024
+    write = w["kib_per_launch"] * w["factor"] * 1024
+    traffic = {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+               "scale0_bytes": f["scale0_kib"] * f["factor"] * 1024 + w["scale0_kib"] * w["factor"] * 1024,
+               "fetch_factor": round(f["factor"], 3), "write_factor": round(w["factor"], 3),
+               "launches": f["launches"]}
+    progress("counters: VALUBusy pass")
+    valu = None
+    run = _pmc_run("VALUBusy", child, out, "valu_pass")
+    if run:
+        def pick(prefix):
+            v = [x for k, _, x in run if k.startswith(prefix)]
+            return round(v[0], 1) if v else None
+        valu = {"k_ssim_scale_scale0_pct": pick(SCALE0), "k_encode_pct": pick("k_encode"),
+                "k_ssim_ref_scale0_pct": pick("k_ssim_ref<256, 0u>"),
+                "counter": "VALUBusy = 100*SQ_ACTIVE_INST_VALU/CU_NUM/GRBM_GUI_ACTIVE (rocprofv3), one eager pass"}
+    return traffic, valu
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--steps", type=int, default=20, help="timed replays of the batched pass")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch-images", type=int, default=256)
+    ap.add_argument("--size", type=int, default=4096, help="side of the plain drop-in image (extra figures)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the precision report")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-image search")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 256-image batched search")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the plain drop-in (one 4096x4096 image) figures")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
     ap.add_argument("--cli-files", type=int, default=24)
-    ap.add_argument("--batch-images", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests of the serving run (Poisson arrivals at 200 per second)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (HBM traffic counter calibration)")
-    ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that give roofline.traffic")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (traffic, VALU busy)")
     ap.add_argument("--tile-policy", default=None, metavar="JSON",
-                    help="install this headline tile policy (as the bench line prints it) instead of autotuning")
-    ap.add_argument("--autotune-only", type=int, default=0, metavar="NCAND",
-                    help="only autotune the tile policy for NCAND candidates and print it as JSON")
+                    help="install this batch tile policy (as the bench line prints it) instead of autotuning")
+    ap.add_argument("--autotune-only", type=int, default=-1, metavar="JOBS",
+                    help="only autotune the tile policy for launches of JOBS pairs (0: throughput) and print it")
+    ap.add_argument("--batch-pass-child", default=None, metavar="JSON",
+                    help="only run one eager pass of the batched configuration under this tile policy")
     args = ap.parse_args()
-    if args.autotune_only:
+    if args.autotune_only >= 0:
         rc = importlib.import_module("rust-compress_amd")
         print(json.dumps(rc.Context(0).autotune(args.autotune_only).as_dict()), flush=True)
         return
@@ -152,67 +199,73 @@ def main():
         ctx.synchronize()
         print(json.dumps({"copy_calibration_bytes": n}), flush=True)
         return
+    if args.batch_pass_child:
+        rc = importlib.import_module("rust-compress_amd")
+        ctx = rc.Context(0)
+        install_policy(rc, ctx, json.loads(args.batch_pass_child))
+        sizes, dev = upload_batch(rc, ctx, args.batch_images)
+        ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
+        ctx.score_batch(None, [(q, rc.S420) for q in CANDS_B], TARGET_B, device_ptrs=ptrs)
+        ctx.synchronize()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(images would shard across GPUs with no collective; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
-    cands_n = 6  # q{50,75,90} x {4:2:0, 4:4:4}
-    policy = json.loads(args.tile_policy) if args.tile_policy else None
-    traffic = None
-    if not args.no_traffic:
-        # Child processes, before any GPU use here; the timed steps then run under the policy the passes measured.
-        policy = policy or tuned_policy(cands_n)
-        traffic = measure_traffic(args.size, policy) if policy else None
+    # A batch launch holds many pairs of every size and runs many rounds of workgroups: its tile policy is tuned for
+    # throughput (time per pair over launches of many rounds). Tuned in a child, so that the counter passes (children
+    # too) run before this process touches the GPU, under the policy the timed steps use.
+    policy = json.loads(args.tile_policy) if args.tile_policy else tuned_policy(0)
+    traffic = valu = None
+    if not args.no_counters and policy:
+        traffic, valu = counter_passes(args.batch_images, policy)
     rc = importlib.import_module("rust-compress_amd")
     ctx = rc.Context(0)
-    W = H = args.size
-    img = rc.synth_rgb8(2, W, H)
-    d_img = ctx.upload(img)  # resident in HBM before the timed region
-    ptr = d_img.ptr
-    cands = [(q, s) for s in (rc.S420, rc.S444) for q in (50, 75, 90)]
-    assert len(cands) == cands_n
-    # Tile heights per scale-height class, measured on this GPU for this workload's candidate count.
-    if policy:
-        tune = rc.Tune()
-        tune.wide_nt, tune.narrow_w, tune.nclass = policy["wide_nt"], policy["narrow_w"], len(policy["classes"])
-        for i, (min_h, th) in enumerate(policy["classes"]):
-            tune.min_h[i], tune.th[i] = min_h, th
-        ctx.set_tune(tune)
-        tune = ctx.get_tune()
-        progress(f"installed tile policy: {tune.as_dict()}")
-    else:
-        tune = ctx.autotune(len(cands))
-        progress(f"autotuned tile policy: {tune.as_dict()}")
-    target = 70.0
+    tune_b = install_policy(rc, ctx, policy) if policy else ctx.autotune(0)
+    progress(f"batch tile policy: {tune_b.as_dict()}")
+    t0 = time.perf_counter()
+    sizes, dev = upload_batch(rc, ctx, args.batch_images)
+    gen_s = time.perf_counter() - t0
+    ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
+    cands = [(q, rc.S420) for q in CANDS_B]
+    total_px = sum(w * h for w, h in sizes)
+    total_mp = total_px / 1e6
 
-    def step():
-        return ctx.score_batch(None, cands, target, device_ptrs=[(ptr, W, H, 3)])
-
+    # The timed steps: replays of the captured pass.
+    plan = ctx.batch_plan(ptrs, cands, TARGET_B)
     for _ in range(args.warmup):
-        res = step()
+        plan.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gres = plan.run()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    plan.close()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    value = total_mp / (ms_per_step / 1000.0)
+    progress(f"batched pass as a graph: {ms_per_step:.2f} ms per replay")
+
+    # The same pass eagerly, as many times, with HIP events around every launch on the stream it runs on: the
+    # per-kernel times and the dominant kernel's roofline.
+    res = ctx.score_batch(None, cands, TARGET_B, device_ptrs=ptrs)
     ctx.synchronize()
     ctx.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = step()
+        res = ctx.score_batch(None, cands, TARGET_B, device_ptrs=ptrs)
     ctx.synchronize()
-    t1 = time.perf_counter()
-    ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
-    enc_ms, enc_n = ctx.profile_get("k_encode")
-    fin_ms, fin_n = ctx.profile_get("k_ssim_finalize")
-    ref_ms, ref_n = ctx.profile_get("k_ssim_ref")
+    eager_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    prof = {k: ctx.profile_get(k) for k in ("k_ssim_scale", "k_ssim_ref", "k_encode", "k_ssim_scale_f64",
+                                            "k_ssim_finalize")}
     ctx.profile(False)
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    progress(f"headline workload timed: {ms_per_step:.3f} ms/step")
-    mp = W * H / 1e6
-    evals = len(cands)
-    value = mp * evals / (ms_per_step / 1000.0)
-
-    # Dominant kernel roofline: algorithmic bytes of all its launches / their summed duration.
-    ssim_bytes = SSIM_BYTES_PER_PX * W * H * evals * args.steps
-    achieved = ssim_bytes / (ssim_ms / 1000.0) / 1e9 if ssim_ms > 0 else 0.0
-    r0 = res[0]
+    ssim_ms, ssim_n = prof["k_ssim_scale"]
+    # k_ssim_scale is the FP32 kernel (the FP64 settling launches are timed apart): every pair, every scale.
+    launches_per_pass = ssim_n / args.steps
+    alg_per_pass = SSIM_BYTES_PER_PX * total_px * len(CANDS_B)
+    achieved = alg_per_pass * args.steps / (ssim_ms / 1000.0) / 1e9 if ssim_ms > 0 else 0.0
+    eval_bytes = (EVAL_BYTES_PER_PX[3] * len(CANDS_B) + REF_BYTES_PER_PX) * total_px
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -226,69 +279,54 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain drop-in: one {W}x{H} RGB8 synthetic image, q{{50,75,90}} x {{4:2:0,4:4:4}}, "
-                               "trellis off, encode+reconstruct+SSIMULACRA2 per candidate",
-                   "candidates": evals, "megapixels": mp},
-        "roofline": {"bound": "hbm", "kernel": "k_ssim_scale", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "config": {"workload": f"batched target-SSIMULACRA2 search: {len(sizes)} synthetic RGB8 images, W and H "
+                               "uniform in [64,3000] (seed 2024), 8 candidates q{40,50,60,70,80,85,90,95} at 4:2:0, "
+                               "trellis off, target 70; one hipGraph replay of the whole pass per step",
+                   "images": len(sizes), "megapixels": round(total_mp, 2), "candidates": list(CANDS_B),
+                   "target": TARGET_B},
+        "roofline": {"bound": "hbm", "kernel": "k_ssim_scale", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["bytes_per_launch"]) if traffic else None,
                      "traffic_detail": traffic,
-                     "algorithmic_bytes_per_launch": round(SSIM_BYTES_PER_PX * W * H * evals / max(1, ssim_n / args.steps)),
-                     "avg_launch_ms": round(ssim_ms / max(1, ssim_n), 4), "launches": ssim_n,
-                     "bytes_per_px_per_candidate": SSIM_BYTES_PER_PX},
-        "kernels_ms_per_step": {"k_ssim_scale": round(ssim_ms / args.steps, 3),
-                                "k_ssim_ref": round(ref_ms / args.steps, 3),
-                                "k_encode": round(enc_ms / args.steps, 3),
-                                "k_ssim_finalize": round(fin_ms / args.steps, 3)},
-        "tile_policy": tune.as_dict(),
-        "scores": [round(x, 4) for x in list(r0.scores)[:evals]],
-        "settled_fp64": int(sum(r0.settled[:evals])),
+                     "algorithmic_bytes_per_launch": round(alg_per_pass / max(1.0, launches_per_pass)),
+                     "avg_launch_ms": round(ssim_ms / max(1, ssim_n), 4), "launches_per_pass": launches_per_pass,
+                     "bytes_per_px_per_candidate": SSIM_BYTES_PER_PX,
+                     "timed": f"{args.steps} eager passes of the same batch, HIP events around every launch"},
+        "valu": valu,
+        "search_mps_batched": round(value, 2),
     }
-    eval_bytes = sum(EVAL_BYTES_PER_PX[1 if s == rc.S444 else 3] for _, s in cands) * W * H
-    out["eval_hbm_frac"] = round(eval_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
-    # The same step from a host buffer (rcg_score_batch copies the source over PCIe on every call): the rate a caller
-    # that hands over host memory sees. Not `value`, which starts with the source resident in HBM.
-    ctx.score_batch([img], cands, target)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.score_batch([img], cands, target)
-    ctx.synchronize()
-    host_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
-    out["pcie_inclusive"] = {"ms_per_step": round(host_ms, 3), "value": round(mp * evals / (host_ms / 1000.0), 2),
-                             "unit": "MP/s"}
-
+    out["kernels_ms_per_pass"] = {k: round(v[0] / args.steps, 3) for k, v in prof.items()}
+    out["eager_pass_ms"] = round(eager_ms, 3)
+    out["eager_search_mps"] = round(total_mp / (eager_ms / 1000.0), 2)
+    out["pass_hbm_frac"] = round(eval_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+    out["batch"] = {"tile_policy": tune_b.as_dict(), "generate_s": round(gen_s, 1),
+                    "settled_fp64": int(sum(sum(r.settled[:len(CANDS_B)]) for r in res)),
+                    "chosen_equals_sweep": int(sum(r.chosen_quality == r.sweep_quality for r in res)),
+                    "graph_matches_eager": int(all(g.chosen_quality == r.chosen_quality and
+                                                   list(g.scores[:8]) == list(r.scores[:8])
+                                                   for g, r in zip(gres, res)))}
     if not args.no_plain:
-        ctx.find_quality_dev(ptr, W, H, 3, target, 100, rc.S420)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        f = ctx.find_quality_dev(ptr, W, H, 3, target, 100, rc.S420)
-        t1 = time.perf_counter()
-        out["plain_search"] = {"quality": f.quality, "steps": f.steps, "ms": round((t1 - t0) * 1000, 2),
-                               "search_mps": round(mp / (t1 - t0), 2),
-                               "eval_mps": round(mp * f.steps / (t1 - t0), 2)}
-
-    if not args.no_cpu:
-        progress("timing the CPU oracle sample")
-        from oracle import oracle as O  # the CPU baseline is the oracle, timed, never the measured path
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sample = [(50, O.S420), (90, O.S444)]
-        t0 = time.perf_counter()
-        for q, s in sample:
-            O.ssimulacra2(img, O.reconstruct(O.build_coefs(img, q, s)))
-        t1 = time.perf_counter()
-        out["cpu_baseline"] = {"value": round(mp * len(sample) / (t1 - t0), 3), "unit": "MP/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"{len(sample)} candidate evaluations (q50 4:2:0, q90 4:4:4) of the same "
-                                         f"{W}x{H} image: oracle build_coefs + reconstruct + f64 SSIMULACRA2"}
-        progress("scores against the f64 oracle")
-        out["precision"] = precision_report(rc, ctx, O, img, cands, r0)
-    if not args.no_batch:
-        progress("batched configuration")
-        out["batched"] = bench_batch(rc, ctx, args)
+        progress("plain per-image search over the same images")
+        out.update(bench_plain(rc, ctx, ptrs, total_mp, value))
+        install_policy(rc, ctx, tune_b.as_dict())
     if not args.no_serve:
         progress("serving configuration")
-        out["serving"] = bench_serve(rc, args, tune)
+        # A serving tick holds a few images with three candidates each (speculate = 3): tuned for that launch shape.
+        tune_s = ctx.autotune(3)
+        install_policy(rc, ctx, tune_b.as_dict())
+        sv = bench_serve(rc, args, tune_s)
+        sv["tile_policy"] = tune_s.as_dict()
+        out["p50_ms"] = sv["p50_ms"]
+        out["p99_ms"] = sv["p99_ms"]
+        out["serving"] = sv
+    if not args.no_cpu:
+        progress("timing the CPU oracle sample")
+        out["cpu_baseline"] = bench_cpu(rc, sizes)
+    for d in dev:
+        d.free()
+    if not args.no_dropin:
+        progress("plain drop-in figures (one 4096x4096 image, 6 candidates)")
+        out["dropin"] = bench_dropin(rc, ctx, args)
     if not args.no_cli:
         progress("command-line end-to-end run")
         out["cli"] = bench_cli(rc, args)
@@ -296,7 +334,103 @@ def main():
     ctx.close()
 
 
-CANDS_B = (40, 50, 60, 70, 80, 85, 90, 95)
+def bench_plain(rc, ctx, ptrs, total_mp, value):
+    """The plain per-image GPU search (serial bisection over 1..100, one candidate per step) over the batch's images,
+    under the tile policy tuned for one pair per launch: the median of three full runs."""
+    pol = ctx.autotune(1)
+    for p in ptrs[:4]:
+        ctx.find_quality_dev(p[0], p[1], p[2], 3, TARGET_B, 100, rc.S420)  # warm-up under the new policy
+    ctx.synchronize()
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        steps = 0
+        for p in ptrs:
+            steps += ctx.find_quality_dev(p[0], p[1], p[2], 3, TARGET_B, 100, rc.S420).steps
+        runs.append((time.perf_counter() - t0, steps))
+    secs, steps = sorted(runs)[1]
+    plain = total_mp / secs
+    return {"plain_search_mps": round(plain, 2), "batched_vs_plain": round(value / plain, 2),
+            "plain": {"seconds": round(secs, 3), "eval_mps": round(total_mp * steps / len(ptrs) / secs, 2),
+                      "mean_steps": round(steps / len(ptrs), 2), "tile_policy": pol.as_dict(),
+                      "runs_s": [round(r[0], 3) for r in runs]}}
+
+
+def bench_cpu(rc, sizes):
+    """The oracle on a bounded sample of the batched workload: the first images of the batch (about 10 MP) swept over
+    the same 8 candidates (build_coefs + reconstruct + f64 SSIMULACRA2 per candidate); one warm-up run, then the
+    median of 5. value is search-MP/s (image megapixels per second), as the headline."""
+    from oracle import oracle as O  # the CPU baseline is the oracle, timed, never the measured path
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sample, mp = [], 0.0
+    for i, (w, h) in enumerate(sizes):
+        sample.append(rc.synth_rgb8(1000 + i, w, h))
+        mp += w * h / 1e6
+        if mp >= 10.0:
+            break
+
+    def run():
+        for img in sample:
+            for q in CANDS_B:
+                O.ssimulacra2(img, O.reconstruct(O.build_coefs(img, q, O.S420)))
+
+    run()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[2]
+    return {"value": round(mp / med, 3), "unit": "MP/s", "cores": threads, "kind": "port",
+            "eval_mps": round(mp * len(CANDS_B) / med, 3), "runs_s": [round(t, 3) for t in times],
+            "sample": f"the first {len(sample)} images of the batched configuration ({mp:.2f} MP) x 8 candidates "
+                      "at 4:2:0: oracle build_coefs + reconstruct + f64 SSIMULACRA2, warm-up then median of 5; "
+                      "value in search-MP/s"}
+
+
+def bench_dropin(rc, ctx, args):
+    """The plain drop-in configuration: one 4096x4096 image, q{50,75,90} x {4:2:0, 4:4:4}, every candidate scored
+    per step (rcg_score_batch_dev), source resident; plus the plain search on it, its scores against the f64 oracle,
+    the precision configuration and the PCIe-inclusive rate."""
+    W = H = args.size
+    img = rc.synth_rgb8(2, W, H)
+    d_img = ctx.upload(img)
+    cands = [(q, s) for s in (rc.S420, rc.S444) for q in (50, 75, 90)]
+    ctx.autotune(len(cands))
+    step = lambda: ctx.score_batch(None, cands, TARGET_B, device_ptrs=[(d_img.ptr, W, H, 3)])
+    for _ in range(3):
+        r0 = step()[0]
+    ctx.synchronize()
+    n = max(10, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        r0 = step()[0]
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / n
+    mp = W * H / 1e6
+    out = {"workload": f"one {W}x{H} RGB8 image, q{{50,75,90}} x {{4:2:0,4:4:4}}, every candidate per step",
+           "ms_per_step": round(ms, 3), "candidate_mps": round(mp * len(cands) / (ms / 1000.0), 2),
+           "scores": [round(x, 4) for x in list(r0.scores)[:len(cands)]]}
+    ctx.score_batch([img], cands, TARGET_B)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        ctx.score_batch([img], cands, TARGET_B)
+    ctx.synchronize()
+    host_ms = (time.perf_counter() - t0) * 1000.0 / n
+    out["pcie_inclusive"] = {"ms_per_step": round(host_ms, 3),
+                             "candidate_mps": round(mp * len(cands) / (host_ms / 1000.0), 2)}
+    ctx.find_quality_dev(d_img.ptr, W, H, 3, TARGET_B, 100, rc.S420)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    f = ctx.find_quality_dev(d_img.ptr, W, H, 3, TARGET_B, 100, rc.S420)
+    t1 = time.perf_counter()
+    out["plain_search"] = {"quality": f.quality, "steps": f.steps, "ms": round((t1 - t0) * 1000, 2),
+                           "search_mps": round(mp / (t1 - t0), 2)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        out["precision"] = precision_report(rc, ctx, O, img, cands, r0)
+    d_img.free()
+    return out
 
 
 def precision_report(rc, ctx, O, img, cands, r0):
@@ -326,87 +460,6 @@ def precision_report(rc, ctx, O, img, cands, r0):
                                                        np.abs(st64 - wst).max(axis=(1, 2))]})
     return {"bound": 0.02, "headline": head, "max_abs_err_headline": max(x["abs_err"] for x in head),
             "configs": configs}
-
-
-def bench_batch(rc, ctx, args):
-    """256 synthetic RGB8 images, W and H uniform in [64, 3000], each swept over 8 candidate qualities at 4:2:0 in
-    one batched pass; choice = lowest q with SSIMULACRA2 >= 70 by the reference's bisection over the sorted list.
-    Compared with the plain per-image GPU search (serial bisection over q in 1..100) on the same images."""
-    import numpy as np
-    rng = np.random.default_rng(2024)
-    sizes = [(int(rng.integers(64, 3001)), int(rng.integers(64, 3001))) for _ in range(args.batch_images)]
-    t0 = time.perf_counter()
-    dev = []
-    for i, (w, h) in enumerate(sizes):
-        dev.append(ctx.upload(rc.synth_rgb8(1000 + i, w, h)))
-        if i % 64 == 63:
-            progress(f"generated {i + 1} batch images")
-    ctx.synchronize()
-    gen_s = time.perf_counter() - t0
-    ptrs = [(t.ptr, w, h, 3) for t, (w, h) in zip(dev, sizes)]
-    cands = [(q, rc.S420) for q in CANDS_B]
-    # A batch launch holds many pairs of every size and runs many rounds of workgroups: tune for throughput (time per
-    # pair over launches of many rounds), not for one image's candidates.
-    tune_b = ctx.autotune(0)
-    total_mp = sum(w * h for w, h in sizes) / 1e6
-    ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)  # warm-up (sizes the workspace)
-    ctx.synchronize()
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    res = ctx.score_batch(None, cands, 70.0, device_ptrs=ptrs)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ssim_ms, ssim_n = ctx.profile_get("k_ssim_scale")
-    ref_ms, _ = ctx.profile_get("k_ssim_ref")
-    ctx.profile(False)
-    batch_s = t1 - t0
-    out = {"images": len(sizes), "megapixels": round(total_mp, 2), "candidates": list(CANDS_B),
-           "seconds": round(batch_s, 4), "search_mps": round(total_mp / batch_s, 2),
-           "eval_mps": round(total_mp * len(CANDS_B) / batch_s, 2),
-           "hbm_frac": round(total_mp * 1e6 * (30.0 * len(CANDS_B) + 7.0) / batch_s / 1e9 / HBM_PEAK_GBS, 4),
-           "k_ssim_scale_ms": round(ssim_ms, 2), "k_ssim_scale_launches": ssim_n, "k_ssim_ref_ms": round(ref_ms, 2),
-           "tile_policy": tune_b.as_dict(),
-           "settled_fp64": int(sum(sum(r.settled[:len(CANDS_B)]) for r in res)),
-           "chosen_equals_sweep": int(sum(r.chosen_quality == r.sweep_quality for r in res)),
-           "generate_s": round(gen_s, 1)}
-    progress(f"batched pass: {batch_s:.3f} s")
-    # The same pass captured once as a hipGraph and replayed.
-    plan = ctx.batch_plan(ptrs, cands, 70.0)
-    plan.run()
-    ctx.synchronize()
-    times = []
-    for _ in range(3):  # the median of three replays
-        t0 = time.perf_counter()
-        gres = plan.run()
-        ctx.synchronize()
-        times.append(time.perf_counter() - t0)
-    graph_s = sorted(times)[1]
-    plan.close()
-    out["graph_seconds"] = round(graph_s, 4)
-    out["graph_search_mps"] = round(total_mp / graph_s, 2)
-    out["graph_matches_eager"] = int(all(g.chosen_quality == r.chosen_quality for g, r in zip(gres, res)))
-    if not args.no_plain:
-        # The plain search scores one pair per launch: it gets the policy tuned for that, not the batch's.
-        out["plain_tile_policy"] = ctx.autotune(1).as_dict()
-        for p in ptrs[:4]:
-            ctx.find_quality_dev(p[0], p[1], p[2], 3, 70.0, 100, rc.S420)  # warm-up under the new policy
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        steps = 0
-        for p in ptrs:
-            f = ctx.find_quality_dev(p[0], p[1], p[2], 3, 70.0, 100, rc.S420)
-            steps += f.steps
-        t1 = time.perf_counter()
-        out["plain_search_mps"] = round(total_mp / (t1 - t0), 2)
-        out["plain_eval_mps"] = round(total_mp * steps / len(ptrs) / (t1 - t0), 2)
-        out["plain_seconds"] = round(t1 - t0, 3)
-        out["batched_vs_plain"] = round(out["search_mps"] / out["plain_search_mps"], 2)
-        # The batch as one captured hipGraph replay (the form the batched configuration names) against the same plain
-        # per-image searches.
-        out["graph_vs_plain"] = round(out["graph_search_mps"] / out["plain_search_mps"], 2)
-    for d in dev:
-        d.free()
-    return out
 
 
 def bench_serve(rc, args, tune=None):

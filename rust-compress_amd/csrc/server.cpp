@@ -168,11 +168,27 @@ struct rcg_server {
         return b + work;
     }
 
-    void free_spares() {
-        for (auto& kv : spare) (void)hipFree(kv.second);
-        spare.clear();
+    // Give kept blocks back, the largest first, until at most `keep` bytes of them remain.
+    void free_spares(size_t keep = 0) {
+        size_t freed = 0;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (spare_bytes <= keep) return;
+        }
+        while (!spare.empty()) {
+            size_t now;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                now = spare_bytes - freed;
+            }
+            if (now <= keep) break;
+            auto it = std::prev(spare.end());
+            (void)hipFree(it->second);
+            freed += it->first;
+            spare.erase(it);
+        }
         std::lock_guard<std::mutex> lk(mu);
-        spare_bytes = 0;
+        spare_bytes -= freed;
     }
     int device_alloc(size_t n, void** p) {
         const size_t c = size_class(n);
@@ -224,17 +240,15 @@ struct rcg_server {
         delete r;
     }
 
-    // Admit queued requests in order while they fit beside the kept blocks (mu held); kept blocks give way to a
-    // request that would otherwise wait (free_spares() at the next tick).
-    bool admit() {
-        bool want_room = false;
+    // Admit queued requests in order while they fit (mu held). Kept blocks give way to admitted requests: the
+    // return value is how many bytes of kept blocks may remain (free_spares() before the next tick).
+    size_t admit() {
         while (!pending.empty()) {
             Request* r = pending.front();
             const size_t budget = opts.hbm_budget;
             const size_t need = std::min(r->footprint, budget);
             if (opts.max_active > 0 && int(active.size()) >= opts.max_active) break;
             if (!active.empty() && used + need > budget) break;
-            if (used + spare_bytes + need > budget && spare_bytes) want_room = true;
             pending.pop_front();
             r->admit_s = now_s();
             used += need;
@@ -242,7 +256,7 @@ struct rcg_server {
             stats.peak_footprint = std::max(stats.peak_footprint, used);
             active.push_back(r);
         }
-        return want_room;
+        return used >= opts.hbm_budget ? 0 : opts.hbm_budget - used;
     }
 
     // Take resident buffers and enqueue the source upload of a newly admitted request (worker thread, mu not held).
@@ -403,17 +417,17 @@ struct rcg_server {
         RCG_HIP_VOID(hipSetDevice(ctx->device));
         for (;;) {
             std::vector<Request*> fresh, batch;
-            bool want_room = false;
+            size_t room = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
                 if (stop) return;
                 const size_t before = active.size();
-                want_room = admit();
+                room = admit();
                 for (size_t i = before; i < active.size(); i++) fresh.push_back(active[i]);
                 batch = active;
             }
-            if (want_room) free_spares();
+            free_spares(room);
             double t0 = now_s(), t1 = t0;
             try {
                 for (Request* r : fresh)

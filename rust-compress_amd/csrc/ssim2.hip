@@ -993,7 +993,7 @@ int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const 
                       uint32_t kind_a, uint32_t kind_b, hipStream_t stream) {
     if (nunits == 0) return RCG_OK;
     const hipStream_t st = stream ? stream : ctx->stream;
-    ProfScope ps(ctx, "k_ssim_scale", st);
+    ProfScope ps(ctx, precision == RCG_PREC_FP64 ? "k_ssim_scale_f64" : "k_ssim_scale", st);
     int rc;
     if (precision == RCG_PREC_FP64) {
         Taps<double> t;
