@@ -20,6 +20,17 @@
 
 namespace rcg {
 
+// A descriptor pointer that is uniform over the wave, seen through the constant address space: loads through it are
+// invariant for the kernel, so the compiler issues them as scalar loads into SGPRs (the scalar cache) instead of
+// per-lane vector loads that each wait for a round trip to L2. Only for read-only descriptors and uniform addresses.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* const_desc(const T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<const __attribute__((address_space(4))) T*>((uint64_t(hi) << 32) | lo);
+}
+
 __device__ __forceinline__ int find_job_by_tile(const uint32_t* begins, int njobs, uint32_t bid) {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
@@ -128,8 +139,8 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
     __shared__ uint32_t s_off[16];
 
     const int tid = threadIdx.x;
-    const int j = find_job_by_tile(tile_begins, njobs, blockIdx.x);
-    const EncJob& J = jobs[j];
+    const int j = __builtin_amdgcn_readfirstlane(find_job_by_tile(tile_begins, njobs, blockIdx.x));
+    const auto& J = const_desc(jobs)[j];
     const uint32_t tile = blockIdx.x - J.tile_begin;
     const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
     const int C = J.channels;
@@ -250,9 +261,10 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
     fdct_block(blk);  // blk[u*8+v]
 
     const uint64_t blk_index = uint64_t(by) * bw_comp + bx;
+    const auto* cands = const_desc(J.cand);
     for (uint32_t ci = 0; ci < J.ncand; ci++) {
-        const EncCand& K = J.cand[ci];
-        const float* div = K.div[cls];
+        const auto& K = cands[ci];
+        const auto* div = K.div[cls];
         // Quantise: natural position n = v*8+u reads the transposed DCT output at u*8+v (encode.rs:45-54,203-205).
         int16_t* co = K.coef[comp];
         if (co) {
@@ -294,7 +306,7 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
         uint8_t* rp = K.rec[comp];
         if (rp) {
             // Dequantise straight from the rounded values (decode.rs:1317-1322), then the IDCT.
-            const float* mult = K.mult[cls];
+            const auto* mult = K.mult[cls];
             float b2[64];
 #pragma unroll
             for (int nidx = 0; nidx < 64; nidx++) {
