@@ -150,6 +150,8 @@ Layout max_layout(uint32_t w, uint32_t h, uint32_t channels, int subsampling) {
     return L;
 }
 
+constexpr size_t kTailMinPixels = size_t(1) << 19;
+
 PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
     PairPlan P{};
     P.w = w;
@@ -166,18 +168,50 @@ PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& T) {
                 P.th[s] = uint32_t(T.th[i]);
                 break;
             }
-        P.tiles_x[s] = ssim_tiles_x(sw, int(P.nt[s]));
         P.tiles_y[s] = ssim_tiles_y(sh, int(P.th[s]));
-        P.tiles[s] = P.tiles_x[s] * P.tiles_y[s];
-        // The reference-cache kernel's output does not depend on its tiling: its tiles are as tall as one round of the
-        // GPU's workgroup slots allows for this scale (at least 4 rows), so one image keeps the GPU full without a
-        // partly idle last round, and small scales walk few rows.
-        const uint32_t slots = P.nt[s] == 128 ? T.ref_slots_narrow : T.ref_slots_wide;
-        const uint32_t rows_of_tiles = std::max(1u, slots / P.tiles_x[s]);
+        // FP32: the column tiling with the fewest lanes (a lane per column plus 12 halo lanes per tile; every lane
+        // walks the same rows, so lanes measure the work) among whole tiles of one thread count (the scale's class:
+        // 256 or 128 threads, or narrower) and, on scales of at least kTailMinPixels, whole tiles followed by one
+        // column of narrower tail tiles for the columns left over (at most 52 or 116). A tail is a second launch on
+        // a second stream, joined before the next scale: on a smaller scale, whose tiles leave most of the GPU idle
+        // when one image is searched alone, that costs more than its lanes save (measured: the per-image search over
+        // 256 images of 64..3000 px was 4% slower with tails on every scale, the batched pass 0.25% faster). On a
+        // tie the single thread count wins, then the wider tiles. The FP64 kernel has one thread count for every tile.
+        const uint32_t nt_class = P.nt[s];
+        P.tnt[s] = 0;
+        P.tiles_x[s] = ssim_tiles_x(sw, int(nt_class));
+        if (!T.fixed_nt) {
+            uint32_t best = P.tiles_x[s] * nt_class, best_launches = 1;
+            auto consider = [&](uint32_t lanes, uint32_t launches, uint32_t nt, uint32_t tiles_x, uint32_t tnt) {
+                if (lanes > best || (lanes == best && launches >= best_launches)) return;
+                best = lanes;
+                best_launches = launches;
+                P.nt[s] = nt;
+                P.tiles_x[s] = tiles_x;
+                P.tnt[s] = tnt;
+            };
+            for (uint32_t nt : {256u, 128u, 64u}) {
+                if (nt > nt_class) continue;
+                const uint32_t ow = nt - 12, full = sw / ow, rest = sw - full * ow;
+                const uint32_t whole = ssim_tiles_x(sw, int(nt));
+                consider(whole * nt, 1, nt, whole, 0);
+                if (size_t(sw) * sh < kTailMinPixels || full == 0 || rest == 0) continue;
+                for (uint32_t tnt : {128u, 64u})
+                    if (tnt < nt && rest <= tnt - 12) consider(full * nt + tnt, 2, nt, full, tnt);
+            }
+        }
+        P.tiles[s] = P.tiles_x[s] * P.tiles_y[s] + (P.tnt[s] ? P.tiles_y[s] : 0);
+        // The reference-cache kernel's output does not depend on its tiling: whole tiles of the scale's class thread
+        // count, as tall as one round of the GPU's workgroup slots allows for this scale (at least 4 rows), so one
+        // image keeps the GPU full without a partly idle last round, and small scales walk few rows.
+        P.nt_ref[s] = nt_class;
+        P.tiles_x_ref[s] = ssim_tiles_x(sw, int(P.nt_ref[s]));
+        const uint32_t slots = P.nt_ref[s] == 128 ? T.ref_slots_narrow : T.ref_slots_wide;
+        const uint32_t rows_of_tiles = std::max(1u, slots / P.tiles_x_ref[s]);
         uint32_t tr = (sh + rows_of_tiles - 1) / rows_of_tiles;
         tr = std::max(4u, (tr + 1) & ~1u);
         P.th_ref[s] = tr;
-        P.tiles_ref[s] = P.tiles_x[s] * ssim_tiles_y(sh, int(tr));
+        P.tiles_ref[s] = P.tiles_x_ref[s] * ssim_tiles_y(sh, int(tr));
         P.nscales++;
         sw = (sw + 1) / 2;
         sh = (sh + 1) / 2;
@@ -261,6 +295,7 @@ struct StagedSsim {
     uint32_t kind_a0, kind_b0;
     const RefJob* ref_jobs[6];
     const SsimJob* jobs[6];
+    const SsimJob* tail_jobs[6];  // the scale's tail tiles (PairPlan::tnt), or null
     const FinalDesc* fin;
     PairPlan P;
     int precision;
@@ -299,6 +334,13 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
         j.b_next = next ? B.b_planes[s + 1] : nullptr;
         j.partials = B.partials[s];
         st.jobs[s] = D.put(&j, 1);
+        st.tail_jobs[s] = nullptr;
+        if (P.tnt[s]) {
+            j.x_begin = P.x_tail(int(s));
+            j.tiles_x = 1;
+            j.partials = B.partials[s] + size_t(P.tiles_main(int(s))) * 18;
+            st.tail_jobs[s] = D.put(&j, 1);
+        }
     }
     FinalDesc f;
     std::memset(&f, 0, sizeof(f));
@@ -321,7 +363,7 @@ StagedSsim stage_ssim(Desc& D, const PairPlan& P, const SsimSrc& a0, const SsimS
 int launch_staged_refs(rcg_ctx* ctx, const StagedSsim& st) {
     if (!st.build_ref) return RCG_OK;
     for (uint32_t s = 0; s < st.P.nscales; s++) {
-        RCG_TRY(launch_ssim_ref(ctx, st.ref_jobs[s], nullptr, st.P.tiles_ref[s], int(st.P.nt[s]),
+        RCG_TRY(launch_ssim_ref(ctx, st.ref_jobs[s], nullptr, st.P.tiles_ref[s], int(st.P.nt_ref[s]),
                                 s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes), ctx->side));
         if (s == 0) RCG_TRY(mark_side(ctx));
     }
@@ -329,16 +371,29 @@ int launch_staged_refs(rcg_ctx* ctx, const StagedSsim& st) {
     return RCG_OK;
 }
 
+// A scale's tail tiles run on the side stream beside its main tiles; the two streams join before the next scale,
+// whose tiles read the pyramid level both wrote.
 int launch_staged_ssim(rcg_ctx* ctx, const StagedSsim& st) {
     const bool f64 = st.precision == RCG_PREC_FP64;
     if (st.build_ref) RCG_TRY(wait_side_mark(ctx));
+    bool side_open = st.build_ref;  // the side stream holds work the context's stream has not joined
     for (uint32_t s = 0; s < st.P.nscales; s++) {
-        if (s == 1 && st.build_ref) RCG_TRY(join_side(ctx));
+        if (s > 0 && side_open) {
+            RCG_TRY(join_side(ctx));
+            side_open = false;
+        }
         const uint32_t ka = !f64 ? uint32_t(kSrcRefCache) : (s == 0 ? st.kind_a0 : uint32_t(kSrcPlanes));
-        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], nullptr, st.P.tiles[s], int(st.P.nt[s]), ka,
-                                  s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes)));
+        const uint32_t kb = s == 0 ? st.kind_b0 : uint32_t(kSrcPlanes);
+        if (st.tail_jobs[s]) {
+            RCG_TRY(fork_side(ctx));
+            RCG_TRY(launch_ssim_scale(ctx, st.precision, st.tail_jobs[s], nullptr, st.P.tiles_y[s],
+                                      int(st.P.tnt[s]), ka, kb, ctx->side));
+            side_open = true;
+        }
+        RCG_TRY(launch_ssim_scale(ctx, st.precision, st.jobs[s], nullptr, st.P.tiles_main(int(s)), int(st.P.nt[s]),
+                                  ka, kb));
     }
-    if (st.build_ref && st.P.nscales < 2) RCG_TRY(join_side(ctx));
+    if (side_open) RCG_TRY(join_side(ctx));
     return launch_finalize(ctx, st.fin, 1);
 }
 

@@ -155,7 +155,7 @@ RefJob make_ref_job(const PairPlan& P, uint32_t s, const SsimSrc& a0, const RefS
     j.w = P.sw[s];
     j.h = P.sh[s];
     j.th = P.th_ref[s];
-    j.tiles_x = P.tiles_x[s];
+    j.tiles_x = P.tiles_x_ref[s];
     j.tiles_y = ssim_tiles_y(P.sh[s], int(P.th_ref[s]));
     j.a = s == 0 ? a0 : planes_src(R.planes[s]);
     j.a_next = s + 1 < P.nscales ? R.planes[s + 1] : nullptr;
@@ -264,13 +264,13 @@ struct PairRef {
 
 size_t desc_bytes(rcg_ctx* ctx, const Item* items, int n) {
     // Fixed part: alignment of every put and the idle units that pad each launch's unit table (at most
-    // kXcds * RCG_MAX_CAND per launch, 24 launches per pass).
-    size_t d = (1 << 16) + 2 * 24 * kXcds * RCG_MAX_CAND * sizeof(Unit);
+    // kXcds * RCG_MAX_CAND per launch, 30 launches per pass: per scale three candidate and two reference launches).
+    size_t d = (1 << 16) + 2 * 32 * kXcds * RCG_MAX_CAND * sizeof(Unit);
     for (int i = 0; i < n; i++) {
         const Item& it = items[i];
         d += sizeof(EncJob) * RCG_MAX_CAND + sizeof(EncCand) * it.ncand + 64 * (RCG_MAX_CAND + 2);
         // FP32 pass plus, after it, an FP64 settling pass of up to every pair.
-        d += 2 * ((sizeof(SsimJob) + 64) * 6 * it.ncand + sizeof(FinalDesc) * it.ncand + 256);
+        d += 2 * ((sizeof(SsimJob) + 64) * 12 * it.ncand + sizeof(FinalDesc) * it.ncand + 256);  // main + tail jobs
         d += (sizeof(RefJob) + 64) * 6;
         d += 4 * 6 * 128;
         const PairPlan P32 = plan_pair(it.w, it.h, ctx->tune32);
@@ -278,7 +278,7 @@ size_t desc_bytes(rcg_ctx* ctx, const Item* items, int n) {
         size_t units = 0;
         // A reference launch keeps tile rows together on one XCD: its idle padding is at most kXcds - 1 tile rows.
         for (uint32_t s = 0; s < P32.nscales; s++)
-            units += size_t(P32.tiles[s]) * it.ncand + P32.tiles_ref[s] + size_t(kXcds) * P32.tiles_x[s];
+            units += size_t(P32.tiles[s]) * it.ncand + P32.tiles_ref[s] + size_t(kXcds) * P32.tiles_x_ref[s];
         for (uint32_t s = 0; s < P64.nscales; s++) units += size_t(P64.tiles[s]) * it.ncand;
         d += units * sizeof(Unit);
     }
@@ -322,7 +322,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
     if (!f64)
         for (const PairRef& pr : pairs) build[pr.item] = !items[pr.item].ref_ready;
     for (int s = 0; s < 6; s++) {
-        for (int nt : {256, 128}) {
+        for (int nt : {256, 128, 64}) {
             if (!f64 && (parts & kStageRefs)) {
                 std::vector<RefJob> rj;
                 std::vector<UnitGroup> groups;
@@ -330,11 +330,11 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     if (!build[i]) continue;
                     const Item& it = items[i];
                     const PairPlan& P = bufs[i].P32;
-                    if (uint32_t(s) >= P.nscales || int(P.nt[s]) != nt) continue;
+                    if (uint32_t(s) >= P.nscales || int(P.nt_ref[s]) != nt) continue;
                     // A tile row on one XCD, so that neighbouring tiles write the two halves of the cache lines at
                     // their seams (a tile row segment is 976 bytes) through one L2. Measured on one 4096x4096 image
                     // with six candidates: the whole step 2.3% faster.
-                    groups.push_back(UnitGroup{uint32_t(rj.size()), 1, P.tiles_ref[s], P.tiles_x[s]});
+                    groups.push_back(UnitGroup{uint32_t(rj.size()), 1, P.tiles_ref[s], P.tiles_x_ref[s]});
                     rj.push_back(make_ref_job(P, uint32_t(s), bytes_src(it.d_src, it.ch), it.ref));
                 }
                 if (!rj.empty()) {
@@ -359,13 +359,18 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                 ItemBufs& b = bufs[pairs[i].item];
                 const int c = pairs[i].cand;
                 const PairPlan& P = f64 ? b.P64 : b.P32;
-                if (uint32_t(s) >= P.nscales || int(P.nt[s]) != nt) continue;
+                if (uint32_t(s) >= P.nscales) continue;
+                // This launch holds the pair's main tiles (nt threads) or its tail tiles (tnt threads), if any.
+                const bool main = int(P.nt[s]) == nt && P.tiles_x[s] > 0;
+                const bool tail = int(P.tnt[s]) == nt;
+                if (!main && !tail) continue;
                 SsimJob j;
                 std::memset(&j, 0, sizeof(j));
                 j.w = P.sw[s];
                 j.h = P.sh[s];
                 j.th = P.th[s];
-                j.tiles_x = P.tiles_x[s];
+                j.tiles_x = main ? P.tiles_x[s] : 1;
+                j.x_begin = main ? 0 : P.x_tail(s);
                 j.tiles_y = P.tiles_y[s];
                 if (!f64)
                     j.a = cache_src(it.ref.cache[s]);
@@ -373,11 +378,12 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                     j.a = s == 0 ? bytes_src(it.d_src, it.ch) : planes_src(it.ref.planes[s]);
                 j.b = s == 0 ? recon_src(b.L[c], b.rec[c]) : planes_src(b.dist_planes[c][s]);
                 if (!f64 && uint32_t(s + 1) < P.nscales) j.b_next = b.dist_planes[c][s + 1];
-                j.partials = f64 ? b.part64[c][s] : b.part32[c][s];
+                double* const part = f64 ? b.part64[c][s] : b.part32[c][s];
+                j.partials = main ? part : part + size_t(P.tiles_main(s)) * 18;  // tail tiles after the main ones
                 FinalDesc& f = fins[i];
                 f.nscales = P.nscales;
                 f.ntiles[s] = P.tiles[s];
-                f.partials[s] = j.partials;
+                f.partials[s] = part;
                 f.count[s] = double(size_t(P.sw[s]) * P.sh[s]);
                 // The FP32 candidates of one item share the item's cache: one group, so they run side by side on one
                 // XCD. FP64 jobs read no shared data and stay single.
@@ -386,7 +392,7 @@ StagedPairs stage_pairs(Desc& D, const Item* items, int nitems, std::vector<Item
                 if (same_item)
                     groups.back().njobs++;
                 else
-                    groups.push_back(UnitGroup{uint32_t(sj.size()), 1, P.tiles[s]});
+                    groups.push_back(UnitGroup{uint32_t(sj.size()), 1, main ? P.tiles_main(s) : P.tiles_y[s]});
                 sj.push_back(j);
             }
             if (!sj.empty()) {
@@ -434,10 +440,11 @@ int launch_refs_side(rcg_ctx* ctx, const StagedPairs& sp) {
 // pass's reference state), the scale-0 launches wait for the scale-0 reference caches only and the later ones for
 // the whole side stream, so the later reference scales are built while scale 0 is scored.
 //
-// A scale with both a wide (256-thread) and a narrow (128-thread) launch runs the narrow one on the side stream,
-// beside the wide one: the narrow launch holds the few images at most 160 pixels wide at that scale, whose tiles walk
-// many rows one after another, so alone it would leave the GPU mostly idle for its duration. The two join before
-// the next scale (an image can move from the wide launch of one scale to the narrow launch of the next).
+// A scale with launches of several thread counts runs the widest on the context's stream and the narrower ones on
+// the side stream, beside it: they hold the tail tiles of wide images and the images narrow at that scale, a small
+// share of the work whose tiles walk many rows one after another, so alone they would leave the GPU mostly idle for
+// their duration. The two streams join before the next scale (an image's pyramid level is written by its main and
+// tail tiles, and an image can move between launches from one scale to the next).
 int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp, bool refs) {
     auto launch = [&](const StagedLaunch& g, hipStream_t st) {
         return launch_ssim_scale(ctx, sp.precision, static_cast<const SsimJob*>(g.jobs), g.units, g.nunits, g.nt,
@@ -449,21 +456,21 @@ int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp, bool refs) {
         const int s = sp.scales[k].scale;
         size_t e = k;
         while (e < sp.scales.size() && sp.scales[e].scale == s) e++;
-        // [k, e): this scale's launches, the wide one first when there are two.
-        const bool split = e - k == 2 && sp.scales[k].nt == 256 && sp.scales[k + 1].nt == 128;
+        // [k, e): this scale's launches, widest first. The widest runs on the context's stream, the narrower ones
+        // (few images at most 160 pixels wide at this scale, and the tail tiles of the others) on the side stream.
         if (s > 0 && side_open) {
-            RCG_TRY(join_side(ctx));  // the previous scale's narrow launch (at scale 1 also the reference chain)
+            RCG_TRY(join_side(ctx));  // the previous scale's narrow launches (at scale 1 also the reference chain)
             side_open = false;
         }
-        if (split) {
-            // The side stream runs the narrow launch after everything enqueued here so far (the encode, earlier
-            // scales) and, at scale 0, after the reference chain it already holds.
+        if (e - k > 1) {
+            // The side stream runs them after everything enqueued here so far (the encode, earlier scales) and, at
+            // scale 0, after the reference chain it already holds.
             RCG_TRY(fork_side(ctx));
-            RCG_TRY(launch(sp.scales[k + 1], ctx->side));
+            for (size_t i = k + 1; i < e; i++) RCG_TRY(launch(sp.scales[i], ctx->side));
             side_open = true;
         }
         if (s == 0 && refs) RCG_TRY(wait_side_mark(ctx));
-        for (size_t i = k; i < (split ? k + 1 : e); i++) RCG_TRY(launch(sp.scales[i], nullptr));
+        RCG_TRY(launch(sp.scales[k], nullptr));
         k = e;
     }
     if (side_open) RCG_TRY(join_side(ctx));
@@ -519,11 +526,11 @@ void retile_refs(const SsimTune& T, const Item* items, int n, std::vector<ItemBu
         for (uint32_t nt : {256u, 128u}) {
             auto builds = [&](int i) {
                 const PairPlan& P = bufs[i].P32;
-                return !items[i].ref_ready && items[i].ncand > 0 && s < P.nscales && P.nt[s] == nt;
+                return !items[i].ref_ready && items[i].ncand > 0 && s < P.nscales && P.nt_ref[s] == nt;
             };
             uint32_t cols = 0;
             for (int i = 0; i < n; i++)
-                if (builds(i)) cols += bufs[i].P32.tiles_x[s];
+                if (builds(i)) cols += bufs[i].P32.tiles_x_ref[s];
             if (cols == 0) continue;
             const uint32_t slots = nt == 128 ? T.ref_slots_narrow : T.ref_slots_wide;
             const uint32_t rows_of_tiles = std::max(1u, slots / cols);
@@ -534,7 +541,7 @@ void retile_refs(const SsimTune& T, const Item* items, int n, std::vector<ItemBu
                 tr = std::min(kRefMaxRows, std::max(4u, (tr + 1) & ~1u));
                 if (tr > P.th_ref[s]) {
                     P.th_ref[s] = tr;
-                    P.tiles_ref[s] = P.tiles_x[s] * ssim_tiles_y(P.sh[s], int(tr));
+                    P.tiles_ref[s] = P.tiles_x_ref[s] * ssim_tiles_y(P.sh[s], int(tr));
                 }
             }
         }

@@ -250,12 +250,21 @@ inline uint32_t ssim_tiles_x(uint32_t w, int nt) { return (w + (nt - 12) - 1) / 
 inline uint32_t ssim_tiles_y(uint32_t h, int th) { return (h + th - 1) / th; }
 
 // One SSIMULACRA2 evaluation of an image pair through all of its scales, described for the batched launcher.
+// The FP32 candidate kernel covers a scale with tiles_x tiles of nt threads (nt - 12 output columns each; nt is 256,
+// 128 or 64) from column 0, then, when tnt is not 0, one column of narrower tail tiles of tnt threads for the columns
+// left over, so that a width that is not a multiple of nt - 12 leaves few lanes idle. Partial sums are stored main
+// tiles first, then tail tiles: tiles[s] counts both. The reference-cache kernel has a tiling of its own (nt_ref
+// threads, the scale's class: 128 up to the policy's narrow width, else 256; its output does not depend on it).
 struct PairPlan {
     uint32_t w, h, nscales;
     uint32_t sw[6], sh[6];
     uint32_t nt[6], th[6];
     uint32_t tiles[6], tiles_x[6], tiles_y[6];
-    uint32_t th_ref[6], tiles_ref[6];  // tiling of the reference-cache kernel (same columns, its own rows)
+    uint32_t tnt[6];                   // threads of the tail tiles, 0 for none
+    uint32_t nt_ref[6], tiles_x_ref[6];
+    uint32_t th_ref[6], tiles_ref[6];  // tiling of the reference-cache kernel (its own columns and rows)
+    uint32_t tiles_main(int s) const { return tiles_x[s] * tiles_y[s]; }
+    uint32_t x_tail(int s) const { return tiles_x[s] * (nt[s] - 12); }
 };
 PairPlan plan_pair(uint32_t w, uint32_t h, const SsimTune& tune);
 

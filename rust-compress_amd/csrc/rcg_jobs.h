@@ -62,6 +62,7 @@ struct SsimJob {
     uint32_t w, h;          // size at this scale
     uint32_t th;            // output rows per tile (even)
     uint32_t tiles_x, tiles_y;
+    uint32_t x_begin;       // first output column of the job's tiles (a tail job covers the image's last columns)
     SsimSrc a, b;           // a = reference, b = distorted
     float* a_next;          // next scale's planes [3][ceil(h/2)][ceil(w/2)] or null
     float* b_next;

@@ -657,7 +657,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     st.tid = tid;
     st.w = int(J.w);
     st.h = int(J.h);
-    st.x0 = int(tx) * OW;
+    st.x0 = int(J.x_begin) + int(tx) * OW;
     const int TH = int(J.th);
     st.y0 = int(ty) * TH;
     st.gx = st.x0 - 6 + tid;
@@ -1002,8 +1002,9 @@ int launch_ssim_scale(rcg_ctx* ctx, int precision, const SsimJob* d_jobs, const 
     } else {
         Taps<float> t;
         for (int i = 0; i < 6; i++) t.k[i] = ctx->taps32[i];
-        rc = nt == 128 ? launch_kinds<float, 128>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b)
-                       : launch_kinds<float, 256>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b);
+        rc = nt == 64    ? launch_kinds<float, 64>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b)
+             : nt == 128 ? launch_kinds<float, 128>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b)
+                         : launch_kinds<float, 256>(ctx, st, d_jobs, d_units, nunits, t, kind_a, kind_b);
     }
     if (rc != RCG_OK) return rc;
     RCG_HIP(hipGetLastError());

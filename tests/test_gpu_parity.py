@@ -253,6 +253,29 @@ def test_batch_scores_do_not_depend_on_the_rest_of_the_batch(ctx, rc):
         assert r.chosen_quality == alone.chosen_quality
 
 
+def test_column_tilings_agree_across_paths_and_with_the_oracle(ctx, rc):
+    # Scale-0 column tilings of the FP32 kernel: 1000x600 four 256-thread tiles and a 64-thread tail column,
+    # 1050x600 four 256-thread tiles and a 128-thread tail column, 348x400 three 128-thread tiles, 52x200 one
+    # 64-thread tile, 1400x380 six 256-thread tiles (the later scales take other forms). Each image's scores in a
+    # batch with the others equal its scores alone bit for bit, lie within the FP32 bound of the oracle, and the
+    # pair entry point on the oracle's decode gives the batch's score bit for bit.
+    sizes = [(1000, 600), (1050, 600), (348, 400), (52, 200), (1400, 380)]
+    imgs = [rc.synth_rgb8(70 + i, w, h) for i, (w, h) in enumerate(sizes)]
+    cands = [(50, rc.S420), (85, rc.S420)]
+    together = ctx.score_batch(imgs, cands, 70.0)
+    for img, r in zip(imgs, together):
+        alone = ctx.score_batch([img], cands, 70.0)[0]
+        assert r.status == 0 and alone.status == 0
+        assert list(r.scores[:2]) == list(alone.scores[:2]), img.shape
+        for k, (q, sub) in enumerate(cands):
+            dist = O.reconstruct(O.build_coefs(img, q, sub))
+            want = O.ssimulacra2(img, dist)
+            pair = ctx.ssimulacra2(img, dist, rc.FP32)
+            assert abs(pair - want) <= TOL32, (img.shape, q, pair, want)
+            if not r.settled[k]:
+                assert r.scores[k] == pair, (img.shape, q, r.scores[k], pair)
+
+
 def test_large_call_scores_equal_single_candidate_calls(ctx, rc):
     # A call of 2^25 candidate-pixels or more uploads its descriptors in two parts and stages the candidate launches
     # while the encode and reference launches run; a call below that (one candidate of the same image) uploads them
