@@ -29,9 +29,10 @@ def build():
 
 
 def lib():
+    """The oracle library; RCG_ORACLE_LIB names another build of oracle.c (the sanitizer test loads its own)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.environ.get("RCG_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)

@@ -61,7 +61,42 @@ def build(verbose=False):
     # C++ tests of the host API (host/rc_gpu.hpp), linked against the library.
     _run(["g++", "-std=c++17", "-O2", "-Wall", API_TEST_SRC, "-o", API_TEST_BIN, "-L" + HERE, "-lrcgpu",
           "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../rust-compress_amd"], verbose)
+    build_sanitized(objs, verbose)
     return LIB
+
+
+# Sanitizer build of the host code: every C++ source compiled again with the host side instrumented by
+# AddressSanitizer and UndefinedBehaviorSanitizer (-Xarch_host: device code is never instrumented), linked with the
+# kernels' objects of the regular build into two drivers:
+#   tests/cpp/host_sanitize      the host reader, writer, requantisation and trellis on the reference's JPEG fixtures
+#                                and mutations of them (tests/test_host_sanitizers.py, no device needed);
+#   tests/cpp/server_race_asan   the serving queue with concurrent submitting threads (tests/test_gpu_server_race.py).
+CLANG = os.environ.get("RCG_CLANGXX", "/opt/rocm/llvm/bin/clang++")
+HOST_SOURCES = [s for s in SOURCES if s.endswith(".cpp")]
+CHECKS = ["address", "undefined"]
+DRIVERS = [(os.path.join(ROOT, "tests", "cpp", "host_sanitize.cpp"), os.path.join(ROOT, "tests", "cpp", "host_sanitize")),
+           (os.path.join(ROOT, "tests", "cpp", "server_race.cpp"),
+            os.path.join(ROOT, "tests", "cpp", "server_race_asan"))]
+
+
+def build_sanitized(kernel_objs, verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
+    kobjs = [o for o, s in zip(kernel_objs, SOURCES) if not s.endswith(".cpp")]
+    flags = [f for f in FLAGS if f != "-O3"] + ["-O1", "-g", "-fno-omit-frame-pointer"]
+    for c in CHECKS:
+        flags += ["-Xarch_host", "-fsanitize=" + c]
+    flags += ["-Xarch_host", "-fno-sanitize-recover=undefined"]
+    d = os.path.join(OBJ_DIR, "asan")
+    os.makedirs(d, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES] + [drv for drv, _ in DRIVERS]
+    objs = [os.path.join(d, os.path.splitext(os.path.basename(src))[0] + ".o") for src in srcs]
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as ex:
+        for f in [ex.submit(_run, [HIPCC] + flags + ["-c", src, "-o", o], verbose) for src, o in zip(srcs, objs)]:
+            f.result()
+    lib_objs = objs[:len(HOST_SOURCES)] + kobjs
+    for (drv, out), o in zip(DRIVERS, objs[len(HOST_SOURCES):]):
+        _run([CLANG, "-fsanitize=" + ",".join(CHECKS), "-o", out, o] + lib_objs +
+             ["-L/opt/rocm/lib", "-lamdhip64", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"], verbose)
 
 
 if __name__ == "__main__":
