@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the precision report")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-image search")
     ap.add_argument("--no-dropin", action="store_true", help="skip the plain drop-in (one 4096x4096 image) figures")
+    ap.add_argument("--dropin-only", action="store_true",
+                    help="only the plain drop-in figures, printed as {\"dropin\": ...} (for profiling that leg alone)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
     ap.add_argument("--cli-files", type=int, default=24)
@@ -212,6 +214,13 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(images would shard across GPUs with no collective; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.dropin_only:
+        rc = importlib.import_module("rust-compress_amd")
+        ctx = rc.Context(0)
+        print(json.dumps({"dropin": bench_dropin(rc, ctx, args)}), flush=True)
+        ctx.close()
+        return
 
     # A batch launch holds many pairs of every size and runs many rounds of workgroups: its tile policy is tuned for
     # throughput (time per pair over launches of many rounds). Tuned in a child, so that the counter passes (children
