@@ -308,7 +308,7 @@ int check_image(uint32_t w, uint32_t h, uint32_t channels);
 
 // The reference side of an image's f32 scoring, built once per image and read by every candidate: its linear-light
 // pyramid (planes[s], [3][h_s][w_s] for scales 1..; the f64 kernel reads it too) and its per-scale cache (cache[s],
-// [9][h_s][w_s], written by k_ssim_ref).
+// 9 * w_s * h_s floats, written by k_ssim_ref).
 struct RefState {
     float* planes[6];
     float* cache[6];

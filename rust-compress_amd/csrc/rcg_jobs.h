@@ -44,8 +44,8 @@ struct PlaneView {
 };
 
 // Source of linear-light RGB for one image at one SSIMULACRA2 scale.
-// kSrcRefCache: the reference's per-scale cache (planes [9][h][w]: shifted XYB, blurred XYB, blurred XYB^2), read
-// by candidate kernels in place of the reference's pixels.
+// kSrcRefCache: the reference's per-scale cache (9 floats per pixel: the shifted XYB, its blur and sigma_aa + C2, in
+// the plane layout of Loader<kSrcRefCache> in ssim2.hip), read by candidate kernels in place of the reference's pixels.
 enum SrcKind : uint32_t { kSrcBytes = 0, kSrcRecon = 1, kSrcPlanes = 2, kSrcRefCache = 3 };
 
 struct SsimSrc {
@@ -75,7 +75,7 @@ struct RefJob {
     uint32_t tiles_x, tiles_y;
     SsimSrc a;
     float* a_next;          // next scale's planes [3][ceil(h/2)][ceil(w/2)] or null
-    float* cache;           // [9][h][w]
+    float* cache;           // 9 * w * h floats (layout of Loader<kSrcRefCache>)
 };
 
 // Workgroup b of an SSIMULACRA2 launch processes tile units[b].tile of job units[b].job (job == kIdleJob: none); a

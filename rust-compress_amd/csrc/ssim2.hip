@@ -288,8 +288,10 @@ struct Loader<kSrcRecon> {
     const uint8_t* ry;
     const uint8_t* ra[2];   // chroma rows of the current row: nearer
     const uint8_t* rb[2];   // and farther (equal to ra unless vertically subsampled)
-    uint32_t ox, oi, ofi;   // the thread's luma column, its chroma column and the farther chroma column
-    int32_t rnd2, rnd4;     // the horizontal rounding terms of the thread's column parity
+    // The thread's luma column, and its chroma column and the farther chroma column packed as 16-bit halves: the
+    // candidate kernel holds its registers at the limit of four waves, and unpacking (and the rounding term of the
+    // column's parity, from ox) costs a few instructions per row.
+    uint32_t ox, ocp;
     __device__ __forceinline__ void init(const SsimSrc& S, uint32_t, uint32_t) {
         ncomp = S.ncomp;
         py = S.comp[0].p;
@@ -304,16 +306,14 @@ struct Loader<kSrcRecon> {
     }
     __device__ __forceinline__ void prep(uint32_t x) {
         ox = x;
-        if (fx == 1) {
-            oi = ofi = x;
-        } else {
+        uint32_t oi = x, ofi = x;
+        if (fx != 1) {
             oi = x >> 1;
             ofi = (x & 1) == 0 ? (oi > 0 ? oi - 1 : 0) : min(oi + 1, cw - 1);
         }
-        rnd2 = (x & 1) ? 2 : 1;
-        rnd4 = (x & 1) ? 7 : 8;
+        ocp = oi | (ofi << 16);  // chroma planes are at most 32768 samples wide
     }
-    __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox), "+v"(oi), "+v"(ofi)); }
+    __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox), "+v"(ocp)); }
     __device__ __forceinline__ void begin_row(uint32_t r) {
         ry = py + uint64_t(r) * stride_y;
         uint64_t c0, c1;
@@ -330,16 +330,17 @@ struct Loader<kSrcRecon> {
             rb[k] = pc[k] + c1;
         }
     }
-    __device__ __forceinline__ int32_t chroma(int k) const {
+    __device__ __forceinline__ int32_t chroma(int k, uint32_t oi, uint32_t ofi) const {
         const uint8_t* a = ra[k];
         const uint8_t* b = rb[k];
+        const int32_t odd = int32_t(ox & 1);
         if (fx == 1) return gload_b(a, oi);
         if (fy == 2) {
             const int32_t nearv = 3 * int32_t(gload_b(a, oi)) + int32_t(gload_b(b, oi));
             const int32_t far = 3 * int32_t(gload_b(a, ofi)) + int32_t(gload_b(b, ofi));
-            return (3 * nearv + far + rnd4) >> 4;
+            return (3 * nearv + far + 8 - odd) >> 4;
         }
-        return (3 * int32_t(gload_b(a, oi)) + int32_t(gload_b(a, ofi)) + rnd2) >> 2;
+        return (3 * int32_t(gload_b(a, oi)) + int32_t(gload_b(a, ofi)) + 1 + odd) >> 2;
     }
     __device__ __forceinline__ void load(const float* lut, float out[3]) const {
         const int32_t y = gload_b(ry, ox);
@@ -348,23 +349,30 @@ struct Loader<kSrcRecon> {
             return;
         }
         int32_t r, g, b;
-        ycc_to_rgb(y, chroma(0), chroma(1), r, g, b);
+        const uint32_t oi = ocp & 0xFFFFu, ofi = ocp >> 16;
+        ycc_to_rgb(y, chroma(0, oi, ofi), chroma(1, oi, ofi), r, g, b);
         out[0] = lut[r];
         out[1] = lut[g];
         out[2] = lut[b];
     }
 };
 
-// The reference image's per-scale cache written by k_ssim_ref: planes [9][h][w] holding the shifted XYB (0-2),
-// its blur mu_a (3-5) and sigma_aa + C2 = blur(a^2) - mu_a^2 + 0.0009 (6-8). load() returns XYB, not linear RGB.
+// The reference image's per-scale cache written by k_ssim_ref, 9 floats per pixel in planes of w x h elements: M =
+// float4 (mu_X, mu_Y, mu_B, s_X), S = float2 (s_Y, s_B), then A_X, A_Y, A_B = float, at kCacheS, kCacheA floats per
+// pixel from the start; a is the shifted XYB, mu = blur(a) and s = sigma_aa + C2 = blur(a^2) - mu^2 + 0.0009. Every
+// store and read of a lane covers 16, 8 or 4 consecutive bytes, so each access of a wave spans whole lines. The input
+// rows read a alone (three planes, as the ring needs), the output rows all nine values in five loads.
+constexpr uint32_t kCacheS = 4, kCacheA = 6;  // plane offsets in floats per pixel of the image
+typedef float f32x4 __attribute__((ext_vector_type(4)));  // native vectors (HIP's float4 and float2 are structs)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 template <>
 struct Loader<kSrcRefCache> {
     const float* base;
     const float* row;
-    const float* orow;
     uint64_t n;
     uint32_t w;
-    uint32_t ox;
+    uint32_t ox;    // byte offset of the lane's pixel in the float planes (twice and four times it in S and M)
+    uint64_t orow;  // element index of the current output row's first pixel
     __device__ __forceinline__ void init(const SsimSrc& S, uint32_t w_, uint32_t h) {
         base = S.planes;
         w = w_;
@@ -372,21 +380,26 @@ struct Loader<kSrcRefCache> {
     }
     __device__ __forceinline__ void prep(uint32_t x) { ox = 4 * x; }
     __device__ __forceinline__ void fresh() { asm volatile("" : "+v"(ox)); }
-    __device__ __forceinline__ void begin_row(uint32_t r) { row = base + uint64_t(r) * w; }
+    __device__ __forceinline__ void begin_row(uint32_t r) { row = base + kCacheA * n + uint64_t(r) * w; }
     __device__ __forceinline__ void load(const float*, float out[3]) const {
         out[0] = gload_b(row, ox);
         out[1] = gload_b(row + n, ox);
         out[2] = gload_b(row + 2 * n, ox);
     }
-    // a, mu_a and sigma_aa at an output row.
-    __device__ __forceinline__ void begin_out_row(uint32_t r) { orow = base + uint64_t(r) * w; }
+    // a, mu_a and sigma_aa + C2 at an output row.
+    __device__ __forceinline__ void begin_out_row(uint32_t r) { orow = uint64_t(r) * w; }
     __device__ __forceinline__ void load_stats(float a[3], float mu[3], float sig[3]) const {
+        const f32x4 m = gload_b(reinterpret_cast<const f32x4*>(base) + orow, ox << 2);
+        const f32x2 sv = gload_b(reinterpret_cast<const f32x2*>(base + kCacheS * n) + orow, ox << 1);
+        const float* ar = base + kCacheA * n + orow;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            a[c] = gload_b(orow + c * n, ox);
-            mu[c] = gload_b(orow + (3 + c) * n, ox);
-            sig[c] = gload_b(orow + (6 + c) * n, ox);
-        }
+        for (int c = 0; c < 3; c++) a[c] = gload_b(ar + c * n, ox);
+        mu[0] = m.x;
+        mu[1] = m.y;
+        mu[2] = m.z;
+        sig[0] = m.w;
+        sig[1] = sv.x;
+        sig[2] = sv.y;
     }
 };
 
@@ -726,7 +739,7 @@ struct RefWalk {
     float* s_q;
     uint64_t n;
     int tid, w, h, x0, y0, gx, y_end, ylast, nw, nh;
-    uint32_t oc, po;  // byte offsets of the lane's column in the cache planes and in the next pyramid level
+    uint32_t oc, po;  // byte offsets of the lane's column in the cache's float planes and in the next pyramid level
     bool in_img, is_out, ds_writer;
     int yy;
     float ra[11][3];
@@ -782,8 +795,8 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
         }
         __syncthreads();
         if (st.is_out) {
-            float* o = st.cache + uint64_t(yy - 5) * st.w;  // uniform row; the lane's column is the offset
-            const uint64_t n = st.n;
+            const uint64_t orow = uint64_t(yy - 5) * st.w;  // uniform row; the lane's column is the offset
+            float mu[3], sg[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float v[2];
@@ -798,13 +811,25 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
                     for (int t = 1; t <= 5; t++) sum = mad(k[t], row[-t * Q::kCol] + row[t * Q::kCol], sum);
                     v[i] = sum;
                 }
-                // Non-temporal: the cache is written once and read back only by later launches. Measured on one
-                // 4096x4096 image with six candidates, where this launch runs beside the encode launch, the whole
-                // pass was about 2% shorter than with ordinary stores.
-                gstore_nt_b(o + c * n, st.oc, st.ra[SC][c]);
-                gstore_nt_b(o + (3 + c) * n, st.oc, v[0]);
-                gstore_nt_b(o + (6 + c) * n, st.oc, mad(-v[0], v[0], v[1]) + 0.0009f);  // sigma_aa + C2
+                mu[c] = v[0];
+                sg[c] = mad(-v[0], v[0], v[1]) + 0.0009f;  // sigma_aa + C2
             }
+            // Five stores in the layout of Loader<kSrcRefCache>, each covering whole lines across the wave (nine 4-byte
+            // planes took the launch 8.5 ms on the 256-image batch, these five 7.0 ms). Non-temporal: the cache is
+            // written once and read back only by later launches (on one 4096x4096 image with six candidates, where this
+            // launch runs beside the encode launch, the whole step was about 2% shorter than with ordinary stores).
+            f32x4 m;
+            m.x = mu[0];
+            m.y = mu[1];
+            m.z = mu[2];
+            m.w = sg[0];
+            f32x2 sv;
+            sv.x = sg[1];
+            sv.y = sg[2];
+            gstore_nt_b(reinterpret_cast<f32x4*>(st.cache) + orow, st.oc << 2, m);
+            gstore_nt_b(reinterpret_cast<f32x2*>(st.cache + kCacheS * st.n) + orow, st.oc << 1, sv);
+#pragma unroll
+            for (int c = 0; c < 3; c++) gstore_nt_b(st.cache + (kCacheA + c) * st.n + orow, st.oc, st.ra[SC][c]);
         }
     }
     st.yy = yy + 1;
@@ -812,8 +837,8 @@ __device__ __forceinline__ bool ref_row(RefWalk<NT, KA>& st) {
 }
 
 template <int NT, uint32_t KA>
-__global__ __launch_bounds__(NT) void k_ssim_ref(const RefJob* __restrict__ jobs, const Unit* __restrict__ units,
-                                                 Taps<float> taps, const float* __restrict__ lut_g) {
+__global__ __launch_bounds__(NT) void k_ssim_ref(
+    const RefJob* __restrict__ jobs, const Unit* __restrict__ units, Taps<float> taps, const float* __restrict__ lut_g) {
     constexpr int OW = NT - 12;
     __shared__ float s_q[2 * RefWalk<NT, KA>::Q::kBuf];
     __shared__ float s_lut[256];
