@@ -457,7 +457,7 @@ int launch_pairs(rcg_ctx* ctx, const StagedPairs& sp, bool refs) {
         size_t e = k;
         while (e < sp.scales.size() && sp.scales[e].scale == s) e++;
         // [k, e): this scale's launches, widest first. The widest runs on the context's stream, the narrower ones
-        // (few images at most 160 pixels wide at this scale, and the tail tiles of the others) on the side stream.
+        // (the images tiled narrower at this scale, and the tail tiles of the others) on the side stream.
         if (s > 0 && side_open) {
             RCG_TRY(join_side(ctx));  // the previous scale's narrow launches (at scale 1 also the reference chain)
             side_open = false;
