@@ -227,8 +227,10 @@ int rcg_set_tune(rcg_ctx* ctx, const rcg_tune* tune);
 int rcg_autotune(rcg_ctx* ctx, int jobs, rcg_tune* out);
 
 /*
- * Serving loop with continuous batching. A worker thread owns one context. Requests (an image and a
- * --target-ssimulacra2 goal) are admitted in arrival order while their device footprint fits the HBM budget, the
+ * Serving loop with continuous batching. Two lanes, each a worker thread with a context of its own, serve the
+ * requests below 2 MP and those of 2 MP or more, so that a small request's ticks do not wait for a large image's;
+ * the lanes' ticks run side by side on the GPU. Requests (an image and a --target-ssimulacra2 goal) are admitted in
+ * arrival order within their lane while the device footprint of both lanes fits the HBM budget, the
  * GPU analogue of cli/src/budget.rs:11-50: a request that alone exceeds the budget runs, but by itself. Every tick
  * scores the next candidate(s) of every admitted request's bisection (jpeg/src/target.rs:37-55) in one batched
  * launch sequence; requests that arrive meanwhile join at the next tick. Sources and reference pyramids stay
@@ -247,7 +249,7 @@ typedef struct {
     size_t hbm_budget;              /* bytes for admitted requests; 0 = 16 GiB */
     int max_active;                 /* requests in flight at once; 0 = no limit */
     int speculate;                  /* 1 or 3 */
-    const rcg_tune* tune;           /* tile policy of the worker's context, or NULL for the default one */
+    const rcg_tune* tune;           /* tile policy of the workers' contexts, or NULL for the default one */
 } rcg_server_opts;
 typedef struct {
     uint64_t id;                    /* caller's tag, returned in the completion */
@@ -274,7 +276,7 @@ typedef struct {
     uint64_t ticks, requests_done;
     double mean_batch;              /* mean requests per tick */
     size_t peak_footprint;          /* largest admitted footprint, bytes */
-    double busy_s;                  /* time spent inside ticks */
+    double busy_s;                  /* time spent inside ticks, summed over the two lanes */
 } rcg_server_stats;
 int rcg_server_create(int device, const rcg_server_opts* opts, rcg_server** out);
 /* RCG_INVALID / RCG_UNSUPPORTED for a request that can never run (bad size, under 8x8); nothing is queued then. */

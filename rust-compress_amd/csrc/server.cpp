@@ -1,9 +1,10 @@
 // Serving loop (rcg_server_*): continuous batching of target-quality searches on one GPU.
 //
-// The worker thread repeats ticks. Before a tick it admits queued requests in arrival order while the admitted
-// footprint (resident source + reference pyramid + the largest per-tick working set of its candidates) fits the
-// HBM budget; like cli/src/budget.rs:34-43, a request larger than the whole budget is admitted only when nothing
-// else is running. A tick gathers the next candidate(s) of every admitted request's bisection
+// Two lanes, each a context (stream pair) and a worker thread, serve requests below and above kLargePixels; their
+// ticks run side by side on the GPU. A lane's worker repeats ticks. Before a tick it admits its class's queued
+// requests in arrival order while the admitted footprint of both lanes (resident source + reference pyramid + the
+// largest per-tick working set of its candidates) fits the HBM budget; like cli/src/budget.rs:34-43, a request larger
+// than the whole budget is admitted only when nothing else is running. A tick gathers the next candidate(s) of every admitted request's bisection
 // (jpeg/src/target.rs:37-55) and scores them all with one score_items pass (one encode launch, one launch per
 // SSIMULACRA2 scale, one pooling launch, FP64 settling near each target). Finished requests leave the batch and
 // free their memory; requests that arrived meanwhile join at the next tick. A pair's score depends only on its own
@@ -116,26 +117,40 @@ int test_throw_tick() {
 
 }  // namespace
 
+// Requests of at least kLargePixels run on a lane of their own: a second context, stream pair and worker thread. A
+// tick lasts as long as its largest image takes, so small requests scored in the same ticks as a 24 MP image waited
+// for it at every step of their bisection; on their own lane their ticks run beside the large ones on the GPU.
+constexpr size_t kLargePixels = size_t(2) << 20;
+constexpr int kLanes = 2;
+
 struct rcg_server {
-    rcg_ctx* ctx = nullptr;
+    struct Lane {
+        rcg_ctx* ctx = nullptr;
+        std::vector<Request*> active;  // admitted requests of this lane (mu held)
+        std::thread worker;
+    };
+    Lane lanes[kLanes];
     rcg_server_opts opts{};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
-    std::vector<Request*> active;
     std::deque<rcg_completion> done;
-    size_t used = 0;
+    size_t used = 0;      // admitted footprints (mu held)
+    size_t nactive = 0;   // admitted requests over both lanes (mu held)
+    uint64_t epoch = 0;   // bumped when a request is submitted or finishes (mu held): a lane that could admit nothing
+                          // waits for it to change
     // Device blocks of finished requests, kept for reuse by size class: hipMalloc and hipFree of blocks of many
-    // megabytes take milliseconds each, and on the worker thread they stalled the ticks (a sustained 200 requests/s
-    // spent two thirds of its time there). Touched by the worker thread only (spare_bytes also read under mu by
-    // admit()). Admitted footprints plus kept blocks stay within hbm_budget.
+    // megabytes take milliseconds each, and on a worker thread they stalled the ticks (a sustained 200 requests/s
+    // spent two thirds of its time there). Admitted footprints plus kept blocks stay within hbm_budget (mu held;
+    // hipMalloc and hipFree are called outside the lock).
     std::multimap<size_t, void*> spare;
     size_t spare_bytes = 0;
     bool stop = false;
-    std::thread worker;
     rcg_server_stats stats{};
     uint64_t batch_sum = 0;
     int throw_tick = 0;
+
+    static int lane_of(const rcg_request& r) { return size_t(r.width) * r.height >= kLargePixels ? 1 : 0; }
 
     // Eighth-of-an-octave size classes (at most 12.5% slack), 1 MiB at least.
     static size_t size_class(size_t n) {
@@ -146,8 +161,9 @@ struct rcg_server {
     }
 
     // Device bytes a request holds while admitted: its resident blocks as allocated (size classes) and the largest
-    // per-tick working set of its candidates.
+    // per-tick working set of its candidates. Both lanes' contexts carry the same tile policy.
     size_t footprint_of(const rcg_request& r) const {
+        rcg_ctx* ctx = lanes[0].ctx;
         const PairPlan P = plan_pair(r.width, r.height, ctx->tune32);
         size_t b = size_class(size_t(r.width) * r.height * r.channels) + size_class(ref_state_bytes(P));
         // Under Auto a tick's candidates are 4:4:4 (q >= 90) or 4:2:0 (q < 90): charge the larger working set.
@@ -170,35 +186,29 @@ struct rcg_server {
 
     // Give kept blocks back, the largest first, until at most `keep` bytes of them remain.
     void free_spares(size_t keep = 0) {
-        size_t freed = 0;
+        std::vector<void*> out;
         {
             std::lock_guard<std::mutex> lk(mu);
-            if (spare_bytes <= keep) return;
-        }
-        while (!spare.empty()) {
-            size_t now;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                now = spare_bytes - freed;
+            while (spare_bytes > keep && !spare.empty()) {
+                auto it = std::prev(spare.end());
+                spare_bytes -= it->first;
+                out.push_back(it->second);
+                spare.erase(it);
             }
-            if (now <= keep) break;
-            auto it = std::prev(spare.end());
-            (void)hipFree(it->second);
-            freed += it->first;
-            spare.erase(it);
         }
-        std::lock_guard<std::mutex> lk(mu);
-        spare_bytes -= freed;
+        for (void* p : out) (void)hipFree(p);
     }
     int device_alloc(size_t n, void** p) {
         const size_t c = size_class(n);
-        auto it = spare.find(c);
-        if (it != spare.end()) {
-            *p = it->second;
-            spare.erase(it);
+        {
             std::lock_guard<std::mutex> lk(mu);
-            spare_bytes -= c;
-            return RCG_OK;
+            auto it = spare.find(c);
+            if (it != spare.end()) {
+                *p = it->second;
+                spare.erase(it);
+                spare_bytes -= c;
+                return RCG_OK;
+            }
         }
         if (hipMalloc(p, c) == hipSuccess) return RCG_OK;
         (void)hipGetLastError();
@@ -233,6 +243,8 @@ struct rcg_server {
         c.done_s = t;
         c.coefs_written = r->coefs_written;
         used -= std::min(used, r->footprint);
+        nactive--;
+        epoch++;
         device_release(r->d_src, r->src_bytes);
         device_release(r->d_ref, r->ref_bytes);
         done.push_back(c);
@@ -240,29 +252,44 @@ struct rcg_server {
         delete r;
     }
 
-    // Admit queued requests in order while they fit (mu held). Kept blocks give way to admitted requests: the
-    // return value is how many bytes of kept blocks may remain (free_spares() before the next tick).
-    size_t admit() {
-        while (!pending.empty()) {
-            Request* r = pending.front();
+    // mu held.
+    bool has_pending(int lane) const {
+        for (const Request* r : pending)
+            if (lane_of(r->req) == lane) return true;
+        return false;
+    }
+
+    // Admit queued requests of the lane's class in arrival order while they fit (mu held); the footprints of both
+    // lanes share the budget, and a request larger than the whole budget is admitted only when no request is admitted
+    // on either lane. Kept blocks give way to admitted requests: the return value is how many bytes of kept blocks
+    // may remain (free_spares() before the next tick).
+    size_t admit(int lane) {
+        Lane& L = lanes[lane];
+        for (auto it = pending.begin(); it != pending.end();) {
+            Request* r = *it;
+            if (lane_of(r->req) != lane) {
+                ++it;
+                continue;
+            }
             const size_t budget = opts.hbm_budget;
             const size_t need = std::min(r->footprint, budget);
-            if (opts.max_active > 0 && int(active.size()) >= opts.max_active) break;
-            if (!active.empty() && used + need > budget) break;
-            pending.pop_front();
+            if (opts.max_active > 0 && int(nactive) >= opts.max_active) break;
+            if (nactive > 0 && used + need > budget) break;
+            it = pending.erase(it);
             r->admit_s = now_s();
             used += need;
             r->footprint = need;
+            nactive++;
             stats.peak_footprint = std::max(stats.peak_footprint, used);
-            active.push_back(r);
+            L.active.push_back(r);
         }
         return used >= opts.hbm_budget ? 0 : opts.hbm_budget - used;
     }
 
     // Take resident buffers and enqueue the source upload of a newly admitted request (worker thread, mu not held).
-    // The upload is ordered before the request's first tick on the context's stream; the host pixels are released
-    // once that tick has synchronised the stream.
-    int materialise(Request* r) {
+    // The upload is ordered before the request's first tick on the lane context's stream; the host pixels are
+    // released once that tick has synchronised the stream.
+    int materialise(rcg_ctx* ctx, Request* r) {
         r->src_bytes = size_t(r->req.width) * r->req.height * r->req.channels;
         RCG_TRY(device_alloc(r->src_bytes, reinterpret_cast<void**>(&r->d_src)));
         RCG_HIP(hipMemcpyAsync(r->d_src, r->pixels.data(), r->src_bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -276,8 +303,8 @@ struct rcg_server {
         return RCG_OK;
     }
 
-    // One scoring pass over `items` on the worker's context.
-    int score(std::vector<Item>& items) {
+    // One scoring pass over `items` on a lane's context.
+    static int score(rcg_ctx* ctx, std::vector<Item>& items) {
         std::lock_guard<std::mutex> clk(ctx->mu);
         ctx->arena.reset();
         RCG_TRY(ctx->arena.ensure(score_items_bytes(ctx, items.data(), int(items.size()))));
@@ -285,7 +312,7 @@ struct rcg_server {
     }
 
     // The chosen quality's coefficients of the requests in `fin` that asked for them, in one encode launch.
-    int write_winners(const std::vector<Request*>& fin) {
+    static int write_winners(rcg_ctx* ctx, const std::vector<Request*>& fin) {
         std::vector<Item> items;
         std::vector<int> which, chosen;
         std::vector<WinnerDest> dest;
@@ -317,7 +344,9 @@ struct rcg_server {
 
     // A tick: score the wanted candidates of every request in `batch` (the failing ones are marked), apply the
     // scores, and finish the requests that are done. Returns the number of pairs scored.
-    size_t tick(const std::vector<Request*>& batch, const std::vector<Request*>& fresh, double& t0, double& t1) {
+    size_t tick(Lane& L, const std::vector<Request*>& batch, const std::vector<Request*>& fresh, double& t0,
+                double& t1) {
+        rcg_ctx* const ctx = L.ctx;
         std::vector<Item> items;
         std::vector<std::vector<int>> want(batch.size());
         std::vector<int> item_of(batch.size(), -1);
@@ -345,16 +374,21 @@ struct rcg_server {
         t0 = now_s();
         std::vector<int> status(items.size(), RCG_OK);
         if (!items.empty()) {
-            if (throw_tick > 0 && stats.ticks + 1 == uint64_t(throw_tick)) {
-                throw_tick = 0;
-                throw std::bad_alloc();
+            bool throw_now = false;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (throw_tick > 0 && stats.ticks + 1 == uint64_t(throw_tick)) {
+                    throw_tick = 0;
+                    throw_now = true;
+                }
             }
-            int rc = score(items);
+            if (throw_now) throw std::bad_alloc();
+            int rc = score(ctx, items);
             if (rc != RCG_OK && items.size() > 1) {
                 // Isolate the failure: each request alone, so that only those that fail alone fail.
                 for (size_t k = 0; k < items.size(); k++) {
                     std::vector<Item> one(1, items[k]);
-                    status[k] = score(one);
+                    status[k] = score(ctx, one);
                     items[k] = one[0];
                 }
             } else if (rc != RCG_OK) {
@@ -394,7 +428,7 @@ struct rcg_server {
         std::vector<Request*> ok;
         for (Request* r : fin)
             if (r->status == RCG_OK && r->req.coefs[0]) ok.push_back(r);
-        const int wrc = write_winners(ok);
+        const int wrc = write_winners(ctx, ok);
         for (Request* r : ok) {
             if (wrc == RCG_OK)
                 r->coefs_written = 1;
@@ -406,36 +440,45 @@ struct rcg_server {
         stats.busy_s += t1 - t0;
         batch_sum += items.size();
         for (Request* r : fin) {
-            active.erase(std::find(active.begin(), active.end(), r));
+            L.active.erase(std::find(L.active.begin(), L.active.end(), r));
             finish(r, t1);
         }
         if (stats.ticks) stats.mean_batch = double(batch_sum) / double(stats.ticks);
         return items.size();
     }
 
-    void run() {
+    void run(int lane) {
+        Lane& L = lanes[lane];
+        rcg_ctx* const ctx = L.ctx;
         RCG_HIP_VOID(hipSetDevice(ctx->device));
+        uint64_t seen = ~uint64_t(0);  // the epoch at which queued requests of this lane last could not be admitted
         for (;;) {
             std::vector<Request*> fresh, batch;
             size_t room = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+                cv_work.wait(lk, [&] { return stop || !L.active.empty() || (epoch != seen && has_pending(lane)); });
                 if (stop) return;
-                const size_t before = active.size();
-                room = admit();
-                for (size_t i = before; i < active.size(); i++) fresh.push_back(active[i]);
-                batch = active;
+                const size_t before = L.active.size();
+                room = admit(lane);
+                for (size_t i = before; i < L.active.size(); i++) fresh.push_back(L.active[i]);
+                batch = L.active;
+                if (batch.empty()) {
+                    // Its queued requests wait for budget that the other lane holds: until a request finishes or
+                    // arrives.
+                    seen = epoch;
+                    continue;
+                }
             }
             free_spares(room);
             double t0 = now_s(), t1 = t0;
             try {
                 for (Request* r : fresh)
                     if (r->d_src == nullptr) {
-                        const int rc = materialise(r);
+                        const int rc = materialise(ctx, r);
                         if (rc != RCG_OK) r->status = rc;
                     }
-                tick(batch, fresh, t0, t1);
+                tick(L, batch, fresh, t0, t1);
             } catch (...) {
                 // Nothing may leave the worker: the requests of this tick fail, and the loop goes on.
                 const int st = [] {
@@ -453,12 +496,14 @@ struct rcg_server {
                 for (Request* r : fresh) std::vector<uint8_t>().swap(r->pixels);
                 std::lock_guard<std::mutex> lk(mu);
                 for (Request* r : batch) {
-                    if (std::find(active.begin(), active.end(), r) == active.end()) continue;
+                    if (std::find(L.active.begin(), L.active.end(), r) == L.active.end()) continue;
                     if (r->status == RCG_OK) r->status = st;
-                    active.erase(std::find(active.begin(), active.end(), r));
+                    L.active.erase(std::find(L.active.begin(), L.active.end(), r));
                     finish(r, now_s());
                 }
             }
+            // Finished requests free budget the other lane may be waiting for.
+            cv_work.notify_all();
             cv_done.notify_all();
         }
     }
@@ -475,20 +520,37 @@ int rcg_server_create(int device, const rcg_server_opts* opts, rcg_server** out)
         if (s->opts.hbm_budget == 0) s->opts.hbm_budget = size_t(16) << 30;
         if (s->opts.speculate != 3) s->opts.speculate = 1;
         s->throw_tick = test_throw_tick();
-        RCG_TRY(rcg_create(device, &s->ctx));
-        struct Owner {  // destroys the context unless the worker took it over
-            rcg_ctx* c;
+        struct Owner {  // destroys the contexts unless the workers took them over
+            rcg_server* s;
             bool armed = true;
             ~Owner() {
-                if (armed) rcg_destroy(c);
+                if (!armed) return;
+                for (auto& L : s->lanes)
+                    if (L.ctx) rcg_destroy(L.ctx);
             }
-        } owner{s->ctx};
-        if (s->opts.tune) {
-            RCG_TRY(rcg_set_tune(s->ctx, s->opts.tune));
-            s->opts.tune = nullptr;
+        } owner{s.get()};
+        for (auto& L : s->lanes) {
+            RCG_TRY(rcg_create(device, &L.ctx));
+            if (s->opts.tune) RCG_TRY(rcg_set_tune(L.ctx, s->opts.tune));
         }
+        s->opts.tune = nullptr;
         rcg_server* raw = s.get();
-        s->worker = std::thread([raw] { raw->run(); });
+        int started = 0;
+        try {
+            for (; started < kLanes; started++) {
+                const int lane = started;
+                s->lanes[lane].worker = std::thread([raw, lane] { raw->run(lane); });
+            }
+        } catch (...) {
+            // A worker that did start is stopped before the contexts go.
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->stop = true;
+            }
+            s->cv_work.notify_all();
+            for (int i = 0; i < started; i++) s->lanes[i].worker.join();
+            throw;
+        }
         owner.armed = false;
         *out = s.release();
         return RCG_OK;
@@ -518,9 +580,10 @@ int rcg_server_submit(rcg_server* s, const rcg_request* req) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->pending.push_back(r.get());
+            s->epoch++;
         }
         r.release();
-        s->cv_work.notify_one();
+        s->cv_work.notify_all();
         return RCG_OK;
     });
 }
@@ -558,17 +621,21 @@ void rcg_server_destroy(rcg_server* s) {
             s->stop = true;
         }
         s->cv_work.notify_all();
-        if (s->worker.joinable()) s->worker.join();
-        (void)hipSetDevice(s->ctx->device);
-        (void)hipStreamSynchronize(s->ctx->stream);
-        for (Request* r : s->active) {
-            if (r->d_src) (void)hipFree(r->d_src);
-            if (r->d_ref) (void)hipFree(r->d_ref);
-            delete r;
+        for (auto& L : s->lanes)
+            if (L.worker.joinable()) L.worker.join();
+        for (auto& L : s->lanes) {
+            (void)hipSetDevice(L.ctx->device);
+            (void)hipStreamSynchronize(L.ctx->side);
+            (void)hipStreamSynchronize(L.ctx->stream);
+            for (Request* r : L.active) {
+                if (r->d_src) (void)hipFree(r->d_src);
+                if (r->d_ref) (void)hipFree(r->d_ref);
+                delete r;
+            }
         }
         for (auto& b : s->spare) (void)hipFree(b.second);
         for (Request* r : s->pending) delete r;
-        rcg_destroy(s->ctx);
+        for (auto& L : s->lanes) rcg_destroy(L.ctx);
         delete s;
     });
 }
