@@ -170,3 +170,31 @@ def test_kept_blocks_stay_within_the_budget(rc):
     srv.close()
     assert all(c.status == 0 for c in got.values())
     assert st.peak_footprint <= 96 << 20 or st.mean_batch == 1.0
+
+
+def test_small_requests_do_not_wait_for_a_large_one(rc, ctx):
+    # Requests below 2 MP tick on a lane of their own: submitted just after a 16 MP request, small ones complete
+    # before it does, and every result still equals the request's alone result.
+    big = rc.synth_rgb8(990, 4000, 4000)
+    small = [(i, rc.synth_rgb8(991 + i, 320, 240), t) for i, t in enumerate([60.0, 70.0, 80.0, 70.0])]
+    srv = rc.Server(0, speculate=3)
+    srv.submit(100, big, 70.0)
+    for i, img, t in small:
+        srv.submit(i, img, t)
+    order = []
+    got = {}
+    t0 = time.time()
+    while len(got) < 5 and time.time() - t0 < 120:
+        for c in srv.poll(16, 200):
+            got[c.id] = c
+            order.append(c.id)
+    st = srv.stats()
+    srv.close()
+    assert len(got) == 5
+    assert order.index(100) > 0, order  # at least one small request finished first
+    for i, img, t in small:
+        f = ctx.find_quality(img, t, 100, rc.S420)
+        assert got[i].status == 0 and (got[i].quality, got[i].score) == (f.quality, f.score)
+    f = ctx.find_quality(big, 70.0, 100, rc.S420)
+    assert got[100].status == 0 and (got[100].quality, got[100].score) == (f.quality, f.score)
+    assert st.requests_done == 5
