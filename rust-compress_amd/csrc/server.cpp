@@ -261,20 +261,25 @@ struct rcg_server {
 
     // Admit queued requests of the lane's class in arrival order while they fit (mu held); the footprints of both
     // lanes share the budget, and a request larger than the whole budget is admitted only when no request is admitted
-    // on either lane. Kept blocks give way to admitted requests: the return value is how many bytes of kept blocks
-    // may remain (free_spares() before the next tick).
+    // on either lane. Admission stays first come, first served across the lanes: an older request of the other lane
+    // keeps the budget it needs (it is admitted by its own lane's worker), and one that does not fit stops the scan.
+    // Kept blocks give way to admitted requests: the return value is how many bytes of kept blocks may remain
+    // (free_spares() before the next tick).
     size_t admit(int lane) {
         Lane& L = lanes[lane];
+        const size_t budget = opts.hbm_budget;
+        size_t ahead = 0;  // budget the older requests of the other lane will take
         for (auto it = pending.begin(); it != pending.end();) {
             Request* r = *it;
+            const size_t need = std::min(r->footprint, budget);
+            const bool fits = (nactive == 0 && ahead == 0) || used + ahead + need <= budget;
+            if (!fits) break;
             if (lane_of(r->req) != lane) {
+                ahead += need;
                 ++it;
                 continue;
             }
-            const size_t budget = opts.hbm_budget;
-            const size_t need = std::min(r->footprint, budget);
             if (opts.max_active > 0 && int(nactive) >= opts.max_active) break;
-            if (nactive > 0 && used + need > budget) break;
             it = pending.erase(it);
             r->admit_s = now_s();
             used += need;
