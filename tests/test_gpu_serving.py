@@ -198,3 +198,23 @@ def test_small_requests_do_not_wait_for_a_large_one(rc, ctx):
     f = ctx.find_quality(big, 70.0, 100, rc.S420)
     assert got[100].status == 0 and (got[100].quality, got[100].score) == (f.quality, f.score)
     assert st.requests_done == 5
+
+
+def test_admission_stays_in_arrival_order_across_lanes(rc):
+    # A budget smaller than the 2.2 MP request's footprint: it runs alone, once nothing else is admitted. Small
+    # requests that arrive after it must not take the budget first (their lane is another one), so it completes
+    # before all of them but the one admitted ahead of it.
+    srv = rc.Server(0, hbm_budget=32 << 20)
+    small = [rc.synth_rgb8(1200 + i, 160, 120) for i in range(5)]
+    large = rc.synth_rgb8(1210, 1600, 1400)
+    srv.submit(0, small[0], 70.0)
+    srv.submit(100, large, 70.0)
+    for i in range(1, 5):
+        srv.submit(i, small[i], 70.0)
+    order = []
+    t0 = time.time()
+    while len(order) < 6 and time.time() - t0 < 120:
+        order += [c.id for c in srv.poll(16, 200) if c.status == 0]
+    srv.close()
+    assert sorted(order) == [0, 1, 2, 3, 4, 100], order
+    assert order.index(100) == 1, order
