@@ -239,6 +239,34 @@ def test_score_batch_mixed_sizes_matches_oracle(ctx, rc):
         assert r.chosen_quality == qs[order[best]]
 
 
+def test_score_batch_mixed_layouts_and_gray_match_oracle(ctx, rc):
+    # Per-image candidate lists over every subsampling mode (Auto included), RGB8 and Gray8 sources of odd sizes in
+    # one batch: every score within the bound of the f64 oracle's, the choice the reference's bisection over the
+    # oracle's scores of the list.
+    gray = np.ascontiguousarray(rc.synth_rgb8(42, 201, 97)[:, :, 1])
+    imgs = [rc.synth_rgb8(41, 333, 250), gray, rc.synth_rgb8(43, 97, 64)]
+    cands = [[(40, rc.S444), (60, rc.S422), (80, rc.S420), (92, rc.AUTO)],
+             [(30, rc.S420), (55, rc.S444), (75, rc.AUTO), (95, rc.S422)],
+             [(50, rc.AUTO), (70, rc.S422), (85, rc.S444), (90, rc.AUTO)]]
+    res = ctx.score_batch(imgs, cands, 70.0)
+    for img, cl, r in zip(imgs, cands, res):
+        assert r.status == 0
+        osc = [O.ssimulacra2(img, O.reconstruct(O.build_coefs(img, q, sub))) for q, sub in cl]
+        for k, (q, sub) in enumerate(cl):
+            assert abs(r.scores[k] - osc[k]) <= (TOL64 if r.settled[k] else TOL32), (img.shape, q, sub)
+        lo, hi = 0, len(cl) - 1
+        best = hi
+        if osc[hi] >= 70.0:
+            while lo < hi:
+                mid = lo + (hi - lo) // 2
+                if osc[mid] >= 70.0:
+                    hi = mid
+                    best = mid
+                else:
+                    lo = mid + 1
+        assert r.chosen_quality == cl[best][0], (img.shape, [round(x, 3) for x in osc])
+
+
 def test_batch_scores_do_not_depend_on_the_rest_of_the_batch(ctx, rc):
     # A launch over many images tiles the reference caches taller than one image alone does; the cache, and so every
     # score, must not depend on that tiling: each image's scores in a 24-image batch equal its scores alone, bit for bit.

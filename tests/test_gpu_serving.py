@@ -218,3 +218,19 @@ def test_admission_stays_in_arrival_order_across_lanes(rc):
     srv.close()
     assert sorted(order) == [0, 1, 2, 3, 4, 100], order
     assert order.index(100) == 1, order
+
+
+def test_gray_and_every_subsampling_equal_their_alone_results(rc, ctx):
+    # Gray8 requests and RGB8 ones under each subsampling mode, in one server: each completion equals find_quality's
+    # result for the same request alone.
+    gray = np.ascontiguousarray(rc.synth_rgb8(1300, 300, 211)[:, :, 0])
+    reqs = [(0, gray, 70.0, rc.S420), (1, rc.synth_rgb8(1301, 257, 190), 60.0, rc.S444),
+            (2, rc.synth_rgb8(1302, 190, 257), 80.0, rc.S422), (3, rc.synth_rgb8(1303, 128, 96), 70.0, rc.AUTO)]
+    srv = rc.Server(0, speculate=3)
+    for i, img, t, sub in reqs:
+        srv.submit(i, img, t, 100, sub)
+    got = _drain(srv, len(reqs))
+    srv.close()
+    for i, img, t, sub in reqs:
+        f = ctx.find_quality(img, t, 100, sub)
+        assert got[i].status == 0 and (got[i].quality, got[i].score) == (f.quality, f.score), (i, sub)
