@@ -267,6 +267,20 @@ def test_score_batch_mixed_layouts_and_gray_match_oracle(ctx, rc):
         assert r.chosen_quality == cl[best][0], (img.shape, [round(x, 3) for x in osc])
 
 
+def test_score_batch_extreme_aspect_ratios_match_oracle(ctx, rc):
+    # The longest side JPEG allows (encode.rs:417) and thin strips in one batch, through the column tiling's main and
+    # tail tiles: scores within the bound of the oracle's, one or two scales each.
+    sizes = [(65535, 9), (4097, 17), (9, 3001), (1050, 8), (8, 8)]
+    imgs = [rc.synth_rgb8(80 + i, w, h) for i, (w, h) in enumerate(sizes)]
+    cands = [(45, rc.S420), (75, rc.S444), (92, rc.AUTO)]
+    res = ctx.score_batch(imgs, cands, 70.0)
+    for img, r in zip(imgs, res):
+        assert r.status == 0, img.shape
+        for k, (q, sub) in enumerate(cands):
+            want = O.ssimulacra2(img, O.reconstruct(O.build_coefs(img, q, sub)))
+            assert abs(r.scores[k] - want) <= (TOL64 if r.settled[k] else TOL32), (img.shape, q, r.scores[k], want)
+
+
 def test_batch_scores_do_not_depend_on_the_rest_of_the_batch(ctx, rc):
     # A launch over many images tiles the reference caches taller than one image alone does; the cache, and so every
     # score, must not depend on that tiling: each image's scores in a 24-image batch equal its scores alone, bit for bit.
