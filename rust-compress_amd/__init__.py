@@ -554,6 +554,13 @@ class DeviceBuffer:
         _check(lib().rcg_device_alloc(ctx.handle, nbytes, ctypes.byref(p)))
         self.ptr = p.value
 
+    def write(self, array):
+        """Copy a numpy array into this buffer, from its start (rcg_memcpy_h2d); the address stays the same."""
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        _check(lib().rcg_memcpy_h2d(self._ctx.handle, self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+        return self
+
     def download(self, out):
         assert out.nbytes <= self.nbytes and out.flags["C_CONTIGUOUS"]
         _check(lib().rcg_memcpy_d2h(self._ctx.handle, out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes))

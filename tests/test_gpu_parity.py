@@ -391,6 +391,35 @@ def test_graph_plan_replays_match_eager(ctx, rc):
         d.free()
 
 
+def test_graph_plan_sees_new_pixels_at_the_same_addresses(ctx, rc):
+    # A plan captures device addresses, not pixel values: after new pixels are written into the same buffers, a
+    # replay must give exactly what a fresh eager batch gives on the new pixels, and the oracle's scores.
+    shapes = [(300, 200), (97, 131), (640, 480)]
+    first = [rc.synth_rgb8(720 + i, w, h) for i, (w, h) in enumerate(shapes)]
+    dev = [ctx.upload(im) for im in first]
+    ptrs = [(d.ptr, im.shape[1], im.shape[0], 3) for d, im in zip(dev, first)]
+    plan = ctx.batch_plan(ptrs, CANDS, 70.0)
+    before = plan.run()
+    for seed in (730, 740):
+        new = [rc.synth_rgb8(seed + i, w, h) for i, (w, h) in enumerate(shapes)]
+        for d, im in zip(dev, new):
+            d.write(im)
+        ctx.synchronize()
+        got = plan.run()
+        eager = ctx.score_batch(None, CANDS, 70.0, device_ptrs=ptrs)
+        for g, e, b in zip(got, eager, before):
+            assert list(g.scores[:len(CANDS)]) == list(e.scores[:len(CANDS)]), seed
+            assert g.chosen_quality == e.chosen_quality and g.sweep_quality == e.sweep_quality
+            assert list(g.scores[:len(CANDS)]) != list(b.scores[:len(CANDS)])
+        for g, im in zip(got, new):
+            for k, (q, sub) in enumerate(CANDS):
+                o = O.ssimulacra2(im, O.reconstruct(O.build_coefs(im, q, sub)))
+                assert abs(g.scores[k] - o) <= (TOL64 if g.settled[k] else TOL32), (im.shape, q, sub)
+    plan.close()
+    for d in dev:
+        d.free()
+
+
 def test_device_entry_points_match_host_ones(ctx, rc):
     img = rc.synth_rgb8(31, 333, 250)
     d = ctx.upload(img)
