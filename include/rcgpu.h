@@ -288,7 +288,7 @@ void rcg_server_destroy(rcg_server* server);
 
 /* Per-kernel timing with HIP events recorded around every launch, on the stream the launch runs on. enable = 1
  * starts recording (and clears the totals), 0 stops. rcg_profile_get returns the summed milliseconds and the launch
- * count of one kernel ("k_encode", "k_idct", "k_upsample_color", "k_ssim_ref", "k_ssim_scale" (FP32),
+ * count of one kernel ("k_encode", "k_reconstruct", "k_idct", "k_upsample_color", "k_ssim_ref", "k_ssim_scale" (FP32),
  * "k_ssim_scale_f64", "k_ssim_finalize"). */
 int rcg_profile(rcg_ctx* ctx, int enable);
 int rcg_profile_get(rcg_ctx* ctx, const char* kernel, double* total_ms, int* launches);

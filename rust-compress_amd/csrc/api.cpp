@@ -819,7 +819,11 @@ void rcg::mult_table(const uint8_t q[64], float mult[64]) {
 // IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device. The sample
 // planes go to `planes` (one per component, blocks_w*8 x blocks_h*8 bytes each).
 int rcg::reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4],
-                          uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct) {
+                          uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct,
+                          bool fused_ok) {
+    int fx, fy;
+    if (fused_ok && fused_reconstruct_layout(ci, hmax, vmax, &fx, &fy))
+        return launch_reconstruct_fused(ctx, ci, d_coefs, d_out, hmax, vmax, rgb_direct);
     IdctPlane planes[4] = {};
     PlaneView views[4];
     for (uint32_t c = 0; c < ci->ncomp; c++) {

@@ -458,6 +458,298 @@ __global__ __launch_bounds__(256) void k_upsample_color(UpArgs a) {
 }
 
 // ----------------------------------------------------------------------------------------------------------------
+// Fused reconstruct of the usual layouts: decode.rs:1312-1343 (IDCT), 1769-1829 (upsampling) and 1427-1445 (colour)
+// in one pass, with no sample planes in HBM. Component 0 is at full resolution; components 1 and 2 share the
+// upsampling factors (FX, FY), each 1 or 2. A 256-thread workgroup owns a tile of 32 x 8 luma blocks (256 x 64
+// pixels):
+//   phase 1 (subsampled chroma only): the IDCT of the tile's chroma blocks into two LDS planes, with a ring of one
+//           block in each direction whose factor is 2 (the triangle filter reads one chroma sample beyond the tile);
+//           the sample just outside the component's real extent (column cw, row ch, column/row -1 at the image's
+//           edges) is then overwritten with the edge sample, which is what the reference's clamped index reads, so
+//           phase 2 reads its window without any clamping;
+//   phase 2: one luma block per thread: its IDCT in registers, then for each of its rows the chroma of its 8 pixels
+//           from the LDS window, the fixed-point colour conversion, and 24 (or 8) bytes stored straight into the
+//           output image. Without subsampling a thread runs the IDCT of all three components of its block instead.
+constexpr int kRcBX = 32, kRcBY = 8;  // luma blocks per tile
+constexpr int kRcThreads = kRcBX * kRcBY;
+
+// IDCT of one block of zigzag coefficients (dequantisation with mult, natural order) into its 64 level-shifted,
+// clamped samples, packed four per dword: row y in out[2y] (x 0-3) and out[2y+1] (x 4-7). The final 1/8 is folded
+// into the rounding as in k_encode (round-half-even of x/8, exact for |x| < 2^25).
+__device__ __forceinline__ void load_block(const int16_t* co, uint32_t pk[32]) {
+    const uint4* src = reinterpret_cast<const uint4*>(co);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint4 v = src[k];
+        pk[4 * k] = v.x;
+        pk[4 * k + 1] = v.y;
+        pk[4 * k + 2] = v.z;
+        pk[4 * k + 3] = v.w;
+    }
+}
+__device__ __forceinline__ void idct_packed_u8(const uint32_t pk[32], const float* mult, uint32_t out[16]) {
+    float blk[64];
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+        const int16_t q = int16_t((pk[k / 2] >> (16 * (k & 1))) & 0xFFFF);
+        const int nn = zigzag(k);
+        blk[nn] = __fmul_rn(float(q), mult[nn]);
+    }
+    idct_block<false>(blk);
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            int32_t v = __float_as_int(__fadd_rn(blk[y * 8 + x], 100663296.0f)) - (0x4CC00000 - 128);
+            v = v < 0 ? 0 : (v > 255 ? 255 : v);
+            if (x < 4) lo |= uint32_t(v) << (8 * x);
+            else hi |= uint32_t(v) << (8 * (x - 4));
+        }
+        out[2 * y] = lo;
+        out[2 * y + 1] = hi;
+    }
+    // Make the packed samples opaque so the compiler forms all of them here, rather than sinking parts of the IDCT
+    // into the per-row store branches of the caller, where its 64 values would stay live across them.
+#pragma unroll
+    for (int k = 0; k < 16; k++) asm volatile("" : "+v"(out[k]));
+}
+__device__ __forceinline__ void idct_block_u8(const int16_t* co, const float* mult, uint32_t out[16]) {
+    uint32_t pk[32];
+    load_block(co, pk);
+    idct_packed_u8(pk, mult, out);
+}
+
+__device__ __forceinline__ int32_t byte_of(const uint32_t* d, int i) { return int32_t((d[i >> 2] >> (8 * (i & 3))) & 0xFF); }
+
+// Store the first n bytes of d (little-endian, n <= 4 * ND) at p, which may have any alignment: aligned dword stores
+// for whole rows that start on a dword, else the dwords realigned with v_alignbyte and byte stores at either end.
+template <int ND>
+__device__ __forceinline__ void store_bytes(uint8_t* p, const uint32_t (&d)[ND], uint32_t n) {
+    const uint32_t s = uint32_t(reinterpret_cast<uintptr_t>(p) & 3);
+    if (n == 4 * ND) {
+        if (s == 0) {
+            uint32_t* q = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+            for (int k = 0; k < ND; k++) q[k] = d[k];
+            return;
+        }
+        // Bytes 4 - s .. 4 * ND - 1 - s fill the ND - 1 aligned dwords from p + 4 - s; the 4 - s bytes before them
+        // come from d[0], the s after them from d[ND - 1].
+        uint32_t* q = reinterpret_cast<uint32_t*>(p + 4 - s);
+#pragma unroll
+        for (int k = 0; k + 1 < ND; k++) q[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], 4 - s);
+        for (uint32_t i = 0; i < 4 - s; i++) p[i] = uint8_t(d[0] >> (8 * i));
+        for (uint32_t i = 4 * ND - s; i < 4 * ND; i++) p[i] = uint8_t(d[ND - 1] >> (8 * (i & 3)));
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 4 * ND; i++)
+        if (uint32_t(i) < n) p[i] = uint8_t(byte_of(d, i));
+}
+
+struct FusedRecon {
+    const int16_t* coefs[3];
+    uint32_t bw[3];        // block grid widths
+    float mult[3][64];     // dequantisation per component, natural order
+    uint32_t w, h;         // image size
+    uint32_t cw, ch;       // real size of components 1 and 2
+    uint32_t nbx_c, nby_c; // their blocks that hold real samples: ceil(cw/8), ceil(ch/8)
+    uint8_t* out;          // Gray8 (one component) or RGB8, w x h, rows of w*channels bytes
+};
+
+// One output row of 8 pixels (24 bytes) from the luma row and the 8 chroma samples of each plane.
+template <bool kRgbDirect>
+__device__ __forceinline__ void color_row(const int32_t* yv, const int32_t* cb, const int32_t* cr, uint32_t (&d)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) d[k] = 0;
+#pragma unroll
+    for (int x = 0; x < 8; x++) {
+        int32_t r, g, b;
+        if constexpr (kRgbDirect) {
+            r = yv[x];
+            g = cb[x];
+            b = cr[x];
+        } else {
+            ycc_to_rgb(yv[x], cb[x], cr[x], r, g, b);
+        }
+        const int i = 3 * x;
+        d[i >> 2] |= uint32_t(r) << (8 * (i & 3));
+        d[(i + 1) >> 2] |= uint32_t(g) << (8 * ((i + 1) & 3));
+        d[(i + 2) >> 2] |= uint32_t(b) << (8 * ((i + 2) & 3));
+    }
+}
+
+// The 8 chroma samples of output row yy of a block from the thread's window of one plane (rows and columns from one
+// below the block's first chroma sample where the factor is 2). win[r] holds window row r as bytes: 6 bytes from byte
+// 3 of three dwords (FX = 2, columns i0-1 .. i0+4) or 8 bytes of two dwords (FX = 1).
+template <int FX, int FY, int WD>
+__device__ __forceinline__ void chroma_row(const uint32_t (*win)[WD], int yy, int32_t out[8]) {
+    constexpr int c0 = FX == 2 ? 3 : 0;  // byte of the window's first column
+    if constexpr (FY == 2) {
+        const int jr = (yy >> 1) + 1, fr = (yy & 1) ? jr + 1 : jr - 1;
+        if constexpr (FX == 2) {
+            int32_t vert[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) vert[k] = 3 * byte_of(win[jr], c0 + k) + byte_of(win[fr], c0 + k);
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const int n = (x >> 1) + 1, f = (x & 1) ? n + 1 : n - 1;
+                out[x] = (3 * vert[n] + vert[f] + ((x & 1) ? 7 : 8)) >> 4;
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                out[x] = (3 * byte_of(win[jr], x) + byte_of(win[fr], x) + ((yy & 1) ? 2 : 1)) >> 2;
+        }
+    } else {
+        if constexpr (FX == 2) {
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const int n = (x >> 1) + 1, f = (x & 1) ? n + 1 : n - 1;
+                out[x] = (3 * byte_of(win[yy], c0 + n) + byte_of(win[yy], c0 + f) + ((x & 1) ? 2 : 1)) >> 2;
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < 8; x++) out[x] = byte_of(win[yy], x);
+        }
+    }
+}
+
+// NC = 1: Gray8 from one component. NC = 3, FX = FY = 1: each thread reconstructs all three components of its block.
+// NC = 3 with a factor of 2: the chroma goes through LDS (phases above). RD: RGB-coded components (no colour
+// conversion).
+template <int NC, int FX, int FY, bool RD>
+__global__ __launch_bounds__(kRcThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_reconstruct(FusedRecon a) {
+    constexpr bool kLds = NC == 3 && (FX == 2 || FY == 2);
+    constexpr int HX = FX - 1, HY = FY - 1;                     // ring blocks per side
+    constexpr int NBX = kRcBX / FX + 2 * HX, NBY = kRcBY / FY + 2 * HY;  // chroma blocks of the LDS window
+    constexpr int PW = 8 * NBX, PH = 8 * NBY;                   // LDS plane width and height in samples
+    constexpr int NB = NBX * NBY;                               // chroma blocks per plane
+    constexpr int NBP = (NB + 63) & ~63;                        // per plane, padded to whole waves
+    __shared__ __align__(16) uint8_t s_c[kLds ? 2 * PW * PH : 4];
+
+    const int tid = threadIdx.x;
+    const uint32_t cbx0 = blockIdx.x * (kRcBX / FX), cby0 = blockIdx.y * (kRcBY / FY);  // first chroma block
+    const int lbx = tid % kRcBX, lby = tid / kRcBX;
+    const uint32_t x0 = (blockIdx.x * kRcBX + lbx) * 8, y0 = (blockIdx.y * kRcBY + lby) * 8;
+    const bool has_px = x0 < a.w && y0 < a.h;
+    const uint32_t gbx = x0 / 8, gby = y0 / 8;
+    // The thread's luma coefficients are requested first, so that they arrive while the chroma phase runs.
+    uint32_t ypk[32];
+    if (kLds && has_px) load_block(a.coefs[0] + (uint64_t(gby) * a.bw[0] + gbx) * 64, ypk);
+    if constexpr (kLds) {
+        // Phase 1: one chroma block per thread and pass; a wave never mixes the two planes.
+        for (int k = tid; k < 2 * NBP; k += kRcThreads) {
+            const int comp = __builtin_amdgcn_readfirstlane(k / NBP);
+            const int lb = k - comp * NBP;
+            if (lb >= NB) continue;
+            const int lx = lb % NBX, ly = lb / NBX;
+            const int gbx = int(cbx0) - HX + lx, gby = int(cby0) - HY + ly;
+            if (gbx < 0 || gby < 0 || gbx >= int(a.nbx_c) || gby >= int(a.nby_c)) continue;
+            uint32_t px[16];
+            idct_block_u8(a.coefs[1 + comp] + (uint64_t(gby) * a.bw[1 + comp] + uint32_t(gbx)) * 64, a.mult[1 + comp],
+                          px);
+            uint8_t* dst = s_c + comp * PW * PH + (8 * ly) * PW + 8 * lx;
+#pragma unroll
+            for (int r = 0; r < 8; r++) *reinterpret_cast<uint2*>(dst + r * PW) = make_uint2(px[2 * r], px[2 * r + 1]);
+        }
+        __syncthreads();
+        // Clamped columns, then clamped rows (so the corners take both).
+        if constexpr (HX) {
+            const int ox = (int(cbx0) - HX) * 8;  // global chroma column of LDS column 0
+            const bool left = cbx0 == 0;
+            const int rc = int(a.cw) - ox;        // LDS column of global column cw
+            const bool right = rc < PW;
+            if (left || right) {
+                for (int k = tid; k < 2 * PH; k += kRcThreads) {
+                    uint8_t* row = s_c + (k / PH) * PW * PH + (k % PH) * PW;
+                    if (left) row[8 * HX - 1] = row[8 * HX];
+                    if (right) row[rc] = row[rc - 1];
+                }
+            }
+            __syncthreads();
+        }
+        if constexpr (HY) {
+            const int oy = (int(cby0) - HY) * 8;
+            const bool top = cby0 == 0;
+            const int rr = int(a.ch) - oy;
+            const bool bottom = rr < PH;
+            if (top || bottom) {
+                for (int k = tid; k < 2 * PW; k += kRcThreads) {
+                    uint8_t* col = s_c + (k / PW) * PW * PH + (k % PW);
+                    if (top) col[(8 * HY - 1) * PW] = col[(8 * HY) * PW];
+                    if (bottom) col[rr * PW] = col[(rr - 1) * PW];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // Phase 2: one luma block per thread.
+    if (!has_px) return;
+    uint32_t ypx[16];
+    if constexpr (!kLds) load_block(a.coefs[0] + (uint64_t(gby) * a.bw[0] + gbx) * 64, ypk);
+    idct_packed_u8(ypk, a.mult[0], ypx);
+    // One block's IDCT at a time: its 64 values and its coefficients are most of the thread's registers.
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t ncols = min(8u, a.w - x0), nrows = min(8u, a.h - y0);
+    if constexpr (NC == 1) {
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            if (uint32_t(y) >= nrows) break;
+            const uint32_t d[2] = {ypx[2 * y], ypx[2 * y + 1]};
+            store_bytes(a.out + uint64_t(y0 + y) * a.w + x0, d, ncols);
+        }
+    } else {
+        uint32_t cpx[2][16];  // FX = FY = 1: the block's Cb and Cr samples
+        constexpr int WR = FY == 2 ? 6 : 8, WD = FX == 2 ? 3 : 2;
+        uint32_t win[2][WR][WD];
+        if constexpr (!kLds) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                idct_block_u8(a.coefs[1 + c] + (uint64_t(gby) * a.bw[1 + c] + gbx) * 64, a.mult[1 + c], cpx[c]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+            // Window origin in the LDS planes: one row/column above/left of the block's first chroma sample where the
+            // factor is 2 (the ring block puts that at 8 - 1), the dword holding it for FX = 2.
+            const int wx = FX == 2 ? 4 * lbx + 4 : 8 * lbx;
+            const int wy = FY == 2 ? 4 * lby + 7 : 8 * lby;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int r = 0; r < WR; r++) {
+                    const uint32_t* src = reinterpret_cast<const uint32_t*>(s_c + c * PW * PH + (wy + r) * PW + wx);
+#pragma unroll
+                    for (int k = 0; k < WD; k++) win[c][r][k] = src[k];
+                }
+        }
+        uint8_t* orow = a.out + (uint64_t(y0) * a.w + x0) * 3;
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            if (uint32_t(y) >= nrows) break;
+            int32_t yv[8], cb[8], cr[8];
+#pragma unroll
+            for (int x = 0; x < 8; x++) yv[x] = byte_of(&ypx[2 * y], x);
+            if constexpr (!kLds) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) {
+                    cb[x] = byte_of(&cpx[0][2 * y], x);
+                    cr[x] = byte_of(&cpx[1][2 * y], x);
+                }
+            } else {
+                chroma_row<FX, FY, WD>(win[0], y, cb);
+                chroma_row<FX, FY, WD>(win[1], y, cr);
+            }
+            uint32_t d[6];
+            color_row<RD>(yv, cb, cr, d);
+            store_bytes(orow + uint64_t(y) * a.w * 3, d, 3 * ncols);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------------------------
 // Launchers.
 
 int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t total_tiles) {
@@ -490,6 +782,66 @@ int launch_idct(rcg_ctx* ctx, const IdctPlane* planes, int n) {
     if (maxb == 0) return RCG_OK;
     ProfScope ps(ctx, "k_idct");
     hipLaunchKernelGGL(k_idct, dim3((maxb + 255) / 256, n), dim3(256), 0, ctx->stream, a);
+    RCG_HIP(hipGetLastError());
+    return RCG_OK;
+}
+
+template <int FX, int FY>
+static void launch_rc(bool rgb_direct, dim3 grid, hipStream_t stream, const FusedRecon& a) {
+    if (rgb_direct)
+        hipLaunchKernelGGL((k_reconstruct<3, FX, FY, true>), grid, dim3(kRcThreads), 0, stream, a);
+    else
+        hipLaunchKernelGGL((k_reconstruct<3, FX, FY, false>), grid, dim3(kRcThreads), 0, stream, a);
+}
+
+bool fused_reconstruct_layout(const rcg_coef_image* ci, uint32_t hmax, uint32_t vmax, int* fx, int* fy) {
+    if (ci->ncomp != 1 && ci->ncomp != 3) return false;
+    if (ci->comps[0].h != hmax || ci->comps[0].v != vmax) return false;
+    *fx = *fy = 1;
+    if (ci->ncomp == 1) return true;
+    const rcg_coef_component &c1 = ci->comps[1], &c2 = ci->comps[2];
+    if (c1.h != c2.h || c1.v != c2.v) return false;
+    *fx = int(hmax / c1.h);
+    *fy = int(vmax / c1.v);
+    return (*fx == 1 || *fx == 2) && (*fy == 1 || *fy == 2);
+}
+
+int launch_reconstruct_fused(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4], uint8_t* d_out,
+                             uint32_t hmax, uint32_t vmax, int rgb_direct) {
+    int fx, fy;
+    if (!fused_reconstruct_layout(ci, hmax, vmax, &fx, &fy)) {
+        set_error("internal: layout without a fused reconstruct");
+        return RCG_DEVICE;
+    }
+    FusedRecon a;
+    std::memset(&a, 0, sizeof(a));
+    for (uint32_t c = 0; c < ci->ncomp; c++) {
+        a.coefs[c] = d_coefs[c];
+        a.bw[c] = ci->comps[c].blocks_w;
+        mult_table(ci->qtables[ci->comps[c].tq], a.mult[c]);
+    }
+    a.w = ci->width;
+    a.h = ci->height;
+    if (ci->ncomp == 3) {
+        a.cw = (ci->width * ci->comps[1].h + hmax - 1) / hmax;
+        a.ch = (ci->height * ci->comps[1].v + vmax - 1) / vmax;
+        a.nbx_c = (a.cw + 7) / 8;
+        a.nby_c = (a.ch + 7) / 8;
+    }
+    a.out = d_out;
+    const dim3 grid((ci->width + 8 * kRcBX - 1) / (8 * kRcBX), (ci->height + 8 * kRcBY - 1) / (8 * kRcBY));
+    ProfScope ps(ctx, "k_reconstruct");
+    const bool rd = rgb_direct != 0;
+    if (ci->ncomp == 1)
+        hipLaunchKernelGGL((k_reconstruct<1, 1, 1, false>), grid, dim3(kRcThreads), 0, ctx->stream, a);
+    else if (fx == 1 && fy == 1)
+        launch_rc<1, 1>(rd, grid, ctx->stream, a);
+    else if (fx == 2 && fy == 1)
+        launch_rc<2, 1>(rd, grid, ctx->stream, a);
+    else if (fx == 1 && fy == 2)
+        launch_rc<1, 2>(rd, grid, ctx->stream, a);
+    else
+        launch_rc<2, 2>(rd, grid, ctx->stream, a);
     RCG_HIP(hipGetLastError());
     return RCG_OK;
 }

@@ -83,7 +83,8 @@ extern "C" int rcg_reconstruct_dev_noalloc(rcg_ctx* ctx, const rcg_coef_image* d
             explicit Suspend(rcg_ctx* x) : c(x) { c->prof_suspended = true; }
             ~Suspend() { c->prof_suspended = false; }
         } suspend(ctx);
-        RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
+        // The two-kernel form (IDCT into the scratch planes, then the colour stage), for every layout.
+        RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct, false));
         RCG_HIP(hipStreamSynchronize(ctx->stream));
         return RCG_OK;
     });

@@ -234,13 +234,20 @@ int requant_plan(const rcg_coef_image* ci, int quality, RequantPlan& P);
 // check_coef_image validates it (RCG_INVALID / RCG_UNSUPPORTED as the reference's errors) and derives the sampling
 // maxima and the RGB-coded flag; out_channels is 1 for gray, else 3; sample_plane_bytes is what the IDCT's planes
 // take (256-byte aligned per component); reconstruct_on_device enqueues the IDCT into planes_out and the colour stage
-// into d_out on the context's stream.
+// into d_out on the context's stream, or, when fused_ok and the layout allows it, the fused kernel alone (planes_out
+// unused).
 int check_coef_image(const rcg_coef_image* ci, uint32_t& hmax, uint32_t& vmax, int& rgb_direct);
 uint32_t out_channels(const rcg_coef_image* ci);
 size_t sample_plane_bytes(const rcg_coef_image* ci);
 void mult_table(const uint8_t q[64], float mult[64]);
 int reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4],
-                          uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct);
+                          uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct,
+                          bool fused_ok = true);
+// The fused reconstruct (one kernel, no sample planes) takes component 0 at full resolution and, for colour,
+// components 1 and 2 sharing upsampling factors of 1 or 2 each: Gray, 4:4:4, 4:2:2, 4:4:0 and 4:2:0.
+bool fused_reconstruct_layout(const rcg_coef_image* ci, uint32_t hmax, uint32_t vmax, int* fx, int* fy);
+int launch_reconstruct_fused(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4], uint8_t* d_out,
+                             uint32_t hmax, uint32_t vmax, int rgb_direct);
 // Set-up of the allocation-free entry points on `device` (current): finds the device address of the static scratch
 // array once per device. Called by rcg_create.
 int noalloc_setup(int device);

@@ -92,6 +92,41 @@ def test_reconstruct_reference_files(ctx, rc, name):
     np.testing.assert_array_equal(ctx.reconstruct(ci), O.reconstruct(oc))
 
 
+def _random_coef_image(rng, w, h, factors, ids, adobe, lib):
+    """A coefficient image with random coefficients (DC and AC beyond what 8-bit samples need, so clamping is
+    exercised) on build_coefs' MCU-padded block grids (jpeg/src/encode.rs:440-443), for any sampling factors."""
+    hmax = max(f[0] for f in factors)
+    vmax = max(f[1] for f in factors)
+    mx, my = -(-w // (8 * hmax)), -(-h // (8 * vmax))
+    comps, qt = [], [rng.integers(1, 64, 64).astype(np.uint8), rng.integers(1, 64, 64).astype(np.uint8)]
+    for i, (ch, cv) in enumerate(factors):
+        bw, bh = mx * ch, my * cv
+        co = rng.integers(-120, 121, bw * bh * 64).astype(np.int16)
+        co[::64] = rng.integers(-1100, 1100, bw * bh)
+        comps.append(lib.CoefComponent(ids[i], ch, cv, 0 if i == 0 else 1, bw, co))
+    return lib.CoefImage(w, h, comps, [qt[0], qt[1], None, None], adobe)
+
+
+# The fused reconstruct kernel (Gray, 4:4:4, 4:2:2, 4:4:0, 4:2:0 with the luma at full resolution, YCbCr- and
+# RGB-coded) and the two-kernel form it falls back to (mixed chroma factors, subsampled luma), at sizes that end
+# inside a 256x64 tile, span several and straddle one chroma ring block, against the oracle (decode.rs:1390-1451).
+@pytest.mark.parametrize("w,h", [(1, 1), (13, 7), (257, 65), (300, 131), (513, 200), (1030, 70)])
+def test_reconstruct_every_layout_and_colour_coding(ctx, rc, w, h):
+    rng = np.random.default_rng(w * 1000 + h)
+    cases = [([(1, 1)], b"\x01", -1)]
+    for fy in ((1, 1), (2, 1), (1, 2), (2, 2)):
+        cases += [([fy, (1, 1), (1, 1)], b"\x01\x02\x03", -1), ([fy, (1, 1), (1, 1)], b"RGB", -1),
+                  ([fy, (1, 1), (1, 1)], b"\x01\x02\x03", 0)]
+    cases += [([(2, 2), (1, 1), (2, 1)], b"\x01\x02\x03", -1), ([(1, 1), (2, 2), (2, 2)], b"\x01\x02\x03", -1)]
+    for factors, ids, adobe in cases:
+        dev = _random_coef_image(rng, w, h, factors, ids, adobe, rc)
+        ora = O.CoefImage(w, h, [O.CoefComponent(c.id, c.h, c.v, c.tq, c.blocks_w, c.coefs) for c in dev.comps],
+                          dev.qtables, adobe)
+        got = ctx.reconstruct(dev)
+        want = O.reconstruct(ora)
+        assert np.array_equal(got, want), (factors, ids, adobe, np.argwhere(got != want)[:4])
+
+
 def test_reconstruct_errors(ctx, rc):
     img = _smooth(32, 32, 3)
     ci = ctx.build_coefs(img, 80, rc.S420)
