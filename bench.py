@@ -435,11 +435,54 @@ def bench_dropin(rc, ctx, args):
     t1 = time.perf_counter()
     out["plain_search"] = {"quality": f.quality, "steps": f.steps, "ms": round((t1 - t0) * 1000, 2),
                            "search_mps": round(mp / (t1 - t0), 2)}
+    out["stages"] = bench_stages(rc, ctx, d_img.ptr, W, H, n)
     if not args.no_cpu:
         from oracle import oracle as O
         out["precision"] = precision_report(rc, ctx, O, img, cands, r0)
     d_img.free()
     return out
+
+
+def bench_stages(rc, ctx, d_src, W, H, n):
+    """The forward stage (build_coefs: RGB8 -> quantised coefficients, k_encode) and the reconstruct stage
+    (coefficients -> RGB8, k_reconstruct) alone, at quality 75, 4:2:0 and 4:4:4, on the device-resident drop-in
+    image: each kernel's average launch time by HIP events over n calls, against its algorithmic bytes (read RGB8 and
+    write the int16 coefficients; read the coefficients and write RGB8)."""
+    res = {}
+    for sub, name in ((rc.S420, "4:2:0"), (rc.S444, "4:4:4")):
+        L = rc.coef_layout(W, H, 3, 75, sub)
+        grids = [(int(L.blocks_w[c]), int(L.blocks_h[c])) for c in range(3)]
+        coef_bytes = sum(bw * bh * 64 * 2 for bw, bh in grids)
+        bufs = [rc.DeviceBuffer(ctx, bw * bh * 64 * 2) for bw, bh in grids]
+        ptrs = [b.ptr for b in bufs]
+        out = rc.DeviceBuffer(ctx, W * H * 3)
+        qt = rc.quant_tables(75)
+        comps = [(c + 1, 2 if (c == 0 and sub == rc.S420) else 1, 2 if (c == 0 and sub == rc.S420) else 1,
+                  0 if c == 0 else 1, grids[c][0], grids[c][1], ptrs[c]) for c in range(3)]
+        fwd = lambda: ctx.build_coefs_dev(d_src, W, H, 3, 75, sub, ptrs)
+        rec = lambda: ctx.reconstruct_dev(W, H, comps, [qt[0], qt[1], None, None], out.ptr)
+        entry = {}
+        for stage, call, kernel, alg in (("forward", fwd, "k_encode", 3 * W * H + coef_bytes),
+                                         ("reconstruct", rec, "k_reconstruct", coef_bytes + 3 * W * H)):
+            for _ in range(3):
+                call()
+            ctx.synchronize()
+            ctx.profile(True)
+            for _ in range(n):
+                call()
+            ctx.synchronize()
+            ms, k = ctx.profile_get(kernel)
+            ctx.profile(False)
+            us = ms * 1000.0 / max(1, k)
+            gbs = alg / (us * 1e-6) / 1e9
+            entry[stage] = {"kernel": kernel, "avg_launch_us": round(us, 2), "launches": k,
+                            "algorithmic_bytes": alg, "achieved_gbs": round(gbs, 1),
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "mp_per_s": round(W * H / us, 1)}
+        res[name] = entry
+        for b in bufs:
+            b.free()
+        out.free()
+    return res
 
 
 def precision_report(rc, ctx, O, img, cands, r0):

@@ -502,6 +502,23 @@ class Context:
         _check(lib().rcg_build_coefs_dev(self._h, d_pixels, width, height, channels, quality, subsampling,
                                          ctypes.cast(ptrs, ctypes.c_void_p)))
 
+    def reconstruct_dev(self, width, height, comps, qtables, d_pixels, adobe_transform=-1):
+        """rcg_reconstruct_dev: comps are (id, h, v, tq, blocks_w, blocks_h, device coefficient pointer) tuples,
+        qtables host natural-order tables by slot (or None); d_pixels receives Gray8 or RGB8 on the device."""
+        s = _CoefImage()
+        s.width, s.height, s.ncomp = width, height, len(comps)
+        keep = []
+        for i, (cid, h, v, tq, bw, bh, ptr) in enumerate(comps):
+            c = s.comps[i]
+            c.id, c.h, c.v, c.tq, c.blocks_w, c.blocks_h, c.coefs = cid, h, v, tq, bw, bh, ptr
+        for slot, q in enumerate(qtables):
+            if q is not None:
+                qa = np.ascontiguousarray(q, dtype=np.uint8)
+                keep.append(qa)
+                s.qtables[slot] = qa.ctypes.data
+        s.adobe_transform = adobe_transform
+        _check(lib().rcg_reconstruct_dev(self._h, ctypes.byref(s), d_pixels))
+
     def ssimulacra2_dev(self, d_ref, d_dist, width, height, channels, precision=FP32):
         score = ctypes.c_double()
         ns = ctypes.c_int()

@@ -471,7 +471,7 @@ int stage_and_launch_encode(rcg_ctx* ctx, Desc& D, const Layout& L, const uint8_
                             int16_t* const d_coefs[3], uint8_t* const d_rec[3], float* const d_scaled[3] = nullptr) {
     const StagedEncode se = stage_encode(D, L, d_src, quality, d_coefs, d_rec, d_scaled);
     RCG_TRY(D.upload(ctx->stream));
-    return launch_encode(ctx, se.job, se.begins, 1, se.tiles);
+    return launch_encode(ctx, se.job, se.begins, 1, se.tiles, !d_rec && !d_scaled);
 }
 
 size_t coef_bytes(const Layout& L) {

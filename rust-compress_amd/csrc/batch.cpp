@@ -832,7 +832,7 @@ int encode_winners(rcg_ctx* ctx, Arena& A, Pinned& pin, const Item* items, const
     const EncJob* d_ej = D.put(ej.data(), ej.size());
     const uint32_t* d_eb = D.put(begins.data(), begins.size());
     RCG_TRY(D.upload(ctx->stream));
-    RCG_TRY(launch_encode(ctx, d_ej, d_eb, n, tiles));
+    RCG_TRY(launch_encode(ctx, d_ej, d_eb, n, tiles, true));  // the winners' coefficients only
     if (host_dest)
         for (int k = 0; k < n; k++)
             for (uint32_t c = 0; c < lay[k].ncomp; c++)

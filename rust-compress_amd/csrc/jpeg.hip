@@ -58,12 +58,13 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // a native 16-byte
 constexpr int kRawChunks = (kEncTileW * 3 + 15 + 15) / 16;  // 16-byte chunks per staged source row (97)
 constexpr int kRawRow = kRawChunks * 16;                     // LDS bytes per staged source row
 constexpr int kRawLoads = (16 * kRawChunks + kEncThreads - 1) / kEncThreads;  // chunk loads per thread, 16 rows
-// LDS of one workgroup, carved per subsampling mode: the Y rows (rows x 512), the two chroma planes (2 x 8 x 512/hmax),
-// then the staged source rows (rows x kRawRow). 4:2:0 is the largest: 16 Y rows, 256-wide chroma, 16 source rows.
-// Its last 16 bytes lie beyond every carve-up: the discard slot of the source staging.
-constexpr int kEncLds = 16 * kEncTileW + 2 * 8 * (kEncTileW / 2) + 16 * kRawRow + 16;
-static_assert(kEncLds - 16 >= 8 * kEncTileW + 2 * 8 * kEncTileW + 8 * kRawRow, "4:4:4 carve-up must fit");
-static_assert(kEncLds + 64 <= 40960, "four encode workgroups per CU");
+// LDS of one workgroup: the staged source rows (rows x kRawRow) and, over the same bytes once the conversion has read
+// them into registers, the Y rows (rows x 512) and the two chroma planes (2 x 8 x 512/hmax). 16 staged rows (4:2:0)
+// are the largest part; the last 16 bytes lie beyond it: the discard slot of the source staging.
+constexpr int kEncLds = 16 * kRawRow + 16;
+static_assert(16 * kRawRow >= 16 * kEncTileW + 2 * 8 * (kEncTileW / 2), "4:2:0 Y rows and chroma over the staging");
+static_assert(16 * kRawRow >= 8 * kEncTileW + 2 * 8 * kEncTileW, "4:4:4 Y rows and chroma over the staging");
+static_assert(kEncLds + 64 <= 160 * 1024 / 6, "six encode workgroups per CU");
 
 // The 4 bytes at LDS byte offset o (any alignment), from the two dwords that hold them.
 __device__ __forceinline__ uint32_t lds_bytes4(const uint8_t* base, uint32_t o) {
@@ -71,27 +72,49 @@ __device__ __forceinline__ uint32_t lds_bytes4(const uint8_t* base, uint32_t o) 
     return __builtin_amdgcn_alignbyte(d[1], d[0], o & 3u);
 }
 
-// Gray8: the staged rows are the Y rows; 4 samples per item.
+// Gray8: the staged rows are the Y rows; 4 samples per item. Every item is read into registers before the barrier
+// after which the Y rows overwrite the staged rows (the same LDS bytes).
 __device__ __forceinline__ void convert_gray(const uint8_t* s_raw, const uint32_t* s_off, uint8_t* s_y, int tid) {
-    for (int it = tid; it < 8 * (kEncTileW / 4); it += kEncThreads) {
-        const int py = it / (kEncTileW / 4), q = it % (kEncTileW / 4);
-        *reinterpret_cast<uint32_t*>(s_y + py * kEncTileW + 4 * q) =
-            lds_bytes4(s_raw, uint32_t(py) * kRawRow + s_off[py] + 4 * uint32_t(q));
+    constexpr int NI = 8 * (kEncTileW / 4), NIT = (NI + kEncThreads - 1) / kEncThreads;
+    uint32_t v[NIT];
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+        const int it = tid + k * kEncThreads;
+        if (it < NI) {
+            const int py = it / (kEncTileW / 4), q = it % (kEncTileW / 4);
+            v[k] = lds_bytes4(s_raw, uint32_t(py) * kRawRow + s_off[py] + 4 * uint32_t(q));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+        const int it = tid + k * kEncThreads;
+        if (it < NI) {
+            const int py = it / (kEncTileW / 4), q = it % (kEncTileW / 4);
+            *reinterpret_cast<uint32_t*>(s_y + py * kEncTileW + 4 * q) = v[k];
+        }
     }
 }
 
 // RGB8 -> YCbCr in 16-bit fixed point (encode.rs:466-477) and the HM x VM box mean of Cb and Cr (encode.rs:478-499).
-// One item is 4 chroma samples: 4*HM pixels of each of VM rows, read as dwords and realigned with v_alignbyte; its Y
-// bytes and chroma samples leave as dword stores.
+// One item is 4 chroma samples: 4*HM pixels of each of VM rows, read as dwords and realigned with v_alignbyte. A
+// thread converts all its items into registers first; after a barrier its Y bytes and chroma samples leave as dword
+// stores over the staged rows (the same LDS bytes).
 template <int HM, int VM>
 __device__ __forceinline__ void convert_rgb(const uint8_t* s_raw, const uint32_t* s_off, uint8_t* s_y, uint8_t* s_c,
                                             int tid) {
     constexpr int CW = kEncTileW / HM;  // chroma samples per row
     constexpr int QPR = CW / 4;         // items per chroma row
+    constexpr int NI = 8 * QPR;         // items per strip
+    constexpr int NIT = (NI + kEncThreads - 1) / kEncThreads;
     constexpr int DW = 3 * HM;          // dwords of RGB per row of one item
     constexpr int NSH = (HM - 1) + (VM - 1);
     constexpr int HALF = (1 << NSH) >> 1;
-    for (int it = tid; it < 8 * QPR; it += kEncThreads) {
+    uint32_t ypk[NIT][VM][HM], cbp[NIT], crp[NIT];
+#pragma unroll
+    for (int n = 0; n < NIT; n++) {
+        const int it = tid + n * kEncThreads;
+        if (it >= NI) break;
         const int gy = it / QPR, q = it % QPR;
         int sb[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -104,7 +127,6 @@ __device__ __forceinline__ void convert_rgb(const uint8_t* s_raw, const uint32_t
             for (int k = 0; k <= DW; k++) wd[k] = d[k];
 #pragma unroll
             for (int k = 0; k < DW; k++) a[k] = __builtin_amdgcn_alignbyte(wd[k + 1], wd[k], o & 3u);
-            uint32_t ypk[HM];
 #pragma unroll
             for (int p = 0; p < 4 * HM; p++) {
                 const int32_t r = (a[(3 * p) / 4] >> (8 * ((3 * p) % 4))) & 0xFF;
@@ -113,26 +135,38 @@ __device__ __forceinline__ void convert_rgb(const uint8_t* s_raw, const uint32_t
                 const uint32_t y = uint32_t((19595 * r + 38470 * g + 7471 * b + 32768) >> 16) & 0xFF;
                 const int32_t cb = (-11059 * r - 21709 * g + 32768 * b + (128 << 16) + 32767) >> 16;
                 const int32_t cr = (32768 * r - 27439 * g - 5329 * b + (128 << 16) + 32767) >> 16;
-                ypk[p / 4] = (p % 4 == 0) ? y : (ypk[p / 4] | (y << (8 * (p % 4))));
+                ypk[n][dy][p / 4] = (p % 4 == 0) ? y : (ypk[n][dy][p / 4] | (y << (8 * (p % 4))));
                 sb[p / HM] += cb & 0xFF;
                 sr[p / HM] += cr & 0xFF;
             }
-#pragma unroll
-            for (int k = 0; k < HM; k++)
-                *reinterpret_cast<uint32_t*>(s_y + py * kEncTileW + q * (4 * HM) + 4 * k) = ypk[k];
         }
-        uint32_t cbp = 0, crp = 0;
+        cbp[n] = crp[n] = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             // The rounded mean of HM*VM samples (a power of two; the sums are non-negative).
-            cbp |= (uint32_t((sb[j] + HALF) >> NSH) & 0xFF) << (8 * j);
-            crp |= (uint32_t((sr[j] + HALF) >> NSH) & 0xFF) << (8 * j);
+            cbp[n] |= (uint32_t((sb[j] + HALF) >> NSH) & 0xFF) << (8 * j);
+            crp[n] |= (uint32_t((sr[j] + HALF) >> NSH) & 0xFF) << (8 * j);
         }
-        *reinterpret_cast<uint32_t*>(s_c + gy * CW + 4 * q) = cbp;
-        *reinterpret_cast<uint32_t*>(s_c + 8 * CW + gy * CW + 4 * q) = crp;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NIT; n++) {
+        const int it = tid + n * kEncThreads;
+        if (it >= NI) break;
+        const int gy = it / QPR, q = it % QPR;
+#pragma unroll
+        for (int dy = 0; dy < VM; dy++)
+#pragma unroll
+            for (int k = 0; k < HM; k++)
+                *reinterpret_cast<uint32_t*>(s_y + (gy * VM + dy) * kEncTileW + q * (4 * HM) + 4 * k) = ypk[n][dy][k];
+        *reinterpret_cast<uint32_t*>(s_c + gy * CW + 4 * q) = cbp[n];
+        *reinterpret_cast<uint32_t*>(s_c + 8 * CW + gy * CW + 4 * q) = crp[n];
     }
 }
 
+// kFull: candidates may ask for the trellis hand-off values and the reconstruction planes besides the coefficients;
+// without it the launch writes coefficients only, and the kernel holds far fewer registers.
+template <bool kFull>
 __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict__ jobs,
                                                          const uint32_t* __restrict__ tile_begins, int njobs) {
     __shared__ __align__(16) uint8_t s_mem[kEncLds];
@@ -150,9 +184,9 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
     const uint32_t w = J.w, h = J.h;
     const uint32_t n = min(uint32_t(kEncTileW), w - x0);  // real source pixels in this strip (x0 < w always)
     const int cw = kEncTileW / hmax;                       // chroma samples per row of this strip
+    uint8_t* const s_raw = s_mem;  // staged rows; later the Y rows and chroma planes over them
     uint8_t* const s_y = s_mem;
     uint8_t* const s_c = s_mem + rows * kEncTileW;
-    uint8_t* const s_raw = s_c + 2 * 8 * cw;
 
     // Phase A: stage the strip's source rows (bottom rows replicated) in LDS. Row r starts at byte s_off[r] of its
     // LDS row and is read as the aligned 16-byte chunks that cover it.
@@ -288,7 +322,7 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
                 dst[k8] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
             }
         }
-        float* sp = K.scaled[comp];
+        float* sp = kFull ? K.scaled[comp] : nullptr;
         if (sp) {
             // The trellis branch of quantize_plane (encode.rs:206-211): the same f32 products, unrounded.
             float4* dst = reinterpret_cast<float4*>(sp + blk_index * 64);
@@ -303,7 +337,7 @@ __global__ __launch_bounds__(kEncThreads) void k_encode(const EncJob* __restrict
                 dst[k] = make_float4(a[0], a[1], a[2], a[3]);
             }
         }
-        uint8_t* rp = K.rec[comp];
+        uint8_t* rp = kFull ? K.rec[comp] : nullptr;
         if (rp) {
             // Dequantise straight from the rounded values (decode.rs:1317-1322), then the IDCT.
             const auto* mult = K.mult[cls];
@@ -752,10 +786,16 @@ __global__ __launch_bounds__(kRcThreads) __attribute__((amdgpu_waves_per_eu(4)))
 // ----------------------------------------------------------------------------------------------------------------
 // Launchers.
 
-int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t total_tiles) {
+int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t total_tiles,
+                  bool coef_only) {
     if (total_tiles == 0) return RCG_OK;
     ProfScope ps(ctx, "k_encode");
-    hipLaunchKernelGGL(k_encode, dim3(total_tiles), dim3(kEncThreads), 0, ctx->stream, d_jobs, d_begins, njobs);
+    if (coef_only)
+        hipLaunchKernelGGL(k_encode<false>, dim3(total_tiles), dim3(kEncThreads), 0, ctx->stream, d_jobs, d_begins,
+                           njobs);
+    else
+        hipLaunchKernelGGL(k_encode<true>, dim3(total_tiles), dim3(kEncThreads), 0, ctx->stream, d_jobs, d_begins,
+                           njobs);
     RCG_HIP(hipGetLastError());
     return RCG_OK;
 }

@@ -163,7 +163,9 @@ struct ProfScope {
 };
 
 // Launchers (defined next to their kernels).
-int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t total_tiles);
+// coef_only: no candidate of the launch asks for reconstruction planes or trellis hand-off values.
+int launch_encode(rcg_ctx* ctx, const EncJob* d_jobs, const uint32_t* d_begins, int njobs, uint32_t total_tiles,
+                  bool coef_only = false);
 struct IdctPlane {
     const int16_t* coefs;
     uint8_t* plane;
