@@ -170,6 +170,8 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip the plain drop-in (one 4096x4096 image) figures")
     ap.add_argument("--dropin-only", action="store_true",
                     help="only the plain drop-in figures, printed as {\"dropin\": ...} (for profiling that leg alone)")
+    ap.add_argument("--stages-only", action="store_true",
+                    help="only the forward and reconstruct stages on the drop-in image, printed as {\"stages\": ...}")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
     ap.add_argument("--cli-files", type=int, default=24)
@@ -215,6 +217,15 @@ def main():
               "(images would shard across GPUs with no collective; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.stages_only:
+        rc = importlib.import_module("rust-compress_amd")
+        ctx = rc.Context(0)
+        d_img = ctx.upload(rc.synth_rgb8(2, args.size, args.size))
+        print(json.dumps({"stages": bench_stages(rc, ctx, d_img.ptr, args.size, args.size, max(10, args.steps))}),
+              flush=True)
+        d_img.free()
+        ctx.close()
+        return
     if args.dropin_only:
         rc = importlib.import_module("rust-compress_amd")
         ctx = rc.Context(0)

@@ -38,22 +38,23 @@ __device__ __forceinline__ int32_t upsample_at(const PlaneView& v, uint32_t x, u
     return p[uint64_t(y / v.fy) * s + x / v.fx];
 }
 
-// k * x for a constant k and x that both fit in 24 signed bits, as v_mul_i32_i24 (full rate). A 32-bit multiply is
+// k * x + c for a constant k and x that fit in 24 signed bits, as v_mad_i32_i24 (full rate). A 32-bit multiply is
 // quarter rate, and the compiler emits one wherever it cannot see that x is small (x computed in another basic
-// block: the SSIMULACRA2 loader's chroma comes out of a branch per sampling layout).
-__device__ __forceinline__ int32_t mul_i24(int32_t k, int32_t x) {
+// block: the SSIMULACRA2 loader's chroma comes out of a branch per sampling layout). k is read from an SGPR and c
+// from a VGPR (one scalar operand per instruction on this GPU).
+__device__ __forceinline__ int32_t mad_i24(int32_t k, int32_t x, int32_t c) {
     int32_t r;
-    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "s"(k), "v"(x));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "s"(k), "v"(x), "v"(c));
     return r;
 }
 
 // Fixed-point YCbCr -> RGB with clamping (decode.rs:1434-1441). Every factor and offset sample fits in 24 signed bits
-// and every product in 32, so 24-bit multiplies give the reference's products exactly.
+// and every product in 32, so 24-bit multiply-adds give the reference's sums exactly.
 __device__ __forceinline__ void ycc_to_rgb(int32_t a, int32_t b, int32_t c, int32_t& r, int32_t& g, int32_t& bl) {
     const int32_t cb = b - 128, cr = c - 128;
-    r = a + ((mul_i24(91881, cr) + 32768) >> 16);
-    g = a - ((mul_i24(22554, cb) + mul_i24(46802, cr) + 32768) >> 16);
-    bl = a + ((mul_i24(116130, cb) + 32768) >> 16);
+    r = a + (mad_i24(91881, cr, 32768) >> 16);
+    g = a - (mad_i24(46802, cr, mad_i24(22554, cb, 32768)) >> 16);
+    bl = a + (mad_i24(116130, cb, 32768) >> 16);
     r = r < 0 ? 0 : (r > 255 ? 255 : r);
     g = g < 0 ? 0 : (g > 255 ? 255 : g);
     bl = bl < 0 ? 0 : (bl > 255 ? 255 : bl);
