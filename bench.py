@@ -431,6 +431,20 @@ def bench_dropin(rc, ctx, args):
     out = {"workload": f"one {W}x{H} RGB8 image, q{{50,75,90}} x {{4:2:0,4:4:4}}, every candidate per step",
            "ms_per_step": round(ms, 3), "candidate_mps": round(mp * len(cands) / (ms / 1000.0), 2),
            "scores": [round(x, 4) for x in list(r0.scores)[:len(cands)]]}
+    # The same step captured once as a hipGraph and replayed (rcg_batch_plan): what a caller that scores the same
+    # device buffers repeatedly pays, without the per-call host staging and launches.
+    plan = ctx.batch_plan([(d_img.ptr, W, H, 3)], cands, TARGET_B)
+    for _ in range(3):
+        plan.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        g0 = plan.run()[0]
+    ctx.synchronize()
+    gms = (time.perf_counter() - t0) * 1000.0 / n
+    plan.close()
+    out["graph"] = {"ms_per_step": round(gms, 3), "candidate_mps": round(mp * len(cands) / (gms / 1000.0), 2),
+                    "scores_equal_eager": list(g0.scores)[:len(cands)] == list(r0.scores)[:len(cands)]}
     ctx.score_batch([img], cands, TARGET_B)
     t0 = time.perf_counter()
     for _ in range(n):

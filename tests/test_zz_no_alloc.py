@@ -1,7 +1,9 @@
 """The allocation-free entry point rcg_reconstruct_dev_noalloc, through the C ABI, against rcg_reconstruct_dev (the
 entry point it is a variant of) and the oracle: bit-exact on two calls in a row, the second with a smaller image
 (its planes overwrite part of the first call's in the shared scratch array), on gray, 4:2:0, 4:4:4 and YCCK images,
-and the refusal of an image whose sample planes exceed the array."""
+and the refusal of an image whose sample planes exceed the array. The allocation-free entry point always takes the
+two-kernel form (IDCT into the scratch planes, then the colour stage), where rcg_reconstruct_dev takes the fused
+kernel for the gray, 4:2:0 and 4:4:4 images: those comparisons check the two forms against each other too."""
 import ctypes
 import os
 
