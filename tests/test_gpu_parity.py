@@ -113,7 +113,7 @@ def _random_coef_image(rng, w, h, factors, ids, adobe, lib):
 @pytest.mark.parametrize("w,h", [(1, 1), (13, 7), (257, 65), (300, 131), (513, 200), (1030, 70)])
 def test_reconstruct_every_layout_and_colour_coding(ctx, rc, w, h):
     rng = np.random.default_rng(w * 1000 + h)
-    cases = [([(1, 1)], b"\x01", -1)]
+    cases = [([(1, 1)], b"\x01", -1), ([(2, 2)], b"\x01", -1)]  # a gray file may declare 2x2 sampling
     for fy in ((1, 1), (2, 1), (1, 2), (2, 2)):
         cases += [([fy, (1, 1), (1, 1)], b"\x01\x02\x03", -1), ([fy, (1, 1), (1, 1)], b"RGB", -1),
                   ([fy, (1, 1), (1, 1)], b"\x01\x02\x03", 0)]
