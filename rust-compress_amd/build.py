@@ -71,6 +71,10 @@ def build(verbose=False):
 #   tests/cpp/host_sanitize      the host reader, writer, requantisation and trellis on the reference's JPEG fixtures
 #                                and mutations of them (tests/test_host_sanitizers.py, no device needed);
 #   tests/cpp/server_race_asan   the serving queue with concurrent submitting threads (tests/test_gpu_server_race.py).
+# The HIP runtime finds no device in a ThreadSanitizer process, so the serving queue's race check under
+# ThreadSanitizer runs without one: tests/cpp/server_queue_tsan is the real csrc/server.cpp (and synth.cpp) with the
+# same driver, linked against tests/cpp/server_model.cpp, a host model of the device calls and the scoring pass
+# (tests/test_server_queue_tsan.py, no device needed).
 CLANG = os.environ.get("RCG_CLANGXX", "/opt/rocm/llvm/bin/clang++")
 HOST_SOURCES = [s for s in SOURCES if s.endswith(".cpp")]
 CHECKS = ["address", "undefined"]
@@ -79,24 +83,50 @@ DRIVERS = [(os.path.join(ROOT, "tests", "cpp", "host_sanitize.cpp"), os.path.joi
             os.path.join(ROOT, "tests", "cpp", "server_race_asan"))]
 
 
-def build_sanitized(kernel_objs, verbose=False):
+TSAN_SOURCES = [os.path.join(CSRC, "server.cpp"), os.path.join(CSRC, "synth.cpp"),
+                os.path.join(ROOT, "tests", "cpp", "server_model.cpp"), os.path.join(ROOT, "tests", "cpp", "server_race.cpp")]
+TSAN_BIN = os.path.join(ROOT, "tests", "cpp", "server_queue_tsan")
+
+
+def build_queue_model(verbose=False):
+    """The serving queue and its driver under ThreadSanitizer, over the host model of the device: host code only,
+    compiled as C++ against the HIP headers (no HIP runtime linked)."""
+    from concurrent.futures import ThreadPoolExecutor
+    d = os.path.join(OBJ_DIR, "tsan")
+    os.makedirs(d, exist_ok=True)
+    flags = [CLANG, "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=thread", "-fPIC",
+             "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wall", "-Wno-unused-result"]
+    objs = [os.path.join(d, os.path.splitext(os.path.basename(src))[0] + ".o") for src in TSAN_SOURCES]
+    with ThreadPoolExecutor(max_workers=len(objs)) as ex:
+        for f in [ex.submit(_run, flags + ["-c", src, "-o", o], verbose) for src, o in zip(TSAN_SOURCES, objs)]:
+            f.result()
+    _run([CLANG, "-fsanitize=thread", "-o", TSAN_BIN] + objs + ["-lpthread"], verbose)
+
+
+def _sanitized(kernel_objs, checks, drivers, subdir, verbose):
     from concurrent.futures import ThreadPoolExecutor
     kobjs = [o for o, s in zip(kernel_objs, SOURCES) if not s.endswith(".cpp")]
     flags = [f for f in FLAGS if f != "-O3"] + ["-O1", "-g", "-fno-omit-frame-pointer"]
-    for c in CHECKS:
+    for c in checks:
         flags += ["-Xarch_host", "-fsanitize=" + c]
-    flags += ["-Xarch_host", "-fno-sanitize-recover=undefined"]
-    d = os.path.join(OBJ_DIR, "asan")
+    if "undefined" in checks:
+        flags += ["-Xarch_host", "-fno-sanitize-recover=undefined"]
+    d = os.path.join(OBJ_DIR, subdir)
     os.makedirs(d, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES] + [drv for drv, _ in DRIVERS]
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES] + [drv for drv, _ in drivers]
     objs = [os.path.join(d, os.path.splitext(os.path.basename(src))[0] + ".o") for src in srcs]
     with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as ex:
         for f in [ex.submit(_run, [HIPCC] + flags + ["-c", src, "-o", o], verbose) for src, o in zip(srcs, objs)]:
             f.result()
     lib_objs = objs[:len(HOST_SOURCES)] + kobjs
-    for (drv, out), o in zip(DRIVERS, objs[len(HOST_SOURCES):]):
-        _run([CLANG, "-fsanitize=" + ",".join(CHECKS), "-o", out, o] + lib_objs +
+    for (drv, out), o in zip(drivers, objs[len(HOST_SOURCES):]):
+        _run([CLANG, "-fsanitize=" + ",".join(checks), "-o", out, o] + lib_objs +
              ["-L/opt/rocm/lib", "-lamdhip64", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"], verbose)
+
+
+def build_sanitized(kernel_objs, verbose=False):
+    _sanitized(kernel_objs, CHECKS, DRIVERS, "asan", verbose)
+    build_queue_model(verbose)
 
 
 if __name__ == "__main__":

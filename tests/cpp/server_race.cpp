@@ -35,10 +35,12 @@ static int finish(int rc) {
 
 int main() {
     constexpr int kThreads = 4, kPerThread = 12, kTotal = kThreads * kPerThread;
-    const uint32_t sizes[][2] = {{96, 80}, {128, 128}, {640, 480}, {64, 64}, {333, 250}, {1024, 768}};
+    // The last size is above the server's large-request threshold (2 MiB pixels): both lanes serve requests.
+    const uint32_t sizes[][2] = {{96, 80}, {128, 128}, {640, 480}, {64, 64}, {333, 250}, {1024, 768}, {1600, 1400}};
+    constexpr int kImages = 7;
     const double targets[] = {60.0, 70.0, 80.0};
     std::vector<Img> imgs;
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < kImages; i++) {
         Img m{sizes[i][0], sizes[i][1], {}};
         m.px.resize(size_t(m.w) * m.h * 3);
         if (rcg_synth_rgb8(100 + i, m.w, m.h, m.px.data()) != RCG_OK) return finish(2);
@@ -54,7 +56,7 @@ int main() {
     // What each request is: (image, target index).
     std::vector<std::pair<int, int>> what(kTotal);
     for (int t = 0; t < kThreads; t++)
-        for (int k = 0; k < kPerThread; k++) what[t * kPerThread + k] = {(t + k) % 6, k % 3};
+        for (int k = 0; k < kPerThread; k++) what[t * kPerThread + k] = {(t + k) % kImages, k % 3};
 
     std::atomic<int> submit_failures{0};
     std::atomic<bool> stop{false};
