@@ -816,8 +816,9 @@ void rcg::mult_table(const uint8_t q[64], float mult[64]) {
         for (int u = 0; u < 8; u++) mult[v * 8 + u] = float(q[v * 8 + u]) * kAan[u] * kAan[v];
 }
 
-// IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device. The sample
-// planes go to `planes` (one per component, blocks_w*8 x blocks_h*8 bytes each).
+// IDCT + upsample + colour of a coefficient image whose coefficient arrays are already on the device: the fused
+// kernel for the usual layouts (when fused_ok), else the IDCT into `planes_out` (one per component, blocks_w*8 x
+// blocks_h*8 bytes each) and the colour stage.
 int rcg::reconstruct_on_device(rcg_ctx* ctx, const rcg_coef_image* ci, const int16_t* const d_coefs[4],
                           uint8_t* const planes_out[4], uint8_t* d_out, uint32_t hmax, uint32_t vmax, int rgb_direct,
                           bool fused_ok) {
@@ -862,7 +863,9 @@ int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels) {
         int rgb_direct;
         RCG_TRY(check_coef_image(ci, hmax, vmax, rgb_direct));
         const size_t out_bytes = size_t(ci->width) * ci->height * out_channels(ci);
-        size_t need = Arena::align(out_bytes) + sample_plane_bytes(ci);
+        int fx, fy;
+        const bool fused = fused_reconstruct_layout(ci, hmax, vmax, &fx, &fy);  // no sample planes then
+        size_t need = Arena::align(out_bytes) + (fused ? 0 : sample_plane_bytes(ci));
         for (uint32_t c = 0; c < ci->ncomp; c++)
             need += Arena::align(size_t(ci->comps[c].blocks_w) * ci->comps[c].blocks_h * 64 * sizeof(int16_t));
         Call call(ctx);
@@ -874,7 +877,7 @@ int rcg_reconstruct(rcg_ctx* ctx, const rcg_coef_image* ci, uint8_t* pixels) {
             int16_t* d = static_cast<int16_t*>(ctx->arena.take(nb * 64 * sizeof(int16_t)));
             RCG_HIP(hipMemcpyAsync(d, ci->comps[c].coefs, nb * 64 * sizeof(int16_t), hipMemcpyHostToDevice, ctx->stream));
             d_co[c] = d;
-            planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
+            if (!fused) planes[c] = static_cast<uint8_t*>(ctx->arena.take(nb * 64));
         }
         uint8_t* d_out = static_cast<uint8_t*>(ctx->arena.take(out_bytes));
         RCG_TRY(reconstruct_on_device(ctx, ci, d_co, planes, d_out, hmax, vmax, rgb_direct));
@@ -898,10 +901,12 @@ int rcg_reconstruct_dev(rcg_ctx* ctx, const rcg_coef_image* d_ci, uint8_t* d_pix
             }
             d_co[c] = d_ci->comps[c].coefs;
         }
+        int fx, fy;
+        const bool fused = fused_reconstruct_layout(d_ci, hmax, vmax, &fx, &fy);  // no sample planes then
         Call call(ctx);
-        RCG_TRY(call.begin(sample_plane_bytes(d_ci)));
+        RCG_TRY(call.begin(fused ? 0 : sample_plane_bytes(d_ci)));
         uint8_t* planes[4] = {nullptr, nullptr, nullptr, nullptr};
-        for (uint32_t c = 0; c < d_ci->ncomp; c++)
+        for (uint32_t c = 0; !fused && c < d_ci->ncomp; c++)
             planes[c] = static_cast<uint8_t*>(ctx->arena.take(size_t(d_ci->comps[c].blocks_w) * d_ci->comps[c].blocks_h * 64));
         RCG_TRY(reconstruct_on_device(ctx, d_ci, d_co, planes, d_pixels, hmax, vmax, rgb_direct));
         RCG_HIP(hipStreamSynchronize(ctx->stream));
