@@ -42,6 +42,7 @@ REF_BYTES_PER_PX = 7.0
 CANDS_B = (40, 50, 60, 70, 80, 85, 90, 95)
 TARGET_B = 70.0
 SCALE0 = "k_ssim_scale<float, 256, 3u, 1u>"
+MI355X_XCDS, MI355X_SIMDS = 8, 256 * 4  # accelerator dies and SIMDs (4 per CU) of the MI355X
 
 
 def progress(msg):
@@ -75,9 +76,9 @@ def install_policy(rc, ctx, policy):
     return ctx.get_tune()
 
 
-def _pmc_run(counter, bench_args, outdir, name):
-    """One rocprofv3 counter pass over a child run of this script; returns [(kernel, dispatch, value)] in dispatch
-    order, or None if the pass failed."""
+def _pmc_run(counters, bench_args, outdir, name):
+    """One rocprofv3 pass of the counters named in `counters` (space-separated) over a child run of this script;
+    returns [(kernel, dispatch, {counter: value}, duration_ns)] in dispatch order, or None if the pass failed."""
     import shutil
     import sqlite3
     import subprocess
@@ -85,7 +86,7 @@ def _pmc_run(counter, bench_args, outdir, name):
     if prof is None:
         return None
     d = os.path.join(outdir, name)
-    cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "-d", d, "-o", name, "--",
+    cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc"] + counters.split() + ["-d", d, "-o", name, "--",
            sys.executable, os.path.abspath(__file__)] + bench_args
     r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=ROOT)
     dbs = []
@@ -95,10 +96,14 @@ def _pmc_run(counter, bench_args, outdir, name):
         progress(f"{counter} pass failed (exit {r.returncode})")
         return None
     con = sqlite3.connect(dbs[0])
-    rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
-                       "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, sum(value), max(\"end\" - start) "
+                       "from counters_collection group by dispatch_id, counter_name order by dispatch_id").fetchall()
     con.close()
-    return [(k.split("(")[0].replace("void rcg::", "").replace("rcg::", ""), i, v) for i, k, v in rows]
+    out = {}
+    for i, k, c, v, ns in rows:
+        e = out.setdefault(i, (k.split("(")[0].replace("void rcg::", "").replace("rcg::", ""), i, {}, ns))
+        e[2][c] = v
+    return [out[i] for i in sorted(out)] or None
 
 
 def tuned_policy(jobs):
@@ -130,10 +135,10 @@ def counter_passes(images, policy):
         run = _pmc_run(counter, child, out, counter.lower() + "_pass")
         if not cal or not run:
             return None, None
-        kib = max(v for _, _, v in cal)
+        kib = max(v[counter] for _, _, v, _ in cal)
         factor = (512 * 1024) / kib if kib > 0 else None
-        vals = [v for k, _, v in run if k.startswith("k_ssim_scale<float")]
-        s0 = [v for k, _, v in run if k.startswith(SCALE0)]
+        vals = [v[counter] for k, _, v, _ in run if k.startswith("k_ssim_scale<float")]
+        s0 = [v[counter] for k, _, v, _ in run if k.startswith(SCALE0)]
         if not vals or factor is None:
             return None, None
         res[counter] = {"kib_per_launch": sum(vals) / len(vals), "scale0_kib": sum(s0), "factor": factor,
@@ -147,14 +152,25 @@ def counter_passes(images, policy):
                "launches": f["launches"]}
     progress("counters: VALUBusy pass")
     valu = None
-    run = _pmc_run("VALUBusy", child, out, "valu_pass")
+    run = _pmc_run("VALUBusy SQ_INSTS_VALU GRBM_GUI_ACTIVE", child, out, "valu_pass")
     if run:
         def pick(prefix):
-            v = [x for k, _, x in run if k.startswith(prefix)]
+            v = [x["VALUBusy"] for k, _, x, _ in run if k.startswith(prefix) and "VALUBusy" in x]
             return round(v[0], 1) if v else None
         valu = {"k_ssim_scale_scale0_pct": pick(SCALE0), "k_encode_pct": pick("k_encode"),
                 "k_ssim_ref_scale0_pct": pick("k_ssim_ref<256, 0u>"),
                 "counter": "VALUBusy = 100*SQ_ACTIVE_INST_VALU/CU_NUM/GRBM_GUI_ACTIVE (rocprofv3), one eager pass"}
+        s0 = [(x, ns) for k, _, x, ns in run if k.startswith(SCALE0)]
+        if s0 and s0[0][1] > 0 and s0[0][0].get("GRBM_GUI_ACTIVE") and s0[0][0].get("SQ_INSTS_VALU"):
+            # GRBM_GUI_ACTIVE sums the XCDs' busy cycles: per XCD over the launch's duration it is the clock. At that
+            # clock every SIMD can issue one wave64 VALU instruction per 4 cycles; issue_frac is the share of that
+            # peak the launch's SQ_INSTS_VALU fills.
+            x, ns = s0[0]
+            cycles = x["GRBM_GUI_ACTIVE"] / MI355X_XCDS
+            valu["k_ssim_scale_scale0_clock_ghz"] = round(cycles / ns, 3)
+            valu["k_ssim_scale_scale0_issue_frac"] = round(x["SQ_INSTS_VALU"] * 4 / (MI355X_SIMDS * cycles), 3)
+            valu["issue_frac_def"] = ("SQ_INSTS_VALU*4/(1024 SIMDs * GRBM_GUI_ACTIVE/8 XCDs): share of one wave64 "
+                                      "VALU instruction per SIMD per 4 cycles at the launch's clock")
     return traffic, valu
 
 
