@@ -93,7 +93,7 @@ def _pmc_run(counters, bench_args, outdir, name):
     for root, _, files in os.walk(d):
         dbs += [os.path.join(root, f) for f in files if f.endswith("_results.db")]
     if not dbs:
-        progress(f"{counter} pass failed (exit {r.returncode})")
+        progress(f"{counters} pass failed (exit {r.returncode})")
         return None
     con = sqlite3.connect(dbs[0])
     rows = con.execute("select dispatch_id, kernel_name, counter_name, sum(value), max(\"end\" - start) "
