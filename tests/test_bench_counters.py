@@ -4,7 +4,6 @@ that writes a results database of rocprofv3's schema (table counters_collection)
 GRBM_GUI_ACTIVE over the dispatch's duration, the VALU issue fraction) are checked against figures worked out by
 hand here."""
 import os
-import sqlite3
 import stat
 import sys
 
